@@ -1,0 +1,407 @@
+// capi.hip - the C ABI of include/hdsr.h over one Engine (a batch of one frame).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/hdsr.h"
+#include "engine.h"
+
+using namespace hdsr;
+
+struct hdsr_ctx {
+  hdsr_plan plan{};
+  uint32_t flags = 0;
+  int device = 0;
+  Engine eng;
+  // scene
+  bool have_splats = false;
+  bool own_splats = false;
+  uint32_t n = 0, sh_degree = 0;
+  uint64_t normals_len = 0;
+  float *pos = nullptr, *scale = nullptr, *rot = nullptr, *opac = nullptr, *normals = nullptr, *sh = nullptr;
+  uint32_t *material = nullptr, *sorted = nullptr;
+  bool own_sorted = false;
+  // frame
+  bool have_frame = false;
+  FrameDesc frame{};
+  // outputs
+  float* rgba = nullptr;
+  float* depth = nullptr;
+  uint32_t* gbn = nullptr;
+  uint32_t* gbm = nullptr;
+  uint16_t* rgba16 = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool rendered = false;
+  uint32_t reruns = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(hdsr_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPX(c, expr)                                                              \
+  do {                                                                             \
+    const hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) return fail(c, HDSR_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void free_scene(hdsr_ctx* c) {
+  if (c->own_splats) {
+    void* p[] = {c->pos, c->scale, c->rot, c->opac, c->normals, c->sh, c->material};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+  if (c->own_sorted && c->sorted) (void)hipFree(c->sorted);
+  c->pos = c->scale = c->rot = c->opac = c->normals = c->sh = nullptr;
+  c->material = c->sorted = nullptr;
+  c->have_splats = c->own_splats = c->own_sorted = false;
+}
+
+template <typename T>
+int dev_copy(hdsr_ctx* c, T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return 0;
+  HIPX(c, hipMalloc((void**)dst, count * sizeof(T)));
+  HIPX(c, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+uint32_t sh_stride(uint32_t deg) { return 3u + 3u * ((deg + 1u) * (deg + 1u) - 1u) + 3u; }
+
+void fill_frame_scene(hdsr_ctx* c) {
+  FrameDesc& F = c->frame;
+  F.pos = c->pos;
+  F.logscale = c->scale;
+  F.rot = c->rot;
+  F.opacity = c->opac;
+  F.sorted = c->sorted;
+  F.material = c->material;
+  F.normals = c->normals;
+  F.normals_len = (uint32_t)std::min<uint64_t>(c->normals_len, 0xFFFFFFFFull);
+  F.shdata = c->sh;
+  F.n = c->n;
+  F.sh_degree = c->sh_degree;
+  F.sh_stride = sh_stride(c->sh_degree);
+  F.width = c->plan.viewport_width;
+  F.height = c->plan.viewport_height;
+  F.rgba = c->rgba;
+  F.out_depth = c->depth;
+  F.gb_normal = c->gbn;
+  F.gb_material = c->gbm;
+  F.rgba16 = c->rgba16;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hdsr_version(void) { return "hdsr-mi355x 0.1 (gfx950)"; }
+
+const char* hdsr_last_error(hdsr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out) {
+  if (!in || !out) return HDSR_E_INVALID_ARG;
+  if (in->tile_size_px == 0 || in->viewport_width == 0 || in->viewport_height == 0) return HDSR_E_INVALID_ARG;
+  hdsr_plan p{};
+  p.viewport_width = in->viewport_width;
+  p.viewport_height = in->viewport_height;
+  p.tile_size_px = in->tile_size_px;
+  p.tile_columns = (in->viewport_width + in->tile_size_px - 1) / in->tile_size_px;
+  p.tile_rows = (in->viewport_height + in->tile_size_px - 1) / in->tile_size_px;
+  p.tile_count = p.tile_columns * p.tile_rows;
+  p.morton_tile_count = hdsr_morton2(p.tile_columns - 1, p.tile_rows - 1) + 1;
+  p.splat_count = in->splat_count;
+  const double mult = in->tile_entry_multiplier > 0.0f ? (double)in->tile_entry_multiplier : 16.0;
+  const double want = std::ceil((double)in->splat_count * mult);
+  p.max_total_tile_refs = (uint32_t)std::min(want, 65535.0 * 256.0);
+  *out = p;
+  return HDSR_OK;
+}
+
+int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
+  if (!plan || !out) return HDSR_E_INVALID_ARG;
+  *out = nullptr;
+  if (flags & (HDSR_F16_COMPOSITE | HDSR_REF_CAPS)) return HDSR_E_UNSUPPORTED;
+  if (plan->tile_size_px != HDSR_TILE_PX) return HDSR_E_INVALID_ARG;
+  if (plan->tile_columns == 0 || plan->tile_rows == 0 || plan->tile_columns > 255 || plan->tile_rows > 255)
+    return HDSR_E_INVALID_ARG;  // u8 tile-box packing (gpu_project_splats.wgsl:229-231)
+  std::unique_ptr<hdsr_ctx> c(new hdsr_ctx());
+  c->plan = *plan;
+  c->flags = flags;
+  if (hipGetDevice(&c->device) != hipSuccess) return HDSR_E_HIP;
+  const size_t px = (size_t)plan->viewport_width * plan->viewport_height;
+  if (hipMalloc((void**)&c->rgba, px * 16) != hipSuccess) return HDSR_E_OOM;
+  if (hipMalloc((void**)&c->depth, px * 4) != hipSuccess) return HDSR_E_OOM;
+  if (flags & HDSR_GBUFFER) {
+    if (hipMalloc((void**)&c->gbn, px * 4) != hipSuccess) return HDSR_E_OOM;
+    if (hipMalloc((void**)&c->gbm, px * 16) != hipSuccess) return HDSR_E_OOM;
+  }
+  if (flags & HDSR_OUT_RGBA16) {
+    if (hipMalloc((void**)&c->rgba16, px * 8) != hipSuccess) return HDSR_E_OOM;
+  }
+  const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
+  c->eng.set_min_capacity(base_cap);
+  *out = c.release();
+  return HDSR_OK;
+}
+
+void hdsr_destroy(hdsr_ctx* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  free_scene(c);
+  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete c;
+}
+
+int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
+  if (!c || !s) return HDSR_E_INVALID_ARG;
+  if (s->count != c->plan.splat_count) return fail(c, HDSR_E_INVALID_ARG, "splat count differs from the plan");
+  if (s->sh_degree > 3) return fail(c, HDSR_E_INVALID_ARG, "sh_degree must be 0..3");
+  if (s->count && (!s->positions || !s->scales || !s->rotations || !s->opacities || !s->materials || !s->sh_data ||
+                   !s->sorted_indices))
+    return fail(c, HDSR_E_INVALID_ARG, "missing splat buffer");
+  if (s->normals_len && !s->normals) return fail(c, HDSR_E_INVALID_ARG, "normals_len without normals");
+  free_scene(c);
+  const size_t n = s->count;
+  c->n = s->count;
+  c->sh_degree = s->sh_degree;
+  c->normals_len = s->normals_len;
+  if (s->memory == HDSR_MEM_DEVICE) {
+    c->pos = const_cast<float*>(s->positions);
+    c->scale = const_cast<float*>(s->scales);
+    c->rot = const_cast<float*>(s->rotations);
+    c->opac = const_cast<float*>(s->opacities);
+    c->material = const_cast<uint32_t*>(s->materials);
+    c->normals = const_cast<float*>(s->normals);
+    c->sh = const_cast<float*>(s->sh_data);
+    c->sorted = const_cast<uint32_t*>(s->sorted_indices);
+    c->own_splats = false;
+    c->own_sorted = false;
+  } else if (s->memory == HDSR_MEM_HOST) {
+    c->own_splats = true;
+    c->own_sorted = true;
+    int rc;
+    if ((rc = dev_copy(c, &c->pos, s->positions, n * 3))) return rc;
+    if ((rc = dev_copy(c, &c->scale, s->scales, n * 3))) return rc;
+    if ((rc = dev_copy(c, &c->rot, s->rotations, n * 4))) return rc;
+    if ((rc = dev_copy(c, &c->opac, s->opacities, n))) return rc;
+    if ((rc = dev_copy(c, &c->material, s->materials, n))) return rc;
+    if ((rc = dev_copy(c, &c->normals, s->normals, (size_t)s->normals_len))) return rc;
+    if ((rc = dev_copy(c, &c->sh, s->sh_data, n * sh_stride(s->sh_degree)))) return rc;
+    if ((rc = dev_copy(c, &c->sorted, s->sorted_indices, n))) return rc;
+  } else {
+    return fail(c, HDSR_E_INVALID_ARG, "memory must be HDSR_MEM_HOST or HDSR_MEM_DEVICE");
+  }
+  c->have_splats = true;
+  return HDSR_OK;
+}
+
+int hdsr_update_sorted_indices(hdsr_ctx* c, const uint32_t* sorted, uint32_t memory) {
+  if (!c || (!sorted && c->n)) return HDSR_E_INVALID_ARG;
+  if (!c->have_splats) return fail(c, HDSR_E_STATE, "bind splats first");
+  if (memory == HDSR_MEM_DEVICE) {
+    if (c->own_sorted && c->sorted) (void)hipFree(c->sorted);
+    c->sorted = const_cast<uint32_t*>(sorted);
+    c->own_sorted = false;
+    return HDSR_OK;
+  }
+  if (!c->own_sorted) {
+    c->sorted = nullptr;
+    if (c->n) HIPX(c, hipMalloc((void**)&c->sorted, (size_t)c->n * 4));
+    c->own_sorted = true;
+  }
+  if (c->n) {
+    HIPX(c, hipDeviceSynchronize());
+    HIPX(c, hipMemcpy(c->sorted, sorted, (size_t)c->n * 4, hipMemcpyHostToDevice));
+  }
+  return HDSR_OK;
+}
+
+int hdsr_set_frame(hdsr_ctx* c, const void* u) {
+  if (!c || !u) return HDSR_E_INVALID_ARG;
+  FrameDesc f{};
+  parse_uniforms(u, &f);
+  const uint32_t* w = static_cast<const uint32_t*>(u);
+  if (f.tiles_x != c->plan.tile_columns || f.tiles_y != c->plan.tile_rows)
+    return fail(c, HDSR_E_INVALID_ARG, "uniform tile grid differs from the plan");
+  if (w[22] != c->plan.splat_count) return fail(c, HDSR_E_INVALID_ARG, "uniform splatCount differs from the plan");
+  if (w[25] > 3) return fail(c, HDSR_E_INVALID_ARG, "uniform shDegree must be 0..3");
+  c->frame = f;
+  c->have_frame = true;
+  return HDSR_OK;
+}
+
+int hdsr_render(hdsr_ctx* c, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  if (!c->have_splats || !c->have_frame) return fail(c, HDSR_E_STATE, "bind splats and set a frame first");
+  fill_frame_scene(c);
+  const int rc = c->eng.run(&c->frame, 1, static_cast<hipStream_t>(stream), (c->flags & HDSR_GBUFFER) != 0,
+                            (c->flags & HDSR_DEBUG) != 0);
+  if (rc) return fail(c, rc, c->eng.err);
+  c->last_stream = static_cast<hipStream_t>(stream);
+  c->rendered = true;
+  return HDSR_OK;
+}
+
+int hdsr_composite_only(hdsr_ctx* c, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
+  fill_frame_scene(c);
+  const int rc = c->eng.composite_only(&c->frame, 1, static_cast<hipStream_t>(stream), (c->flags & HDSR_GBUFFER) != 0);
+  if (rc) return fail(c, rc, c->eng.err);
+  return HDSR_OK;
+}
+
+int hdsr_sync(hdsr_ctx* c) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  if (!c->rendered) return HDSR_OK;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    int rc = c->eng.wait();
+    if (rc) return fail(c, rc, c->eng.err);
+    HIPX(c, hipStreamSynchronize(c->last_stream));
+    if (!c->eng.overflowed()) return HDSR_OK;
+    c->eng.grow_for_overflow();
+    c->reruns++;
+    rc = c->eng.run(&c->frame, 1, c->last_stream, (c->flags & HDSR_GBUFFER) != 0, (c->flags & HDSR_DEBUG) != 0);
+    if (rc) return fail(c, rc, c->eng.err);
+  }
+  return fail(c, HDSR_E_CAPACITY, "pair capacity still overflowing after regrowth");
+}
+
+int hdsr_get_outputs(hdsr_ctx* c, hdsr_outputs* o) {
+  if (!c || !o) return HDSR_E_INVALID_ARG;
+  o->rgba = c->rgba;
+  o->depth = c->depth;
+  o->gbuffer_normal = c->gbn;
+  o->gbuffer_material = c->gbm;
+  o->rgba16 = c->rgba16;
+  o->width = c->plan.viewport_width;
+  o->height = c->plan.viewport_height;
+  return HDSR_OK;
+}
+
+int hdsr_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
+  if (!c || !host) return HDSR_E_INVALID_ARG;
+  int rc = hdsr_sync(c);
+  if (rc) return rc;
+  const size_t px = (size_t)c->plan.viewport_width * c->plan.viewport_height;
+  const void* src = nullptr;
+  size_t need = 0;
+  switch (which) {
+    case HDSR_OUT_RGBA: src = c->rgba; need = px * 16; break;
+    case HDSR_OUT_DEPTH: src = c->depth; need = px * 4; break;
+    case HDSR_OUT_GB_NORMAL: src = c->gbn; need = px * 4; break;
+    case HDSR_OUT_GB_MATERIAL: src = c->gbm; need = px * 16; break;
+    case HDSR_OUT_RGBA16_BUF: src = c->rgba16; need = px * 8; break;
+    default: return fail(c, HDSR_E_INVALID_ARG, "unknown output");
+  }
+  if (!src) return fail(c, HDSR_E_STATE, "output not enabled by the create flags");
+  if (bytes < need) return fail(c, HDSR_E_INVALID_ARG, "host buffer too small");
+  HIPX(c, hipMemcpy(host, src, need, hipMemcpyDeviceToHost));
+  return HDSR_OK;
+}
+
+int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
+  if (!c || !o) return HDSR_E_INVALID_ARG;
+  int rc = hdsr_sync(c);
+  if (rc) return rc;
+  memset(o, 0, sizeof(*o));
+  const DevStats& s = c->eng.host_stats();
+  o->refs = s.refs;
+  o->small_pairs = s.small_pairs;
+  o->large_pairs = s.large_pairs;
+  o->large_splats = s.large_splats;
+  o->visible_splats = s.visible;
+  o->small_tiles = s.small_tiles;
+  o->large_tiles = s.large_tiles;
+  o->chunks = s.chunks;
+  o->segments = s.segments;
+  o->pair_capacity = c->eng.bufs().cap;
+  o->overflow_reruns = c->reruns;
+  o->cap_refs_tripped = s.refs > c->plan.max_total_tile_refs;
+  o->cap_large_splats_tripped = s.large_splats > 65535u;
+  o->cap_chunks_tripped = s.chunks > 1024u;
+  return HDSR_OK;
+}
+
+int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {
+  if (!c || !bytes) return HDSR_E_INVALID_ARG;
+  int rc = hdsr_sync(c);
+  if (rc) return rc;
+  if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
+  const DevStats& s = c->eng.host_stats();
+  const uint64_t M = c->plan.morton_tile_count;
+  switch (which) {
+    case HDSR_DBG_PROJCACHE: *bytes = (uint64_t)c->n * 56; break;
+    case HDSR_DBG_DEPTH: *bytes = (uint64_t)c->n * 4; break;
+    case HDSR_DBG_TILE_COUNTS:
+    case HDSR_DBG_TILE_OFFSETS: *bytes = M * 4; break;
+    case HDSR_DBG_ENTRIES: *bytes = (uint64_t)s.refs * 4; break;
+    case HDSR_DBG_ENTRIES_CANON:
+    case HDSR_DBG_KEYS20:
+      if (!(c->flags & HDSR_DEBUG)) return fail(c, HDSR_E_STATE, "create the context with HDSR_DEBUG");
+      *bytes = (uint64_t)s.refs * 4;
+      break;
+    case HDSR_DBG_SMALL_LIST: *bytes = (uint64_t)s.small_tiles * 4; break;
+    case HDSR_DBG_LARGE_LIST:
+    case HDSR_DBG_LARGE_BASES: *bytes = (uint64_t)s.large_tiles * 4; break;
+    case HDSR_DBG_CHUNKS: *bytes = (uint64_t)s.chunks * 8; break;
+    case HDSR_DBG_LARGE_SPLATS: *bytes = (uint64_t)s.large_splats * 4; break;
+    default: return fail(c, HDSR_E_INVALID_ARG, "unknown debug buffer");
+  }
+  return HDSR_OK;
+}
+
+int hdsr_debug_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
+  uint64_t need = 0;
+  int rc = hdsr_debug_size(c, which, &need);
+  if (rc) return rc;
+  if (!host && need) return HDSR_E_INVALID_ARG;
+  if (bytes < need) return fail(c, HDSR_E_INVALID_ARG, "host buffer too small");
+  if (need == 0) return HDSR_OK;
+  const Bufs& b = c->eng.bufs();
+  const DevStats& s = c->eng.host_stats();
+  switch (which) {
+    case HDSR_DBG_PROJCACHE: {
+      uint32_t* tmp = nullptr;
+      HIPX(c, hipMalloc((void**)&tmp, need));
+      launch_cache14(b, 0, c->n, tmp, nullptr);
+      const hipError_t e = hipMemcpy(host, tmp, need, hipMemcpyDeviceToHost);
+      (void)hipFree(tmp);
+      HIPX(c, e);
+      return HDSR_OK;
+    }
+    case HDSR_DBG_DEPTH: HIPX(c, hipMemcpy(host, b.depth, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_TILE_COUNTS: HIPX(c, hipMemcpy(host, b.tile_counts, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_TILE_OFFSETS: HIPX(c, hipMemcpy(host, b.tile_offsets, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_ENTRIES: HIPX(c, hipMemcpy(host, b.entries, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_ENTRIES_CANON: HIPX(c, hipMemcpy(host, b.entries_canon, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_KEYS20: HIPX(c, hipMemcpy(host, b.keys20, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_SMALL_LIST: HIPX(c, hipMemcpy(host, b.small_list, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_LARGE_LIST: HIPX(c, hipMemcpy(host, b.large_list_t, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_LARGE_BASES: HIPX(c, hipMemcpy(host, b.large_bases, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_LARGE_SPLATS: HIPX(c, hipMemcpy(host, b.large_list, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_CHUNKS: {
+      std::vector<Seg> segs(s.segments);
+      HIPX(c, hipMemcpy(segs.data(), b.segs, segs.size() * sizeof(Seg), hipMemcpyDeviceToHost));
+      uint32_t* o = static_cast<uint32_t*>(host);
+      size_t k = 0;
+      for (const Seg& g : segs)
+        if (g.large != 0xFFFFFFFFu) {
+          o[k++] = g.dst;
+          o[k++] = g.count;
+        }
+      return HDSR_OK;
+    }
+    default: return fail(c, HDSR_E_INVALID_ARG, "unknown debug buffer");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// engine.h - host-side owner of one batch's device buffers and the stage sequence.
+//
+// An Engine renders a batch of frames (FrameDesc, one per view) with one launch per stage.  It
+// keeps its buffers between calls and only grows them.  The pair capacity is checked on the
+// device: a batch whose tile references exceed it is flagged in DevStats and must be run again
+// after grow_for_overflow() (the plain context does that in hdsr_sync).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hdsr.h"
+#include "hdsr_internal.h"
+
+namespace hdsr {
+
+struct Layout {
+  uint32_t num_frames = 0;
+  uint32_t splat_blocks = 0;
+  uint32_t total_tiles = 0;
+  uint32_t total_morton = 0;
+  uint32_t key_bits = 0;
+  std::vector<uint32_t> block_start;  // [V+1]
+  std::vector<uint32_t> tile_start;   // [V+1]
+};
+
+class Engine {
+ public:
+  Engine() = default;
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Fills the batch-layout fields of frames[0..V) and computes the layout.
+  static Layout layout(FrameDesc* frames, uint32_t V);
+
+  // Enqueues the full path for 'frames' on 'stream'.  Returns 0 or an HDSR_E_* code.
+  int run(const FrameDesc* frames, uint32_t V, hipStream_t stream, bool gbuffer, bool debug);
+  // Re-runs only the rasterizer on the previous run's tile lists (same frames, new outputs ok).
+  int composite_only(const FrameDesc* frames, uint32_t V, hipStream_t stream, bool gbuffer);
+
+  // Host copy of the device counters of the last run (valid after wait()).
+  const DevStats& host_stats() const { return *hstats_; }
+  int wait();
+  bool overflowed() const { return hstats_ && hstats_->overflow != 0; }
+  // Grows the pair capacity to hold the last run's references (with headroom).
+  int grow_for_overflow();
+  void set_min_capacity(uint64_t cap) { min_cap_ = cap; }
+
+  const Bufs& bufs() const { return b_; }
+  const Layout& last_layout() const { return lay_; }
+  bool debug_buffers() const { return b_.keys20 != nullptr; }
+  std::string err;
+
+ private:
+  int ensure(const Layout& L, bool debug);
+  int upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t stream);
+  void release();
+  template <typename T>
+  int grow(T** p, size_t* have, size_t need);
+
+  Bufs b_{};
+  Layout lay_;
+  uint64_t min_cap_ = 1u << 20;
+  uint64_t want_cap_ = 0;
+  // allocation sizes (elements)
+  size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;
+  size_t n_morton_ = 0, n_tiles_ = 0, n_seg_ = 0, n_frames_ = 0, n_debug_ = 0;
+  FrameDesc* d_frames_ = nullptr;
+  uint32_t* d_starts_ = nullptr;
+  // pinned staging ring for the frame table
+  static constexpr int kRing = 4;
+  void* stage_[kRing] = {};
+  size_t stage_bytes_ = 0;
+  hipEvent_t stage_ev_[kRing] = {};
+  int stage_next_ = 0;
+  DevStats* hstats_ = nullptr;
+  hipEvent_t done_ev_ = nullptr;
+  bool pending_ = false;
+};
+
+// Parses the 256-byte uniform block into a FrameDesc's camera fields
+// (layout of gpu_project_splats.wgsl:27-44).
+void parse_uniforms(const void* p, FrameDesc* f);
+
+}  // namespace hdsr

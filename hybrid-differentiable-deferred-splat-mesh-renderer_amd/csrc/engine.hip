@@ -1,0 +1,345 @@
+// engine.hip - batch layout, buffer growth and the stage sequence of the compositor.
+//
+// Stage order per batch (reference pass order: gpuTileSplatCompositor.ts:1023-1179):
+//   clear stats -> project (+count, +large classify) -> scan of block counts -> emit pairs
+//   -> radix grouping on the Morton key -> tile ranges -> classify -> bucket cascade
+//   -> segment sort (small tiles + chunks) -> composite
+// The reference's buffer clears and indirect-argument passes have no counterpart: counts come
+// from scans, and launches that depend on device-side totals use grid-stride loops.
+#include "engine.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace hdsr {
+
+static inline uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+void parse_uniforms(const void* p, FrameDesc* F) {
+  const float* f = static_cast<const float*>(p);
+  const uint32_t* u = static_cast<const uint32_t*>(p);
+  memcpy(F->vp, f, 64);
+  F->vw = f[16];
+  F->vh = f[17];
+  F->tile = f[18];
+  F->tiles_x = u[20];
+  F->tiles_y = u[21];
+  F->splat_scale = f[24];
+  F->cam[0] = f[28];
+  F->cam[1] = f[29];
+  F->cam[2] = f[30];
+  memcpy(F->view, f + 32, 64);
+  F->fx = f[48];
+  F->fy = f[49];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) F->nm[c * 3 + r] = f[52 + c * 4 + r];
+}
+
+Layout Engine::layout(FrameDesc* frames, uint32_t V) {
+  Layout L;
+  L.num_frames = V;
+  L.block_start.resize(V + 1);
+  L.tile_start.resize(V + 1);
+  uint32_t sb = 0, mb = 0, tb = 0;
+  for (uint32_t v = 0; v < V; ++v) {
+    FrameDesc& F = frames[v];
+    F.splat_base = sb;
+    F.morton_base = mb;
+    F.tile_base = tb;
+    F.morton_count = (F.tiles_x && F.tiles_y) ? hdsr_morton2(F.tiles_x - 1, F.tiles_y - 1) + 1 : 0;
+    L.block_start[v] = sb / kSplatBlock;
+    L.tile_start[v] = tb;
+    sb += align_up(std::max(F.n, 1u), kSplatBlock);
+    mb += F.morton_count;
+    tb += F.tiles_x * F.tiles_y;
+  }
+  L.block_start[V] = sb / kSplatBlock;
+  L.tile_start[V] = tb;
+  L.splat_blocks = sb / kSplatBlock;
+  L.total_tiles = tb;
+  L.total_morton = mb;
+  uint32_t bits = 0;
+  while (bits < 32 && (1ull << bits) < (uint64_t)mb) ++bits;
+  L.key_bits = bits;
+  return L;
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+  void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
+                  b_.pairs_a, b_.pairs_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch,
+                  b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
+                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  b_ = Bufs{};
+  d_frames_ = nullptr;
+  d_starts_ = nullptr;
+  for (int i = 0; i < kRing; ++i) {
+    if (stage_[i]) (void)hipHostFree(stage_[i]);
+    if (stage_ev_[i]) (void)hipEventDestroy(stage_ev_[i]);
+    stage_[i] = nullptr;
+    stage_ev_[i] = nullptr;
+  }
+  if (hstats_) (void)hipHostFree(hstats_);
+  hstats_ = nullptr;
+  if (done_ev_) (void)hipEventDestroy(done_ev_);
+  done_ev_ = nullptr;
+  n_splat_ = n_blocks_ = n_scan4_ = n_cap_ = n_hist_ = n_hist_tmp_ = 0;
+  n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = 0;
+}
+
+template <typename T>
+int Engine::grow(T** p, size_t* have, size_t need) {
+  if (need <= *have && *p) return 0;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  const size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+  if (hipMalloc((void**)p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    err = "device allocation of " + std::to_string(bytes) + " bytes failed";
+    return HDSR_E_OOM;
+  }
+  *have = need;
+  return 0;
+}
+
+int Engine::ensure(const Layout& L, bool debug) {
+  int rc = 0;
+  const size_t ns = (size_t)L.splat_blocks * kSplatBlock;
+  size_t have;
+  // per-splat arrays (sized together)
+  if (ns > n_splat_ || !b_.hot) {
+    have = n_splat_;
+    if ((rc = grow(&b_.hot, &have, ns * 2))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.cold, &have, ns * 2))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.bounds, &have, ns))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.depth, &have, ns))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.large_list, &have, ns))) return rc;
+    n_splat_ = ns;
+  }
+  if (L.splat_blocks > n_blocks_ || !b_.block_cnt) {
+    have = n_blocks_;
+    if ((rc = grow(&b_.block_cnt, &have, L.splat_blocks))) return rc;
+    have = n_blocks_;
+    if ((rc = grow(&b_.block_pre, &have, L.splat_blocks))) return rc;
+    n_blocks_ = L.splat_blocks;
+  }
+  const size_t scan4 = scan_u4_tmp_words(L.splat_blocks) + 2 * (size_t)L.total_tiles + scan_u4_tmp_words(L.total_tiles);
+  if ((rc = grow(&b_.scan_tmp, &n_scan4_, scan4))) return rc;
+  // pair capacity
+  uint64_t cap = std::max<uint64_t>({min_cap_, want_cap_, 1});
+  cap = std::min<uint64_t>(cap, 0xFFFFFFF0ull);
+  if (cap > n_cap_ || !b_.pairs_a) {
+    have = n_cap_;
+    if ((rc = grow(&b_.pairs_a, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.pairs_b, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.entries, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.tkeys, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.scratch, &have, cap))) return rc;
+    n_cap_ = cap;
+    n_debug_ = 0;  // debug arrays follow the capacity
+  }
+  b_.cap = (uint32_t)n_cap_;
+  if (debug) {
+    if (n_debug_ < n_cap_ || !b_.keys20) {
+      have = n_debug_;
+      if ((rc = grow(&b_.keys20, &have, n_cap_))) return rc;
+      have = n_debug_;
+      if ((rc = grow(&b_.entries_canon, &have, n_cap_))) return rc;
+      n_debug_ = n_cap_;
+    }
+  }
+  const size_t rblocks = (n_cap_ + kRadixBlockItems - 1) / kRadixBlockItems;
+  if ((rc = grow(&b_.radix_hist, &n_hist_, rblocks * kRadixBuckets))) return rc;
+  if ((rc = grow(&b_.radix_tmp, &n_hist_tmp_, scan_u32_tmp_words((uint32_t)(rblocks * kRadixBuckets))))) return rc;
+  if (L.total_morton > n_morton_ || !b_.tile_offsets) {
+    have = n_morton_;
+    if ((rc = grow(&b_.tile_offsets, &have, L.total_morton))) return rc;
+    have = n_morton_;
+    if ((rc = grow(&b_.tile_counts, &have, L.total_morton))) return rc;
+    n_morton_ = L.total_morton;
+  }
+  if (L.total_tiles > n_tiles_ || !b_.small_list) {
+    have = n_tiles_;
+    if ((rc = grow(&b_.small_list, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.large_list_t, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.large_bases, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.large_tiles, &have, L.total_tiles))) return rc;
+    n_tiles_ = L.total_tiles;
+  }
+  const size_t segs = (size_t)L.total_tiles + rblocks + 1;
+  if ((rc = grow(&b_.segs, &n_seg_, segs))) return rc;
+  b_.seg_cap = (uint32_t)n_seg_;
+  if (!b_.stats) {
+    size_t h = 0;
+    if ((rc = grow(&b_.stats, &h, 1))) return rc;
+  }
+  if (L.num_frames > n_frames_ || !d_frames_) {
+    have = n_frames_;
+    if ((rc = grow(&d_frames_, &have, L.num_frames))) return rc;
+    have = n_frames_;
+    if ((rc = grow(&d_starts_, &have, 2 * (L.num_frames + 1)))) return rc;
+    n_frames_ = L.num_frames;
+  }
+  if (!hstats_) {
+    if (hipHostMalloc((void**)&hstats_, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      err = "pinned host allocation failed";
+      return HDSR_E_OOM;
+    }
+    memset(hstats_, 0, sizeof(DevStats));
+  }
+  if (!done_ev_ && hipEventCreateWithFlags(&done_ev_, hipEventDisableTiming) != hipSuccess) {
+    err = "hipEventCreate failed";
+    return HDSR_E_HIP;
+  }
+  const size_t sb = sizeof(FrameDesc) * L.num_frames + 8 * (L.num_frames + 1);
+  if (sb > stage_bytes_) {
+    for (int i = 0; i < kRing; ++i) {
+      if (stage_ev_[i]) (void)hipEventSynchronize(stage_ev_[i]);
+      if (stage_[i]) (void)hipHostFree(stage_[i]);
+      stage_[i] = nullptr;
+      if (hipHostMalloc(&stage_[i], sb, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        err = "pinned host allocation failed";
+        return HDSR_E_OOM;
+      }
+      if (!stage_ev_[i] && hipEventCreateWithFlags(&stage_ev_[i], hipEventDisableTiming) != hipSuccess) {
+        err = "hipEventCreate failed";
+        return HDSR_E_HIP;
+      }
+    }
+    stage_bytes_ = sb;
+  }
+  b_.frames = d_frames_;
+  b_.block_frame_start = d_starts_;
+  b_.tile_frame_start = d_starts_ + (L.num_frames + 1);
+  b_.num_frames = L.num_frames;
+  b_.splat_blocks = L.splat_blocks;
+  b_.total_tiles = L.total_tiles;
+  b_.total_morton = L.total_morton;
+  b_.key_bits = L.key_bits;
+  if (!debug) {
+    // keep the buffers, but the kernels only write them when asked
+  }
+  return 0;
+}
+
+int Engine::upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t stream) {
+  const int slot = stage_next_;
+  stage_next_ = (stage_next_ + 1) % kRing;
+  if (hipEventSynchronize(stage_ev_[slot]) != hipSuccess) {
+    err = "hipEventSynchronize failed";
+    return HDSR_E_HIP;
+  }
+  char* s = static_cast<char*>(stage_[slot]);
+  const size_t fb = sizeof(FrameDesc) * L.num_frames;
+  memcpy(s, frames, fb);
+  uint32_t* st = reinterpret_cast<uint32_t*>(s + fb);
+  for (uint32_t v = 0; v <= L.num_frames; ++v) {
+    st[v] = L.block_start[v];
+    st[L.num_frames + 1 + v] = L.tile_start[v];
+  }
+  if (hipMemcpyAsync(d_frames_, s, fb, hipMemcpyHostToDevice, stream) != hipSuccess ||
+      hipMemcpyAsync(d_starts_, st, 8 * (L.num_frames + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
+      hipEventRecord(stage_ev_[slot], stream) != hipSuccess) {
+    err = "frame table upload failed";
+    return HDSR_E_HIP;
+  }
+  return 0;
+}
+
+int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool gbuffer, bool debug) {
+  if (V == 0 || V > kMaxFrames) {
+    err = "frame count out of range";
+    return HDSR_E_INVALID_ARG;
+  }
+  std::vector<FrameDesc> frames(frames_in, frames_in + V);
+  Layout L = layout(frames.data(), V);
+  int rc = ensure(L, debug);
+  if (rc) return rc;
+  lay_ = L;
+  if ((rc = upload_frames(frames.data(), L, stream))) return rc;
+  Bufs b = b_;
+  if (!debug) {
+    b.keys20 = nullptr;
+    b.entries_canon = nullptr;
+  }
+  launch_clear_stats(b, stream);
+  launch_project(b, stream);
+  launch_scan_blocks(b, stream);
+  launch_emit_pairs(b, stream);
+  launch_group_pairs(b, stream);
+  launch_tile_ranges(b, stream);
+  launch_classify(b, stream);
+  if (debug && b.entries_canon)
+    (void)hipMemcpyAsync(b.entries_canon, b.entries, (size_t)b.cap * 4, hipMemcpyDeviceToDevice, stream);
+  launch_bucket(b, stream);
+  launch_segment_sort(b, stream);
+  launch_composite(b, stream, gbuffer);
+  if (hipMemcpyAsync(hstats_, b.stats, sizeof(DevStats), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipEventRecord(done_ev_, stream) != hipSuccess) {
+    err = "stats copy failed";
+    return HDSR_E_HIP;
+  }
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    err = std::string("kernel launch failed: ") + hipGetErrorString(le);
+    return HDSR_E_HIP;
+  }
+  pending_ = true;
+  return 0;
+}
+
+int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool gbuffer) {
+  if (!pending_ && lay_.num_frames == 0) {
+    err = "composite_only before any render";
+    return HDSR_E_STATE;
+  }
+  if (V != lay_.num_frames) {
+    err = "composite_only needs the frames of the last render";
+    return HDSR_E_STATE;
+  }
+  std::vector<FrameDesc> frames(frames_in, frames_in + V);
+  Layout L = layout(frames.data(), V);
+  int rc = upload_frames(frames.data(), L, stream);
+  if (rc) return rc;
+  launch_composite(b_, stream, gbuffer);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    err = std::string("kernel launch failed: ") + hipGetErrorString(le);
+    return HDSR_E_HIP;
+  }
+  return 0;
+}
+
+int Engine::wait() {
+  if (!done_ev_) return 0;
+  if (hipEventSynchronize(done_ev_) != hipSuccess) {
+    err = std::string("device error: ") + hipGetErrorString(hipGetLastError());
+    return HDSR_E_HIP;
+  }
+  return 0;
+}
+
+int Engine::grow_for_overflow() {
+  const uint64_t need = (uint64_t)hstats_->small_pairs + hstats_->large_pairs;
+  want_cap_ = need + need / 4 + 4096;
+  return 0;
+}
+
+}  // namespace hdsr

@@ -1,0 +1,198 @@
+// hdsr_internal.h - device data layout and kernel launchers of the splat compositor.
+//
+// One launch of each stage processes a batch of V frames (V = 1 for the plain drop-in; up to
+// 64 in the serving form).  Every per-splat, per-pair and per-tile array is one flat array over
+// the batch; a frame owns a contiguous range of it:
+//   splats  g in [splat_base, splat_base + n)          (splat_base a multiple of 256)
+//   Morton  m in [morton_base, morton_base + morton_count)
+//   tiles   t in [tile_base, tile_base + tiles_x * tiles_y)   (row-major inside the frame)
+// A tile pair's key is morton_base + morton(tx, ty), so one stable radix sort over the batch
+// groups every frame's tiles at once and keeps each frame's canonical per-tile order.
+//
+// Projected splat records (the reference's 14-word cache, gpu_project_splats.wgsl:2-16, split
+// by who reads it):
+//   hot[g]  32 B  {px, py, k0, k1 | k2, depth, f16 rg, f16 b | f16 opacity}  - composite reads
+//   cold[g] 32 B  {w5 material, w6 radius|opacity, w8 normal, w13 conf | w11, w12, 0, 0}
+//   bounds[g]     w7 tile box (u8 x 4), HDSR_SENTINEL if invisible           - count / emit
+//   depth[g]      depthNdc bits                                              - per-tile keys
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hdsr_fmath.h"
+
+namespace hdsr {
+
+constexpr uint32_t kSplatBlock = 256;        // splats per project/emit workgroup
+constexpr uint32_t kRadixBits = 8;           // digit width of one radix pass
+constexpr uint32_t kRadixBuckets = 1u << kRadixBits;
+constexpr uint32_t kRadixBlockItems = 4096;  // pairs per radix workgroup (256 threads x 16)
+constexpr uint32_t kScanChunk = 4096;        // elements per workgroup of the generic scan
+constexpr uint32_t kMaxFrames = 64;
+
+struct FrameDesc {
+  // scene (device pointers)
+  const float* pos;
+  const float* logscale;
+  const float* rot;
+  const float* opacity;
+  const uint32_t* sorted;
+  const uint32_t* material;
+  const float* normals;
+  const float* shdata;
+  uint32_t normals_len;
+  uint32_t n;
+  uint32_t sh_degree;
+  uint32_t sh_stride;  // floats per splat in shdata: 3 + 3K + 3
+  // camera (parsed from the 256-byte uniform block)
+  float vp[16];
+  float view[16];
+  float nm[9];
+  float cam[3];
+  float vw, vh, tile, splat_scale, fx, fy;
+  uint32_t tiles_x, tiles_y;
+  // batch layout
+  uint32_t splat_base, morton_base, morton_count, tile_base;
+  uint32_t width, height;
+  // outputs (device pointers)
+  float* rgba;           // width*height*4
+  float* out_depth;      // width*height
+  uint32_t* gb_normal;   // width*height, or null
+  uint32_t* gb_material; // width*height*4, or null
+  uint16_t* rgba16;      // width*height*4 binary16, or null
+};
+
+// Counters of one batch, written by the kernels (device) and copied to pinned host memory.
+struct DevStats {
+  uint32_t small_pairs;   // pairs from splats covering <= 16 tiles
+  uint32_t large_pairs;   // pairs from splats covering > 16 tiles
+  uint32_t large_splats;
+  uint32_t visible;
+  uint32_t refs;          // small_pairs + large_pairs
+  uint32_t overflow;      // 1 if refs exceeded the pair capacity (frame must be re-run)
+  uint32_t small_tiles;
+  uint32_t large_tiles;
+  uint32_t segments;      // small tiles with >= 1 entry + chunks of large tiles
+  uint32_t chunks;
+  uint32_t scratch_used;
+  uint32_t pad[5];
+};
+
+struct Seg {
+  uint32_t src;    // first entry, in entries[] (small tile) or scratch[] (chunk)
+  uint32_t dst;    // first entry in entries[]
+  uint32_t count;  // <= 4096
+  uint32_t large;  // index into the large-tile table, or 0xFFFFFFFF for a small tile
+};
+
+struct LargeTile {
+  uint32_t morton;   // global Morton slot
+  uint32_t start;    // first entry in entries[]
+  uint32_t count;
+  uint32_t scratch;  // first slot in scratch[]
+  uint32_t dmin, dmax, pad0, pad1;
+};
+
+// Device buffers of one batch (all owned by the host context).
+struct Bufs {
+  const FrameDesc* frames;   // [V]
+  const uint32_t* block_frame_start;  // [V+1] first splat block of each frame (host-built)
+  const uint32_t* tile_frame_start;   // [V+1] first tile of each frame
+  uint32_t num_frames;
+  uint32_t splat_blocks;     // sum of padded splats / 256
+  uint32_t total_tiles;      // sum of tiles
+  uint32_t total_morton;     // sum of Morton spans
+  uint32_t key_bits;         // bits needed for a global Morton key
+  uint4* hot;                // [splat_blocks*256*2]
+  uint4* cold;               // [splat_blocks*256*2]
+  uint32_t* bounds;          // [splat_blocks*256]
+  uint32_t* depth;           // [splat_blocks*256]
+  uint4* block_cnt;          // [splat_blocks] per block (small pairs, large pairs, large splats, visible)
+  uint4* block_pre;          // [splat_blocks] exclusive prefix of block_cnt
+  uint4* scan_tmp;           // scratch of the uint4 scan
+  uint32_t* large_list;      // [splat_blocks*256] sort ranks (global g) of large splats
+  unsigned long long* pairs_a;  // [cap] (key << 32) | g
+  unsigned long long* pairs_b;  // [cap]
+  uint32_t* entries;         // [cap] final per-tile order (global g)
+  uint32_t* tkeys;           // [cap] Morton key of each entry after grouping
+  uint32_t* keys20;          // [cap] (q20 << 12) | local index, or null
+  uint32_t* entries_canon;   // [cap] entries before the per-tile sort, or null
+  uint32_t* scratch;         // [cap] bucket-ordered copies of large tiles
+  uint32_t cap;              // pair capacity
+  uint32_t* radix_hist;      // [256 * radix blocks] digit-major
+  uint32_t* radix_tmp;       // scratch of the u32 scan
+  uint32_t* tile_offsets;    // [total_morton]
+  uint32_t* tile_counts;     // [total_morton]
+  uint32_t* small_list;      // [total_tiles] global Morton slots
+  uint32_t* large_list_t;    // [total_tiles] global Morton slots
+  uint32_t* large_bases;     // [total_tiles]
+  LargeTile* large_tiles;    // [total_tiles]
+  Seg* segs;                 // [seg capacity]
+  uint32_t seg_cap;
+  DevStats* stats;
+};
+
+// Launchers (one per pipeline stage); every one enqueues on 'stream' and returns.
+void launch_clear_stats(const Bufs& b, hipStream_t stream);
+void launch_project(const Bufs& b, hipStream_t stream);
+void launch_scan_blocks(const Bufs& b, hipStream_t stream);
+void launch_emit_pairs(const Bufs& b, hipStream_t stream);
+void launch_group_pairs(const Bufs& b, hipStream_t stream);   // radix passes on the Morton key
+void launch_tile_ranges(const Bufs& b, hipStream_t stream);
+void launch_classify(const Bufs& b, hipStream_t stream);
+void launch_bucket(const Bufs& b, hipStream_t stream);
+void launch_segment_sort(const Bufs& b, hipStream_t stream);
+void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer);
+// Rebuilds the reference's 14-word projection cache of frame f into dst (debug readback).
+void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream);
+
+// Generic exclusive scans used by several stages (hierarchical: reduce, scan of partials, downsweep).
+size_t scan_u32_tmp_words(uint32_t n);
+void scan_u32_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t stream);
+size_t scan_u4_tmp_words(uint32_t n);
+void scan_u4_exclusive(const uint4* in, uint4* out, uint32_t n, uint4* tmp, hipStream_t stream);
+
+// Wave64 helpers ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+
+// Which frame owns item 'x' given the ascending starts table [V+1] (uniform per workgroup).
+__device__ __forceinline__ uint32_t frame_of(const uint32_t* starts, uint32_t nframes, uint32_t x) {
+  uint32_t lo = 0, hi = nframes;  // starts[lo] <= x < starts[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (starts[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace hdsr
+
+#define HIP_CHECK_LAUNCH() (void)hipGetLastError()

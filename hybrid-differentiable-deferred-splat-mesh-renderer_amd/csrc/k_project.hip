@@ -1,0 +1,337 @@
+// k_project.hip - projection cache + tile count + large-splat classification, one pass.
+//
+// Reference: project_splats (src/shaders/gpu_project_splats.wgsl:112-358),
+// classify_large_splats (gpu_classify_large_splats.wgsl:33-56) and the per-splat footprint of
+// count_tile_refs (gpu_tile_splat_composite.wgsl:64-84).
+//
+// One thread per sort rank; 256 threads per workgroup; a workgroup never straddles two frames.
+// The splat is gathered through sortedIndices (the only uncoalesced reads); the projected record
+// is written as two coalesced 32-byte records plus the tile box and the depth bits.  Instead of
+// atomically counting tiles, each workgroup reduces (small-splat pairs, large-splat pairs, large
+// splats, visible splats) into one uint4: the scan of those block sums gives every splat's pair
+// offset in canonical order with no atomics.
+//
+// Arithmetic is IEEE fp32 with no contraction, in the operation order of the WGSL, so the
+// record is bit-identical to the CPU restatement.
+#include "hdsr_internal.h"
+
+namespace hdsr {
+namespace {
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 mul(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 smul(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ F3 vdiv(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float vlen(F3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ F3 normalize(F3 a) { return vdiv(a, vlen(a)); }
+
+__device__ __forceinline__ void mat4_point(const float* m, F3 p, float out[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) out[i] = ((m[i] * p.x + m[4 + i] * p.y) + m[8 + i] * p.z) + m[12 + i] * 1.0f;
+}
+
+// rotateAxis (gpu_project_splats.wgsl:75-79), q already divided by max(|q|, 1e-6)
+__device__ __forceinline__ F3 rotate_axis(const float q[4], F3 axis) {
+  const F3 u = f3(q[1], q[2], q[3]);
+  const F3 inner = add(cross(u, axis), smul(q[0], axis));
+  return add(axis, smul(2.0f, cross(u, inner)));
+}
+
+__device__ __forceinline__ void oct_encode(F3 n, float* ox, float* oy) {
+  const float sum = (fabsf(n.x) + fabsf(n.y)) + fabsf(n.z);
+  float px = n.x / sum, py = n.y / sum;
+  if (n.z < 0.0f) {
+    const float sx = px >= 0.0f ? 1.0f : -1.0f;
+    const float sy = py >= 0.0f ? 1.0f : -1.0f;
+    const float nx = (1.0f - fabsf(py)) * sx;
+    const float ny = (1.0f - fabsf(px)) * sy;
+    px = nx;
+    py = ny;
+  }
+  *ox = px;
+  *oy = py;
+}
+
+#define SH_C1 0.4886025119029199f
+#define SH_C2_0 1.0925484305920792f
+#define SH_C2_1 1.0925484305920792f
+#define SH_C2_2 0.31539156525252005f
+#define SH_C2_3 1.0925484305920792f
+#define SH_C2_4 0.5462742152960396f
+#define SH_C3_0 0.5900435899266435f
+#define SH_C3_1 2.890611442640554f
+#define SH_C3_2 0.4570457994644658f
+#define SH_C3_3 0.3731763325901154f
+#define SH_C3_4 0.4570457994644658f
+#define SH_C3_5 1.445305721320277f
+#define SH_C3_6 0.5900435899266435f
+
+__global__ __launch_bounds__(256) void k_project(Bufs b) {
+  const uint32_t blk = blockIdx.x;
+  const uint32_t f = frame_of(b.block_frame_start, b.num_frames, blk);
+  const FrameDesc& F = b.frames[f];
+  const uint32_t g = blk * kSplatBlock + threadIdx.x;
+  const uint32_t r = g - F.splat_base;
+
+  uint32_t n_small = 0, n_large = 0, is_large = 0, vis = 0;
+  uint32_t bounds = HDSR_SENTINEL, depth_bits = HDSR_SENTINEL;
+  uint4 h0 = make_uint4(0, 0, 0, 0), h1 = make_uint4(0, 0, 0, 0);
+  uint4 c0 = make_uint4(0, 0, 0, 0), c1 = make_uint4(0, 0, 0, 0);
+
+  const uint32_t id = (r < F.n) ? F.sorted[r] : 0xFFFFFFFFu;
+  if (id < F.n) {
+    const F3 center = f3(F.pos[id * 3], F.pos[id * 3 + 1], F.pos[id * 3 + 2]);
+    float clip[4];
+    mat4_point(F.vp, center, clip);
+    if (clip[3] > 0.0f) {
+      const float safe_w = fmaxf(clip[3], HDSR_MIN_CLIP_W);
+      const float ndc_x = clip[0] / safe_w, ndc_y = clip[1] / safe_w;
+      const float px = (ndc_x * 0.5f + 0.5f) * F.vw;
+      const float py = (1.0f - (ndc_y * 0.5f + 0.5f)) * F.vh;
+      const float depth = clip[2] / safe_w;
+
+      const float4 q4 = *reinterpret_cast<const float4*>(F.rot + (size_t)id * 4);
+      const float qlen = sqrtf(((q4.x * q4.x + q4.y * q4.y) + q4.z * q4.z) + q4.w * q4.w);
+      const float qd = fmaxf(qlen, 0.000001f);
+      const float q[4] = {q4.x / qd, q4.y / qd, q4.z / qd, q4.w / qd};
+      const float sx = hdsr_expf(F.logscale[id * 3]) * F.splat_scale;
+      const float sy = hdsr_expf(F.logscale[id * 3 + 1]) * F.splat_scale;
+      const float sz = hdsr_expf(F.logscale[id * 3 + 2]) * F.splat_scale;
+      const F3 a0 = mul(rotate_axis(q, f3(1.0f, 0.0f, 0.0f)), sx);
+      const F3 a1 = mul(rotate_axis(q, f3(0.0f, 1.0f, 0.0f)), sy);
+      const F3 a2 = mul(rotate_axis(q, f3(0.0f, 0.0f, 1.0f)), sz);
+      const float len0 = dot(a0, a0), len1 = dot(a1, a1), len2 = dot(a2, a2);
+      const float min_len = fminf(len0, fminf(len1, len2));
+      const F3 normal_axis = (len0 == min_len) ? a0 : ((len1 == min_len) ? a1 : a2);
+
+      const float* V = F.view;
+      const F3 w0 = f3(V[0], V[4], V[8]);
+      const F3 w1 = f3(V[1], V[5], V[9]);
+      const F3 w2 = f3(V[2], V[6], V[10]);
+      float vc[4];
+      mat4_point(V, center, vc);
+      const float vz = fminf(vc[2], -0.001f);
+      const float j1x = F.fx / vz, j1y = F.fy / vz;
+      const float j2x = (-j1x / vz) * vc[0];
+      const float j2y = (-j1y / vz) * vc[1];
+      const F3 tt0 = add(smul(j1x, w0), smul(j2x, w2));
+      const F3 tt1 = add(smul(j1y, w1), smul(j2y, w2));
+      const F3 b0 = f3(dot(a0, tt0), dot(a1, tt0), dot(a2, tt0));
+      const F3 b1 = f3(dot(a0, tt1), dot(a1, tt1), dot(a2, tt1));
+      const float cxx_raw = dot(b0, b0), cxy = dot(b0, b1), cyy_raw = dot(b1, b1);
+      const float det_orig = cxx_raw * cyy_raw - cxy * cxy;
+      const float cxx = cxx_raw + HDSR_LOW_PASS;
+      const float cyy = cyy_raw + HDSR_LOW_PASS;
+      const float det_blur = cxx * cyy - cxy * cxy;
+      const float aa = sqrtf(fmaxf(det_orig / fmaxf(det_blur, 0.000001f), 0.0f));
+      const float mid = 0.5f * (cxx + cyy);
+      const float det = fmaxf(det_blur, 0.1f);
+      const float lambda = mid + sqrtf(fmaxf(mid * mid - det, 0.1f));
+      const float radius = ceilf(HDSR_SIGMA_RADIUS * sqrtf(lambda));
+      if (!(radius <= 0.0f)) {
+        const float det_inv = 1.0f / fmaxf(det_blur, 0.000001f);
+        const float k0 = cyy * det_inv, k1 = -cxy * det_inv, k2 = cxx * det_inv;
+        const float opac = fminf(fmaxf(F.opacity[id] * aa, 0.0f), 0.99f);
+
+        // tile box (gpu_project_splats.wgsl:210-218,229-232)
+        const float ts = fmaxf(F.tile, 1.0f);
+        const uint32_t mtx = max(F.tiles_x, 1u) - 1u, mty = max(F.tiles_y, 1u) - 1u;
+        const float rc = fminf(radius, ts * 8.0f);
+        const float minx = fmaxf(px - rc, 0.0f), miny = fmaxf(py - rc, 0.0f);
+        const float maxx = fminf(px + rc, F.vw), maxy = fminf(py + rc, F.vh);
+        const uint32_t t0x = min(hdsr_u32sat(floorf(minx / ts)), mtx);
+        const uint32_t t0y = min(hdsr_u32sat(floorf(miny / ts)), mty);
+        const uint32_t t1x = min(hdsr_u32sat(floorf(fmaxf((maxx - HDSR_FOOTPRINT_EPS) / ts, 0.0f))), mtx);
+        const uint32_t t1y = min(hdsr_u32sat(floorf(fmaxf((maxy - HDSR_FOOTPRINT_EPS) / ts, 0.0f))), mty);
+        bounds = (t0x & 0xFFu) | ((t0y & 0xFFu) << 8) | ((t1x & 0xFFu) << 16) | ((t1y & 0xFFu) << 24);
+        const uint32_t bx0 = bounds & 0xFFu, by0 = (bounds >> 8) & 0xFFu;
+        const uint32_t bx1 = (bounds >> 16) & 0xFFu, by1 = bounds >> 24;
+        const uint32_t span = (bx1 - bx0 + 1u) * (by1 - by0 + 1u);  // u32 wrap as the WGSL
+        const uint32_t ntiles = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1u) * (by1 - by0 + 1u) : 0u;
+        vis = 1;
+        if (span > HDSR_LARGE_SPLAT_TILES) {
+          is_large = 1;
+          n_large = ntiles;
+        } else {
+          n_small = ntiles;
+        }
+        depth_bits = hdsr_f2u(depth);
+
+        // normals (gpu_project_splats.wgsl:234-291)
+        F3 nrm = normalize(normal_axis);
+        const uint32_t nb = id * 3u;
+        if (nb + 2u < F.normals_len) {
+          const F3 baked = f3(F.normals[nb], F.normals[nb + 1], F.normals[nb + 2]);
+          const float bl = vlen(baked);
+          if (bl > 0.001f) nrm = vdiv(baked, bl);
+        }
+        const uint32_t db = F.n * 3u + id * 3u;
+        if (db + 2u < F.normals_len) {
+          const F3 detail = f3(F.normals[db], F.normals[db + 1], F.normals[db + 2]);
+          const float dl = vlen(detail);
+          if (dl > 0.001f) {
+            F3 tc;
+            if (len0 >= len1 && len0 >= len2) tc = a0;
+            else if (len1 >= len2) tc = a1;
+            else tc = a2;
+            const F3 tp = sub(tc, mul(nrm, dot(tc, nrm)));
+            const float tl = vlen(tp);
+            if (tl > 0.001f) {
+              const F3 T = vdiv(tp, tl);
+              const F3 B = cross(nrm, T);
+              const F3 dn = vdiv(detail, dl);
+              nrm = normalize(add(add(mul(T, dn.x), mul(B, dn.y)), mul(nrm, dn.z)));
+            }
+          }
+        }
+        const F3 cam = f3(F.cam[0], F.cam[1], F.cam[2]);
+        const F3 view_dir = normalize(sub(cam, center));
+        const F3 nn = normalize(nrm);
+        const F3 vv = normalize(view_dir);
+        const F3 faced = dot(nn, vv) < 0.0f ? f3(-nn.x, -nn.y, -nn.z) : nn;
+        const float conf = fminf(fmaxf(dot(faced, view_dir), 0.0f), 1.0f);
+        const float* M = F.nm;
+        F3 wn;
+        wn.x = (M[0] * faced.x + M[3] * faced.y) + M[6] * faced.z;
+        wn.y = (M[1] * faced.x + M[4] * faced.y) + M[7] * faced.z;
+        wn.z = (M[2] * faced.x + M[5] * faced.y) + M[8] * faced.z;
+        float ox, oy;
+        oct_encode(normalize(wn), &ox, &oy);
+
+        // spherical harmonics (gpu_project_splats.wgsl:293-351)
+        const uint32_t deg = F.sh_degree;
+        const float* sh = F.shdata + (size_t)id * F.sh_stride;
+        float c[3] = {sh[0], sh[1], sh[2]};
+        if (deg >= 1u) {
+          const float x = view_dir.x, y = view_dir.y, z = view_dir.z;
+          const float* s1 = sh + 3;
+#pragma unroll
+          for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + SH_C1 * ((s1[ch] * y + s1[3 + ch] * z) + s1[6 + ch] * x);
+          if (deg >= 2u) {
+            const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            const float b2[5] = {SH_C2_0 * xy, SH_C2_1 * yz, SH_C2_2 * ((2.0f * zz - xx) - yy), SH_C2_3 * xz,
+                                 SH_C2_4 * (xx - yy)};
+            const float* s2 = s1 + 9;
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+              float acc = s2[ch] * b2[0];
+#pragma unroll
+              for (int j = 1; j < 5; ++j) acc = acc + s2[3 * j + ch] * b2[j];
+              c[ch] = c[ch] + acc;
+            }
+            if (deg >= 3u) {
+              const float b3[7] = {(SH_C3_0 * y) * (3.0f * xx - yy),
+                                   ((SH_C3_1 * x) * y) * z,
+                                   (SH_C3_2 * y) * ((4.0f * zz - xx) - yy),
+                                   (SH_C3_3 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy),
+                                   (SH_C3_4 * x) * ((4.0f * zz - xx) - yy),
+                                   (SH_C3_5 * z) * (xx - yy),
+                                   (SH_C3_6 * x) * (xx - 3.0f * yy)};
+              const float* s3 = s2 + 15;
+#pragma unroll
+              for (int ch = 0; ch < 3; ++ch) {
+                float acc = s3[ch] * b3[0];
+#pragma unroll
+                for (int j = 1; j < 7; ++j) acc = acc + s3[3 * j + ch] * b3[j];
+                c[ch] = c[ch] + acc;
+              }
+            }
+          }
+        }
+        const float cr = fminf(fmaxf(c[0], 0.0f), 1.0f);
+        const float cg = fminf(fmaxf(c[1], 0.0f), 1.0f);
+        const float cb = fminf(fmaxf(c[2], 0.0f), 1.0f);
+        const float* em = sh + (F.sh_stride - 3u);
+
+        const uint32_t op16 = hdsr_f16_bits(opac);
+        h0 = make_uint4(hdsr_f2u(px), hdsr_f2u(py), hdsr_f2u(k0), hdsr_f2u(k1));
+        h1 = make_uint4(hdsr_f2u(k2), depth_bits, hdsr_pack2x16(cr, cg), hdsr_f16_bits(cb) | (op16 << 16));
+        c0 = make_uint4(F.material[id], hdsr_f16_bits(radius) | (op16 << 16), hdsr_pack2x16(ox, oy),
+                        hdsr_pack2x16(conf, 0.0f));
+        c1 = make_uint4(hdsr_pack2x16(em[0], em[1]), hdsr_pack2x16(em[2], 0.0f), 0u, 0u);
+      }
+    }
+  }
+  b.hot[(size_t)g * 2] = h0;
+  b.hot[(size_t)g * 2 + 1] = h1;
+  b.cold[(size_t)g * 2] = c0;
+  b.cold[(size_t)g * 2 + 1] = c1;
+  b.bounds[g] = bounds;
+  b.depth[g] = depth_bits;
+
+  // workgroup reduction of the four counts
+  __shared__ uint32_t red[4][4];
+  const uint32_t s0 = wave_sum(n_small), s1 = wave_sum(n_large), s2 = wave_sum(is_large), s3 = wave_sum(vis);
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    red[w][0] = s0;
+    red[w][1] = s1;
+    red[w][2] = s2;
+    red[w][3] = s3;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint4 t = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      t.x += red[i][0];
+      t.y += red[i][1];
+      t.z += red[i][2];
+      t.w += red[i][3];
+    }
+    b.block_cnt[blk] = t;
+  }
+}
+
+// Rebuild of the 14-word reference record for frame f (debug readback only).
+__global__ void k_cache14(Bufs b, uint32_t f, uint32_t n, uint32_t* dst) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t g = b.frames[f].splat_base + r;
+  uint32_t* w = dst + (size_t)r * 14;
+  const uint32_t bd = b.bounds[g];
+  if (bd == HDSR_SENTINEL) {
+    for (int i = 0; i < 14; ++i) w[i] = 0;
+    w[7] = HDSR_SENTINEL;
+    return;
+  }
+  const uint4 h0 = b.hot[(size_t)g * 2], h1 = b.hot[(size_t)g * 2 + 1];
+  const uint4 c0 = b.cold[(size_t)g * 2], c1 = b.cold[(size_t)g * 2 + 1];
+  w[0] = h0.x;
+  w[1] = h0.y;
+  w[2] = h0.z;
+  w[3] = h0.w;
+  w[4] = h1.x;
+  w[5] = c0.x;
+  w[6] = c0.y;
+  w[7] = bd;
+  w[8] = c0.z;
+  w[9] = h1.z;
+  w[10] = h1.w & 0xFFFFu;
+  w[11] = c1.x;
+  w[12] = c1.y;
+  w[13] = c0.w;
+}
+
+}  // namespace
+
+void launch_project(const Bufs& b, hipStream_t stream) {
+  if (!b.splat_blocks) return;
+  hipLaunchKernelGGL(k_project, dim3(b.splat_blocks), dim3(kSplatBlock), 0, stream, b);
+}
+
+void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_cache14, dim3((n + 255) / 256), dim3(256), 0, stream, b, frame, n, dst);
+}
+
+}  // namespace hdsr

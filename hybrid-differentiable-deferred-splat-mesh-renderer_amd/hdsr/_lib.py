@@ -1,0 +1,122 @@
+"""ctypes binding of libhdsr.so (the C ABI declared in include/hdsr.h).
+
+The library is the only compute path: there is no CPU fallback, and loading fails loudly when
+the built library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libhdsr.so")
+
+HDSR_OK = 0
+HDSR_E_INVALID_ARG = -1
+HDSR_E_OOM = -2
+HDSR_E_CAPACITY = -3
+HDSR_E_HIP = -4
+HDSR_E_STATE = -5
+HDSR_E_UNSUPPORTED = -6
+
+HDSR_F16_COMPOSITE = 1
+HDSR_REF_CAPS = 2
+HDSR_GBUFFER = 4
+HDSR_DEBUG = 8
+HDSR_OUT_RGBA16 = 16
+
+HDSR_MEM_HOST = 0
+HDSR_MEM_DEVICE = 1
+
+OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16 = range(5)
+
+DBG = {
+    "projcache": 0, "depth": 1, "tile_counts": 2, "tile_offsets": 3, "entries": 4, "entries_canon": 5,
+    "small_list": 6, "large_list": 7, "large_bases": 8, "chunks": 9, "keys20": 10, "large_splats": 11,
+}
+
+
+class PlanIn(C.Structure):
+    _fields_ = [("viewport_width", C.c_uint32), ("viewport_height", C.c_uint32), ("tile_size_px", C.c_uint32),
+                ("splat_count", C.c_uint32), ("tile_entry_multiplier", C.c_float)]
+
+
+class Plan(C.Structure):
+    _fields_ = [("viewport_width", C.c_uint32), ("viewport_height", C.c_uint32), ("tile_size_px", C.c_uint32),
+                ("tile_columns", C.c_uint32), ("tile_rows", C.c_uint32), ("tile_count", C.c_uint32),
+                ("morton_tile_count", C.c_uint32), ("splat_count", C.c_uint32),
+                ("max_total_tile_refs", C.c_uint32)]
+
+
+class Splats(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p),
+                ("opacities", C.c_void_p), ("materials", C.c_void_p), ("normals", C.c_void_p),
+                ("normals_len", C.c_uint64), ("sh_data", C.c_void_p), ("sorted_indices", C.c_void_p),
+                ("count", C.c_uint32), ("sh_degree", C.c_uint32), ("memory", C.c_uint32)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("depth", C.c_void_p), ("gbuffer_normal", C.c_void_p),
+                ("gbuffer_material", C.c_void_p), ("rgba16", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "refs", "small_pairs", "large_pairs", "large_splats", "visible_splats", "small_tiles", "large_tiles",
+        "chunks", "segments", "pair_capacity", "overflow_reruns", "cap_refs_tripped", "cap_large_splats_tripped",
+        "cap_chunks_tripped", "reserved0", "reserved1")]
+
+
+# Every symbol include/hdsr.h declares: (name, restype, argtypes).
+SIGNATURES = [
+    ("hdsr_plan_compute", C.c_int, [C.POINTER(PlanIn), C.POINTER(Plan)]),
+    ("hdsr_create", C.c_int, [C.POINTER(Plan), C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("hdsr_destroy", None, [C.c_void_p]),
+    ("hdsr_bind_splats", C.c_int, [C.c_void_p, C.POINTER(Splats)]),
+    ("hdsr_update_sorted_indices", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("hdsr_set_frame", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hdsr_render", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hdsr_composite_only", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hdsr_sync", C.c_int, [C.c_void_p]),
+    ("hdsr_get_outputs", C.c_int, [C.c_void_p, C.POINTER(Outputs)]),
+    ("hdsr_readback", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
+    ("hdsr_debug_size", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
+    ("hdsr_debug_readback", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
+    ("hdsr_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("hdsr_last_error", C.c_char_p, [C.c_void_p]),
+    ("hdsr_version", C.c_char_p, []),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libhdsr.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP compositor library not built: {LIB_PATH} (run build.py)")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HdsrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hdsr error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != HDSR_OK:
+        msg = ""
+        if ctx is not None:
+            m = load().hdsr_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise HdsrError(rc, msg)

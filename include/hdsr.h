@@ -1,0 +1,181 @@
+/* hdsr.h - C ABI of the MI355X splat compositor (drop-in for the reference's gpuTileSplatCompositor).
+ *
+ * Plain pointers and sizes only; no HIP or torch types cross this boundary (a stream is passed
+ * as void*, NULL = the default stream).  Every function returns 0 on success or a negative
+ * HDSR_E_* code; hdsr_last_error() gives the message of the last failure on a context.  A context
+ * is bound to the device that was current when it was created and must be used from one host
+ * thread at a time.
+ *
+ * Each entry point names the reference interface it replaces (src/gpuTileSplatCompositor.ts
+ * unless stated).  The JavaScript facade (node/gpuTileSplatCompositor.js) and the Python facade
+ * (hdsr/compositor.py) bind exactly these symbols.
+ */
+#ifndef HDSR_H
+#define HDSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDSR_OK 0
+#define HDSR_E_INVALID_ARG (-1)
+#define HDSR_E_OOM (-2)
+#define HDSR_E_CAPACITY (-3)
+#define HDSR_E_HIP (-4)
+#define HDSR_E_STATE (-5)
+#define HDSR_E_UNSUPPORTED (-6)
+
+/* TILE_SPLAT_FRAME_UNIFORM_BYTES (gpuTileSplatCompositor.ts:18) */
+#define HDSR_FRAME_UNIFORM_BYTES 256
+
+/* planTileSplatCompositor input (gpuTileSplatCompositor.ts:109-116). */
+typedef struct hdsr_plan_in {
+  uint32_t viewport_width;
+  uint32_t viewport_height;
+  uint32_t tile_size_px;          /* must be 16 (the rasterizer's tile) */
+  uint32_t splat_count;
+  float tile_entry_multiplier;    /* <= 0 means the reference default 16 */
+} hdsr_plan_in;
+
+/* TileSplatCompositorPlan (gpuTileSplatCompositor.ts:39-49). */
+typedef struct hdsr_plan {
+  uint32_t viewport_width;
+  uint32_t viewport_height;
+  uint32_t tile_size_px;
+  uint32_t tile_columns;
+  uint32_t tile_rows;
+  uint32_t tile_count;
+  uint32_t morton_tile_count;
+  uint32_t splat_count;
+  uint32_t max_total_tile_refs;   /* reference budget, reported; capacities here are lifted */
+} hdsr_plan;
+
+/* planTileSplatCompositor (gpuTileSplatCompositor.ts:109-139): identical arithmetic. */
+int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
+
+/* createTileSplatCompositor options (gpuTileSplatCompositor.ts:141-145) and output selection. */
+#define HDSR_F16_COMPOSITE 1u   /* f16 compositing variant: not provided (returns HDSR_E_UNSUPPORTED) */
+#define HDSR_REF_CAPS 2u        /* emulate WebGPU caps: not provided (returns HDSR_E_UNSUPPORTED) */
+#define HDSR_GBUFFER 4u         /* also write the oct-normal and material/emissive/confidence words */
+#define HDSR_DEBUG 8u           /* keep the pre-sort entry list and the 20-bit keys for readback */
+#define HDSR_OUT_RGBA16 16u     /* also write binary16 RGBA (the reference's rgba16float texture) */
+
+typedef struct hdsr_ctx hdsr_ctx;
+
+/* createTileSplatCompositor (gpuTileSplatCompositor.ts:141-728) / resources.destroy() (:700-726). */
+int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out);
+void hdsr_destroy(hdsr_ctx* ctx);
+
+#define HDSR_MEM_HOST 0u        /* pointers are host memory: the context keeps device copies */
+#define HDSR_MEM_DEVICE 1u      /* pointers are device memory: borrowed, must outlive the context */
+
+/* The splat storage buffers bound by createTileSplatBindGroups (gpuTileSplatCompositor.ts:792-995),
+ * in the layouts of uploadSplatAttributeBuffers (src/splats.ts:376-500):
+ *   positions 3N f32, scales 3N f32 (log-scale), rotations 4N f32 (w,x,y,z), opacities N f32,
+ *   materials N u32 (pack2x16float(roughness, metalness)), normals: normals_len f32
+ *   ([geometric 3N | detail 3N], or 0 = derive from the covariance), sh_data N*(3+3K+3) f32
+ *   (dc rgb, K = (d+1)^2-1 coefficients rgb-interleaved, emissive rgb),
+ *   sorted_indices N u32 (a permutation; the identity is allowed). */
+typedef struct hdsr_splats {
+  const float* positions;
+  const float* scales;
+  const float* rotations;
+  const float* opacities;
+  const uint32_t* materials;
+  const float* normals;
+  uint64_t normals_len;
+  const float* sh_data;
+  const uint32_t* sorted_indices;
+  uint32_t count;
+  uint32_t sh_degree;
+  uint32_t memory;                /* HDSR_MEM_HOST or HDSR_MEM_DEVICE */
+} hdsr_splats;
+
+/* createTileSplatBindGroups (gpuTileSplatCompositor.ts:792-995). */
+int hdsr_bind_splats(hdsr_ctx* ctx, const hdsr_splats* splats);
+
+/* Replaces the sortedIndices buffer contents (the reference's encodeSort output,
+ * src/gpuSortPrototype.ts:275-287 / src/splatSort.ts:32-55). */
+int hdsr_update_sorted_indices(hdsr_ctx* ctx, const uint32_t* sorted_indices, uint32_t memory);
+
+/* queue.writeBuffer(frameUniformBuffer, ...) of writeTileSplatFrameUniforms
+ * (gpuTileSplatCompositor.ts:730-774; src/splatRenderer.ts:1030): the 256-byte block verbatim. */
+int hdsr_set_frame(hdsr_ctx* ctx, const void* uniforms256);
+
+/* encodeFullComputeCompositorPipeline (gpuTileSplatCompositor.ts:1023-1179): enqueues the whole
+ * path on 'stream' and returns.  Outputs are complete once the stream reaches this point, unless
+ * the pair capacity overflowed; hdsr_sync() detects that, grows the capacity and re-renders. */
+int hdsr_render(hdsr_ctx* ctx, void* stream);
+
+/* encodeCompositeOnly (gpuTileSplatCompositor.ts:1001-1014): re-runs only the rasterizer on the
+ * last frame's sorted tile lists (static camera). */
+int hdsr_composite_only(hdsr_ctx* ctx, void* stream);
+
+/* Waits for the last render, and re-renders it with a larger pair capacity if it overflowed. */
+int hdsr_sync(hdsr_ctx* ctx);
+
+/* Output textures (src/splatRenderer.ts:908-943) as device buffers owned by the context. */
+typedef struct hdsr_outputs {
+  float* rgba;                    /* H*W*4 f32, linear RGB + alpha (row-major, top row first) */
+  float* depth;                   /* H*W f32: alpha-weighted NDC depth vote */
+  uint32_t* gbuffer_normal;       /* H*W u32 pack2x16float(oct normal), if HDSR_GBUFFER */
+  uint32_t* gbuffer_material;     /* H*W*4 u32 (material, emissive rg, emissive b, confidence) */
+  uint16_t* rgba16;               /* H*W*4 binary16, if HDSR_OUT_RGBA16 */
+  uint32_t width;
+  uint32_t height;
+} hdsr_outputs;
+int hdsr_get_outputs(hdsr_ctx* ctx, hdsr_outputs* out);
+
+#define HDSR_OUT_RGBA 0
+#define HDSR_OUT_DEPTH 1
+#define HDSR_OUT_GB_NORMAL 2
+#define HDSR_OUT_GB_MATERIAL 3
+#define HDSR_OUT_RGBA16_BUF 4
+/* Copies an output to host memory (after hdsr_sync). */
+int hdsr_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
+
+/* Intermediates of the last frame, in the reference's layouts (ranks, Morton ids). */
+#define HDSR_DBG_PROJCACHE 0      /* N*14 u32 (gpu_project_splats.wgsl:2-16) */
+#define HDSR_DBG_DEPTH 1          /* N u32 depth bits */
+#define HDSR_DBG_TILE_COUNTS 2    /* mortonTileCount u32 */
+#define HDSR_DBG_TILE_OFFSETS 3   /* mortonTileCount u32 */
+#define HDSR_DBG_ENTRIES 4        /* R u32 sort ranks, final per-tile order */
+#define HDSR_DBG_ENTRIES_CANON 5  /* R u32 sort ranks before the per-tile sort (HDSR_DEBUG) */
+#define HDSR_DBG_SMALL_LIST 6     /* small tiles (<= 4096 entries), Morton ids, row-major order */
+#define HDSR_DBG_LARGE_LIST 7     /* large tiles, Morton ids */
+#define HDSR_DBG_LARGE_BASES 8    /* overflow base per large tile */
+#define HDSR_DBG_CHUNKS 9         /* (start, count) u32 pairs */
+#define HDSR_DBG_KEYS20 10        /* R u32 (q20 << 12) | local index (HDSR_DEBUG) */
+#define HDSR_DBG_LARGE_SPLATS 11  /* sort ranks of splats covering > 16 tiles */
+int hdsr_debug_size(hdsr_ctx* ctx, int which, uint64_t* bytes);
+int hdsr_debug_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
+
+typedef struct hdsr_stats {
+  uint32_t refs;                  /* total tile references R */
+  uint32_t small_pairs;
+  uint32_t large_pairs;
+  uint32_t large_splats;
+  uint32_t visible_splats;
+  uint32_t small_tiles;
+  uint32_t large_tiles;
+  uint32_t chunks;
+  uint32_t segments;
+  uint32_t pair_capacity;
+  uint32_t overflow_reruns;       /* frames re-rendered after a capacity overflow */
+  uint32_t cap_refs_tripped;      /* R > maxTotalTileRefs (gpu_tile_splat_composite.wgsl:114) */
+  uint32_t cap_large_splats_tripped; /* > 65535 large splats (gpu_classify_large_splats.wgsl:52) */
+  uint32_t cap_chunks_tripped;    /* > 1024 chunks (gpu_tile_bucket_sort.wgsl:126) */
+  uint32_t reserved[2];
+} hdsr_stats;
+int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
+
+const char* hdsr_last_error(hdsr_ctx* ctx);
+const char* hdsr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDSR_H */

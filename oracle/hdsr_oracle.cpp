@@ -1,0 +1,911 @@
+// hdsr_oracle.cpp - CPU restatement of the reference's compute-compositor splat path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this file: only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark use it, as the checker.
+//
+// It restates, step by step and in plain scalar code, the WGSL kernels of the reference:
+//   projection cache      src/shaders/gpu_project_splats.wgsl:112-358
+//   large-splat classify  src/shaders/gpu_classify_large_splats.wgsl:33-66
+//   count / scatter       src/shaders/gpu_tile_splat_composite.wgsl:64-119,
+//                         src/shaders/gpu_large_splat_scatter.wgsl:52-89
+//   prefix scan           src/shaders/gpu_prefix_sum.wgsl:14-85
+//   tile classify         src/shaders/gpu_tile_classify.wgsl:44-86
+//   small-tile sort       src/shaders/gpu_tile_depth_sort.wgsl:28-79
+//   bucket cascade        src/shaders/gpu_tile_bucket_sort.wgsl:41-144
+//   chunk sort            src/shaders/gpu_tile_chunk_sort.wgsl:36-97
+//   composite (fp32)      src/shaders/gpu_tile_splat_composite.wgsl:162-370
+// and the plan arithmetic of src/gpuTileSplatCompositor.ts:109-139, 730-774.
+//
+// Decisions where the reference is nondeterministic or under-specified:
+//   * atomics resolve in ascending invocation order (per-tile lists hold the small splats in
+//     ascending sort rank, then the large splats in ascending sort rank; tile lists are in
+//     ascending row-major tile order);
+//   * the per-tile sort key is the 20-bit log-depth key q20 (in the tile's log frame, the one
+//     the bucket cascade uses) shifted above the entry's 12-bit local index, so keys are unique
+//     and any correct sort gives the same order;
+//   * capacities are lifted: nothing is dropped; the caps the reference would trip are counted;
+//   * no floating-point contraction (built with -ffp-contract=off); exp/ln come from
+//     hdsr_fmath.h, the same functions the kernels use;
+//   * WGSL max/min are fmaxf/fminf (a NaN operand yields the other operand).
+// An FP64 twin (projection and composite in double, same quantisation points, same per-tile
+// order) gives the rounding-error statement for the fp32 result.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../hybrid-differentiable-deferred-splat-mesh-renderer_amd/csrc/hdsr_fmath.h"
+
+namespace {
+
+struct Uniforms {
+  float vp[16];
+  float vw, vh, tile;
+  uint32_t gx, gy, n, total_refs;
+  float splat_scale;
+  uint32_t sh_degree;
+  float cam[3];
+  float view[16];
+  float fx, fy;
+  float nm[9];
+};
+
+// 256-byte frame uniform block (src/shaders/gpu_project_splats.wgsl:27-44,
+// src/gpuTileSplatCompositor.ts:730-774).
+Uniforms parse_uniforms(const void* p) {
+  const float* f = static_cast<const float*>(p);
+  const uint32_t* u = static_cast<const uint32_t*>(p);
+  Uniforms U;
+  memcpy(U.vp, f, 64);
+  U.vw = f[16];
+  U.vh = f[17];
+  U.tile = f[18];
+  U.gx = u[20];
+  U.gy = u[21];
+  U.n = u[22];
+  U.total_refs = u[23];
+  U.splat_scale = f[24];
+  U.sh_degree = u[25];
+  U.cam[0] = f[28];
+  U.cam[1] = f[29];
+  U.cam[2] = f[30];
+  memcpy(U.view, f + 32, 64);
+  U.fx = f[48];
+  U.fy = f[49];
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) U.nm[c * 3 + r] = f[52 + c * 4 + r];
+  return U;
+}
+
+template <typename F>
+struct V3 {
+  F x, y, z;
+};
+
+template <typename F>
+V3<F> v3(F x, F y, F z) {
+  return V3<F>{x, y, z};
+}
+template <typename F>
+V3<F> add(V3<F> a, V3<F> b) {
+  return v3<F>(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+template <typename F>
+V3<F> sub(V3<F> a, V3<F> b) {
+  return v3<F>(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+template <typename F>
+V3<F> mul(V3<F> a, F s) {
+  return v3<F>(a.x * s, a.y * s, a.z * s);
+}
+template <typename F>
+V3<F> smul(F s, V3<F> a) {
+  return v3<F>(s * a.x, s * a.y, s * a.z);
+}
+template <typename F>
+V3<F> vdiv(V3<F> a, F s) {
+  return v3<F>(a.x / s, a.y / s, a.z / s);
+}
+template <typename F>
+F dot(V3<F> a, V3<F> b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <typename F>
+V3<F> cross(V3<F> a, V3<F> b) {
+  return v3<F>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <typename F>
+F vlen(V3<F> a) {
+  return std::sqrt(dot(a, a));
+}
+template <typename F>
+V3<F> normalize(V3<F> a) {
+  return vdiv(a, vlen(a));
+}
+template <typename F>
+F fmx(F a, F b) {
+  return std::fmax(a, b);
+}
+template <typename F>
+F fmn(F a, F b) {
+  return std::fmin(a, b);
+}
+
+inline float expT(float x) { return hdsr_expf(x); }
+inline double expT(double x) { return std::exp(x); }
+inline float lnT(float x) { return hdsr_logf(x); }
+inline double lnT(double x) { return std::log(x); }
+
+// mat4 (column-major) * vec4(p, 1): component i = ((m0i*x + m1i*y) + m2i*z) + m3i*1
+template <typename F>
+void mat4_point(const float* m, V3<F> p, F out[4]) {
+  for (int i = 0; i < 4; ++i)
+    out[i] = (((F)m[i] * p.x + (F)m[4 + i] * p.y) + (F)m[8 + i] * p.z) + (F)m[12 + i] * (F)1;
+}
+
+// rotateAxis (gpu_project_splats.wgsl:75-79)
+template <typename F>
+V3<F> rotate_axis(const F q4[4], V3<F> axis) {
+  const F len = std::sqrt(((q4[0] * q4[0] + q4[1] * q4[1]) + q4[2] * q4[2]) + q4[3] * q4[3]);
+  const F d = fmx(len, (F)0.000001f);
+  const F q[4] = {q4[0] / d, q4[1] / d, q4[2] / d, q4[3] / d};
+  const V3<F> u = v3<F>(q[1], q[2], q[3]);
+  const V3<F> inner = add(cross(u, axis), smul(q[0], axis));
+  return add(axis, smul((F)2, cross(u, inner)));
+}
+
+// octEncode (gpu_project_splats.wgsl:82-89)
+template <typename F>
+void oct_encode(V3<F> n, F* ox, F* oy) {
+  const F sum = (std::fabs(n.x) + std::fabs(n.y)) + std::fabs(n.z);
+  F px = n.x / sum, py = n.y / sum;
+  if (n.z < (F)0) {
+    const F sx = px >= (F)0 ? (F)1 : (F)-1;
+    const F sy = py >= (F)0 ? (F)1 : (F)-1;
+    const F nx = ((F)1 - std::fabs(py)) * sx;
+    const F ny = ((F)1 - std::fabs(px)) * sy;
+    px = nx;
+    py = ny;
+  }
+  *ox = px;
+  *oy = py;
+}
+
+struct Splats {
+  const float* pos;
+  const float* logscale;
+  const float* rot;
+  const float* opacity;
+  const uint32_t* sorted;
+  const uint32_t* material;
+  const float* normals;
+  uint32_t normals_len;
+  const float* shdata;
+  uint32_t n;
+  uint32_t sh_degree;
+};
+
+// The projected record of one splat, in F precision, before packing.
+template <typename F>
+struct Proj {
+  bool visible;
+  F px, py, k0, k1, k2, radius, opacity, depth;
+  F r, g, b, conf, ox, oy, er, eg, eb;
+  uint32_t material;
+  uint32_t bounds;
+};
+
+const double kSH[13] = {0.4886025119029199, 1.0925484305920792, 1.0925484305920792, 0.31539156525252005,
+                        1.0925484305920792, 0.5462742152960396, 0.5900435899266435, 2.890611442640554,
+                        0.4570457994644658, 0.3731763325901154, 0.4570457994644658, 1.445305721320277,
+                        0.5900435899266435};
+
+// project_splats (gpu_project_splats.wgsl:112-358) for sort rank r.
+template <typename F>
+Proj<F> project_one(const Splats& S, const Uniforms& U, uint32_t rank) {
+  Proj<F> P;
+  memset(&P, 0, sizeof(P));
+  P.visible = false;
+  const uint32_t id = S.sorted[rank];
+  if (id >= S.n) return P;  // the reference writes nothing; the kernels treat it as invisible
+  const V3<F> center = v3<F>(S.pos[id * 3], S.pos[id * 3 + 1], S.pos[id * 3 + 2]);
+  F clip[4];
+  mat4_point<F>(U.vp, center, clip);
+  if (clip[3] <= (F)0) return P;
+  const F safe_w = fmx(clip[3], (F)HDSR_MIN_CLIP_W);
+  const F ndc_x = clip[0] / safe_w, ndc_y = clip[1] / safe_w;
+  const F px = (ndc_x * (F)0.5f + (F)0.5f) * (F)U.vw;
+  const F py = ((F)1 - (ndc_y * (F)0.5f + (F)0.5f)) * (F)U.vh;
+  const F depth = clip[2] / safe_w;
+
+  const F q4[4] = {S.rot[id * 4], S.rot[id * 4 + 1], S.rot[id * 4 + 2], S.rot[id * 4 + 3]};
+  const F sx = expT((F)S.logscale[id * 3]) * (F)U.splat_scale;
+  const F sy = expT((F)S.logscale[id * 3 + 1]) * (F)U.splat_scale;
+  const F sz = expT((F)S.logscale[id * 3 + 2]) * (F)U.splat_scale;
+  const V3<F> a0 = mul(rotate_axis<F>(q4, v3<F>(1, 0, 0)), sx);
+  const V3<F> a1 = mul(rotate_axis<F>(q4, v3<F>(0, 1, 0)), sy);
+  const V3<F> a2 = mul(rotate_axis<F>(q4, v3<F>(0, 0, 1)), sz);
+  const F len0 = dot(a0, a0), len1 = dot(a1, a1), len2 = dot(a2, a2);
+  const F min_len = fmn(len0, fmn(len1, len2));
+  const V3<F> normal_axis = (len0 == min_len) ? a0 : ((len1 == min_len) ? a1 : a2);
+  const V3<F> local_normal = normalize(normal_axis);
+
+  const float* V = U.view;
+  const V3<F> w0 = v3<F>(V[0], V[4], V[8]);
+  const V3<F> w1 = v3<F>(V[1], V[5], V[9]);
+  const V3<F> w2 = v3<F>(V[2], V[6], V[10]);
+  F vc4[4];
+  mat4_point<F>(V, center, vc4);
+  const F vz = fmn(vc4[2], (F)-0.001f);
+  const F j1x = (F)U.fx / vz, j1y = (F)U.fy / vz;
+  const F j2x = (-j1x / vz) * vc4[0];
+  const F j2y = (-j1y / vz) * vc4[1];
+  const V3<F> tt0 = add(smul(j1x, w0), smul(j2x, w2));
+  const V3<F> tt1 = add(smul(j1y, w1), smul(j2y, w2));
+  const V3<F> b0 = v3<F>(dot(a0, tt0), dot(a1, tt0), dot(a2, tt0));
+  const V3<F> b1 = v3<F>(dot(a0, tt1), dot(a1, tt1), dot(a2, tt1));
+  const F cxx_raw = dot(b0, b0), cxy = dot(b0, b1), cyy_raw = dot(b1, b1);
+  const F det_orig = cxx_raw * cyy_raw - cxy * cxy;
+  const F cxx = cxx_raw + (F)HDSR_LOW_PASS;
+  const F cyy = cyy_raw + (F)HDSR_LOW_PASS;
+  const F det_blur = cxx * cyy - cxy * cxy;
+  const F aa = std::sqrt(fmx(det_orig / fmx(det_blur, (F)0.000001f), (F)0));
+  const F mid = (F)0.5f * (cxx + cyy);
+  const F det = fmx(det_blur, (F)0.1f);
+  const F lambda = mid + std::sqrt(fmx(mid * mid - det, (F)0.1f));
+  const F radius = std::ceil((F)HDSR_SIGMA_RADIUS * std::sqrt(lambda));
+  if (radius <= (F)0) return P;
+
+  const F det_inv = (F)1 / fmx(det_blur, (F)0.000001f);
+  P.k0 = cyy * det_inv;
+  P.k1 = -cxy * det_inv;
+  P.k2 = cxx * det_inv;
+  P.opacity = fmn(fmx((F)S.opacity[id] * aa, (F)0), (F)0.99f);
+
+  // tile bounds (gpu_project_splats.wgsl:210-218,229-232); integer results, computed in fp32
+  {
+    const float ts = std::fmax(U.tile, 1.0f);
+    const uint32_t mtx = (U.gx > 1u ? U.gx : 1u) - 1u, mty = (U.gy > 1u ? U.gy : 1u) - 1u;
+    const float rc = std::fmin((float)radius, ts * 8.0f);
+    const float minx = std::fmax((float)px - rc, 0.0f), miny = std::fmax((float)py - rc, 0.0f);
+    const float maxx = std::fmin((float)px + rc, U.vw), maxy = std::fmin((float)py + rc, U.vh);
+    uint32_t t0x = hdsr_u32sat(std::floor(minx / ts)), t0y = hdsr_u32sat(std::floor(miny / ts));
+    uint32_t t1x = hdsr_u32sat(std::floor(std::fmax((maxx - HDSR_FOOTPRINT_EPS) / ts, 0.0f)));
+    uint32_t t1y = hdsr_u32sat(std::floor(std::fmax((maxy - HDSR_FOOTPRINT_EPS) / ts, 0.0f)));
+    t0x = std::min(t0x, mtx);
+    t0y = std::min(t0y, mty);
+    t1x = std::min(t1x, mtx);
+    t1y = std::min(t1y, mty);
+    P.bounds = (t0x & 0xFFu) | ((t0y & 0xFFu) << 8) | ((t1x & 0xFFu) << 16) | ((t1y & 0xFFu) << 24);
+  }
+  P.px = px;
+  P.py = py;
+  P.radius = radius;
+  P.depth = depth;
+  P.material = S.material[id];
+
+  // normals (gpu_project_splats.wgsl:234-291)
+  V3<F> n = local_normal;
+  const uint32_t nb = id * 3u;
+  if (nb + 2u < S.normals_len) {
+    const V3<F> baked = v3<F>(S.normals[nb], S.normals[nb + 1], S.normals[nb + 2]);
+    const F bl = vlen(baked);
+    if (bl > (F)0.001f) n = vdiv(baked, bl);
+  }
+  const uint32_t db = U.n * 3u + id * 3u;
+  if (db + 2u < S.normals_len) {
+    const V3<F> detail = v3<F>(S.normals[db], S.normals[db + 1], S.normals[db + 2]);
+    const F dl = vlen(detail);
+    if (dl > (F)0.001f) {
+      V3<F> tc;
+      if (len0 >= len1 && len0 >= len2)
+        tc = a0;
+      else if (len1 >= len2)
+        tc = a1;
+      else
+        tc = a2;
+      const V3<F> tp = sub(tc, mul(n, dot(tc, n)));
+      const F tl = vlen(tp);
+      if (tl > (F)0.001f) {
+        const V3<F> T = vdiv(tp, tl);
+        const V3<F> B = cross(n, T);
+        const V3<F> dn = vdiv(detail, dl);
+        n = normalize(add(add(mul(T, dn.x), mul(B, dn.y)), mul(n, dn.z)));
+      }
+    }
+  }
+  const V3<F> cam = v3<F>(U.cam[0], U.cam[1], U.cam[2]);
+  const V3<F> view_dir = normalize(sub(cam, center));
+  V3<F> faced;
+  {
+    const V3<F> nn = normalize(n);
+    const V3<F> vv = normalize(view_dir);
+    faced = dot(nn, vv) < (F)0 ? v3<F>(-nn.x, -nn.y, -nn.z) : nn;
+  }
+  P.conf = fmn(fmx(dot(faced, view_dir), (F)0), (F)1);
+  {
+    const float* M = U.nm;
+    V3<F> wn;
+    wn.x = ((F)M[0] * faced.x + (F)M[3] * faced.y) + (F)M[6] * faced.z;
+    wn.y = ((F)M[1] * faced.x + (F)M[4] * faced.y) + (F)M[7] * faced.z;
+    wn.z = ((F)M[2] * faced.x + (F)M[5] * faced.y) + (F)M[8] * faced.z;
+    oct_encode(normalize(wn), &P.ox, &P.oy);
+  }
+
+  // spherical harmonics, WGSL sign convention (gpu_project_splats.wgsl:293-351)
+  const uint32_t deg = U.sh_degree;
+  const uint32_t k = (deg + 1u) * (deg + 1u) - 1u;
+  const uint32_t base = id * (3u + 3u * k + 3u);
+  const float* sh = S.shdata + base;
+  F c[3] = {sh[0], sh[1], sh[2]};
+  if (deg >= 1u) {
+    const F x = view_dir.x, y = view_dir.y, z = view_dir.z;
+    const float* s1 = sh + 3;
+    for (int ch = 0; ch < 3; ++ch)
+      c[ch] = c[ch] + (F)(float)kSH[0] * (((F)s1[ch] * y + (F)s1[3 + ch] * z) + (F)s1[6 + ch] * x);
+    if (deg >= 2u) {
+      const F xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+      const F b2[5] = {(F)(float)kSH[1] * xy, (F)(float)kSH[2] * yz,
+                       (F)(float)kSH[3] * (((F)2 * zz - xx) - yy), (F)(float)kSH[4] * xz,
+                       (F)(float)kSH[5] * (xx - yy)};
+      const float* s2 = s1 + 9;
+      for (int ch = 0; ch < 3; ++ch) {
+        F acc = (F)s2[ch] * b2[0];
+        for (int j = 1; j < 5; ++j) acc = acc + (F)s2[3 * j + ch] * b2[j];
+        c[ch] = c[ch] + acc;
+      }
+      if (deg >= 3u) {
+        const F b3[7] = {((F)(float)kSH[6] * y) * ((F)3 * xx - yy),
+                         (((F)(float)kSH[7] * x) * y) * z,
+                         ((F)(float)kSH[8] * y) * (((F)4 * zz - xx) - yy),
+                         ((F)(float)kSH[9] * z) * (((F)2 * zz - (F)3 * xx) - (F)3 * yy),
+                         ((F)(float)kSH[10] * x) * (((F)4 * zz - xx) - yy),
+                         ((F)(float)kSH[11] * z) * (xx - yy),
+                         ((F)(float)kSH[12] * x) * (xx - (F)3 * yy)};
+        const float* s3 = s2 + 15;
+        for (int ch = 0; ch < 3; ++ch) {
+          F acc = (F)s3[ch] * b3[0];
+          for (int j = 1; j < 7; ++j) acc = acc + (F)s3[3 * j + ch] * b3[j];
+          c[ch] = c[ch] + acc;
+        }
+      }
+    }
+  }
+  P.r = fmn(fmx(c[0], (F)0), (F)1);
+  P.g = fmn(fmx(c[1], (F)0), (F)1);
+  P.b = fmn(fmx(c[2], (F)0), (F)1);
+  const float* em = sh + 3 + 3 * k;
+  P.er = em[0];
+  P.eg = em[1];
+  P.eb = em[2];
+  P.visible = true;
+  return P;
+}
+
+uint32_t f2u(float f) { return hdsr_f2u(f); }
+float u2f(uint32_t u) { return hdsr_u2f(u); }
+float h2f(uint32_t h) { return hdsr_f16_to_f32(h & 0xFFFFu); }
+
+// The 14-word cache record (gpu_project_splats.wgsl:2-16,220-232,290-357).
+void pack_record(const Proj<float>& P, uint32_t* w, uint32_t* depth_bits) {
+  if (!P.visible) {
+    for (int i = 0; i < 14; ++i) w[i] = 0;
+    w[7] = HDSR_SENTINEL;
+    *depth_bits = HDSR_SENTINEL;
+    return;
+  }
+  w[0] = f2u(P.px);
+  w[1] = f2u(P.py);
+  w[2] = f2u(P.k0);
+  w[3] = f2u(P.k1);
+  w[4] = f2u(P.k2);
+  w[5] = P.material;
+  w[6] = hdsr_pack2x16(P.radius, P.opacity);
+  w[7] = P.bounds;
+  w[8] = hdsr_pack2x16(P.ox, P.oy);
+  w[9] = hdsr_pack2x16(P.r, P.g);
+  w[10] = hdsr_pack2x16(P.b, 0.0f);
+  w[11] = hdsr_pack2x16(P.er, P.eg);
+  w[12] = hdsr_pack2x16(P.eb, 0.0f);
+  w[13] = hdsr_pack2x16(P.conf, 0.0f);
+  *depth_bits = f2u(P.depth);
+}
+
+struct Frame {
+  Uniforms U;
+  uint32_t n = 0, tiles_x = 0, tiles_y = 0, width = 0, height = 0, morton_count = 0;
+  std::vector<uint32_t> cache;       // 14 * n
+  std::vector<uint32_t> depth;       // n
+  std::vector<uint32_t> large_list;  // sort ranks with > 16 tiles, ascending
+  std::vector<uint32_t> tile_counts, tile_offsets;  // morton_count
+  std::vector<uint32_t> entries_canon;  // after scatter, before the per-tile sort
+  std::vector<uint32_t> small_tiles, large_tiles, large_bases;
+  std::vector<uint32_t> chunks;  // (start, count) pairs in tileEntries space
+  std::vector<uint32_t> entries;  // final per-tile order
+  std::vector<uint32_t> keys;     // (q20 << 12) | local index, at final positions
+  std::vector<uint32_t> raw_order_inversions;  // per tile: inversions vs raw depth-bit order
+  std::vector<float> rgba, out_depth;  // H*W*4, H*W
+  std::vector<uint32_t> gb_normal, gb_material;  // H*W, H*W*4
+  std::vector<uint32_t> stats;  // see orc_stat_*
+  std::vector<double> rgba64, depth64;
+  Splats S;
+};
+
+enum {
+  ST_REFS = 0,
+  ST_SMALL_TILES,
+  ST_LARGE_TILES,
+  ST_CHUNKS,
+  ST_LARGE_SPLATS,
+  ST_VISIBLE,
+  ST_CAP_REFS_TRIPPED,
+  ST_CAP_LARGE_SPLATS_TRIPPED,
+  ST_CAP_CHUNKS_TRIPPED,
+  ST_REF_CAP,
+  ST_COUNT
+};
+
+uint32_t span_of(uint32_t b, uint32_t* x0, uint32_t* y0, uint32_t* x1, uint32_t* y1) {
+  *x0 = b & 0xFFu;
+  *y0 = (b >> 8) & 0xFFu;
+  *x1 = (b >> 16) & 0xFFu;
+  *y1 = (b >> 24) & 0xFFu;
+  return (*x1 - *x0 + 1u) * (*y1 - *y0 + 1u);  // u32 arithmetic, as the WGSL
+}
+
+// Per-tile sort with 20-bit log-depth keys over list[0..count) (in place).  log_min/scale are
+// the frame of the whole tile.  Writes the keys at their final positions.
+void sort_segment(uint32_t* list, uint32_t* keys_out, uint32_t count, const uint32_t* depth, float log_min,
+                  float scale) {
+  std::vector<uint32_t> key(count), val(list, list + count);
+  for (uint32_t i = 0; i < count; ++i) {
+    const float y = (hdsr_logf(std::fmax(u2f(depth[val[i]]), 1e-6f)) - log_min) * scale;
+    uint32_t q = hdsr_u32sat(y * 8192.0f);
+    if (q > 0xFFFFFu) q = 0xFFFFFu;
+    key[i] = (q << 12) | i;
+  }
+  std::vector<uint32_t> idx(count);
+  for (uint32_t i = 0; i < count; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  for (uint32_t i = 0; i < count; ++i) {
+    list[i] = val[idx[i]];
+    keys_out[i] = key[idx[i]];
+  }
+}
+
+void tile_frame(const uint32_t* list, uint32_t count, const uint32_t* depth, float* log_min, float* scale) {
+  uint32_t dmin = 0xFFFFFFFFu, dmax = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const uint32_t d = depth[list[i]];
+    dmin = std::min(dmin, d);
+    dmax = std::max(dmax, d);
+  }
+  const float lmin = hdsr_logf(std::fmax(u2f(dmin), 1e-6f));
+  const float lrange = hdsr_logf(std::fmax(u2f(dmax), 1e-6f)) - lmin;
+  *log_min = lmin;
+  *scale = (lrange < 1e-10f) ? 0.0f : 128.0f / lrange;
+}
+
+void bin_and_sort(Frame& F) {
+  const uint32_t n = F.n, M = F.morton_count;
+  F.tile_counts.assign(M, 0);
+  F.large_list.clear();
+  uint32_t visible = 0;
+  // classify_large_splats + count_tile_refs
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t b = F.cache[r * 14 + 7];
+    if (b == HDSR_SENTINEL) continue;
+    ++visible;
+    uint32_t x0, y0, x1, y1;
+    const uint32_t span = span_of(b, &x0, &y0, &x1, &y1);
+    if (span > HDSR_LARGE_SPLAT_TILES) F.large_list.push_back(r);
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+      for (uint32_t tx = x0; tx <= x1; ++tx) F.tile_counts[hdsr_morton2(tx, ty)]++;
+  }
+  // exclusive scan over the Morton-indexed counts
+  F.tile_offsets.assign(M, 0);
+  uint64_t run = 0;
+  for (uint32_t m = 0; m < M; ++m) {
+    F.tile_offsets[m] = (uint32_t)run;
+    run += F.tile_counts[m];
+  }
+  const uint32_t R = (uint32_t)run;
+  // scatter: small splats in ascending rank, then large splats in list order
+  F.entries_canon.assign(R, 0);
+  std::vector<uint32_t> cursor(M, 0);
+  for (uint32_t r = 0; r < n; ++r) {
+    const uint32_t b = F.cache[r * 14 + 7];
+    if (b == HDSR_SENTINEL) continue;
+    uint32_t x0, y0, x1, y1;
+    if (span_of(b, &x0, &y0, &x1, &y1) > HDSR_LARGE_SPLAT_TILES) continue;
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+      for (uint32_t tx = x0; tx <= x1; ++tx) {
+        const uint32_t t = hdsr_morton2(tx, ty);
+        F.entries_canon[F.tile_offsets[t] + cursor[t]++] = r;
+      }
+  }
+  for (uint32_t r : F.large_list) {
+    uint32_t x0, y0, x1, y1;
+    span_of(F.cache[r * 14 + 7], &x0, &y0, &x1, &y1);
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+      for (uint32_t tx = x0; tx <= x1; ++tx) {
+        const uint32_t t = hdsr_morton2(tx, ty);
+        F.entries_canon[F.tile_offsets[t] + cursor[t]++] = r;
+      }
+  }
+  // classify tiles in row-major order; overflow base = capacity + earlier large counts
+  F.small_tiles.clear();
+  F.large_tiles.clear();
+  F.large_bases.clear();
+  uint64_t overflow = 0;
+  for (uint32_t li = 0; li < F.tiles_x * F.tiles_y; ++li) {
+    const uint32_t m = hdsr_morton2(li % F.tiles_x, li / F.tiles_x);
+    const uint32_t c = F.tile_counts[m];
+    if (c == 0) continue;
+    if (c <= HDSR_MAX_TILE_ENTRIES) {
+      F.small_tiles.push_back(m);
+    } else {
+      F.large_tiles.push_back(m);
+      F.large_bases.push_back((uint32_t)(R + overflow));
+      overflow += c;
+    }
+  }
+  F.entries = F.entries_canon;
+  F.keys.assign(R, 0);
+  F.raw_order_inversions.assign(M, 0);
+  // small tiles: one sort over the whole tile
+  for (uint32_t m : F.small_tiles) {
+    const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+    float lmin, scale;
+    tile_frame(&F.entries[s], c, F.depth.data(), &lmin, &scale);
+    sort_segment(&F.entries[s], &F.keys[s], c, F.depth.data(), lmin, scale);
+  }
+  // large tiles: 128 log-depth buckets (stable), then consecutive 4096-entry chunks, each sorted
+  F.chunks.clear();
+  for (uint32_t m : F.large_tiles) {
+    const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+    float lmin, scale;
+    tile_frame(&F.entries[s], c, F.depth.data(), &lmin, &scale);
+    std::vector<uint32_t> bucket(c), bcount(HDSR_NUM_BUCKETS + 1, 0);
+    for (uint32_t i = 0; i < c; ++i) {
+      const float y = (hdsr_logf(std::fmax(u2f(F.depth[F.entries[s + i]]), 1e-6f)) - lmin) * scale;
+      uint32_t bk = hdsr_u32sat(y);
+      if (bk > HDSR_NUM_BUCKETS - 1u) bk = HDSR_NUM_BUCKETS - 1u;
+      bucket[i] = bk;
+      bcount[bk + 1]++;
+    }
+    for (uint32_t k = 0; k < HDSR_NUM_BUCKETS; ++k) bcount[k + 1] += bcount[k];
+    std::vector<uint32_t> ordered(c);
+    for (uint32_t i = 0; i < c; ++i) ordered[bcount[bucket[i]]++] = F.entries[s + i];
+    std::copy(ordered.begin(), ordered.end(), F.entries.begin() + s);
+    for (uint32_t cs = 0; cs < c; cs += HDSR_MAX_TILE_ENTRIES) {
+      const uint32_t cc = std::min(HDSR_MAX_TILE_ENTRIES, c - cs);
+      F.chunks.push_back(s + cs);
+      F.chunks.push_back(cc);
+      sort_segment(&F.entries[s + cs], &F.keys[s + cs], cc, F.depth.data(), lmin, scale);
+    }
+  }
+  // diagnostic: inversions of the final order against the live reference's raw depth-bit key
+  for (uint32_t m = 0; m < M; ++m) {
+    const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+    uint64_t inv = 0;
+    for (uint32_t i = 1; i < c; ++i)
+      if (F.depth[F.entries[s + i - 1]] > F.depth[F.entries[s + i]]) ++inv;
+    F.raw_order_inversions[m] = (uint32_t)std::min<uint64_t>(inv, 0xFFFFFFFFu);
+  }
+  F.stats.assign(ST_COUNT, 0);
+  F.stats[ST_REFS] = R;
+  F.stats[ST_SMALL_TILES] = (uint32_t)F.small_tiles.size();
+  F.stats[ST_LARGE_TILES] = (uint32_t)F.large_tiles.size();
+  F.stats[ST_CHUNKS] = (uint32_t)(F.chunks.size() / 2);
+  F.stats[ST_LARGE_SPLATS] = (uint32_t)F.large_list.size();
+  F.stats[ST_VISIBLE] = visible;
+  // plan: maxTotalTileRefs = min(ceil(16 N), 65535*256) (gpuTileSplatCompositor.ts:127-137)
+  const uint64_t cap = std::min<uint64_t>((uint64_t)n * 16u, 65535ull * 256ull);
+  F.stats[ST_REF_CAP] = (uint32_t)cap;
+  F.stats[ST_CAP_REFS_TRIPPED] = R > cap;
+  F.stats[ST_CAP_LARGE_SPLATS_TRIPPED] = F.large_list.size() > 65535u;
+  F.stats[ST_CAP_CHUNKS_TRIPPED] = F.chunks.size() / 2 > 1024u;
+}
+
+// composite (gpu_tile_splat_composite.wgsl:162-370), fp32, colour and depth.
+void composite32(Frame& F, bool gbuffer) {
+  const uint32_t W = F.width, H = F.height;
+  F.rgba.assign((size_t)W * H * 4, 0.0f);
+  F.out_depth.assign((size_t)W * H, 0.0f);
+  if (gbuffer) {
+    F.gb_normal.assign((size_t)W * H, 0);
+    F.gb_material.assign((size_t)W * H * 4, 0);
+  }
+  for (uint32_t ty = 0; ty < F.tiles_y; ++ty)
+    for (uint32_t tx = 0; tx < F.tiles_x; ++tx) {
+      const uint32_t m = hdsr_morton2(tx, ty);
+      const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+      for (uint32_t py = 0; py < 16; ++py)
+        for (uint32_t pxl = 0; pxl < 16; ++pxl) {
+          const uint32_t X = tx * 16 + pxl, Y = ty * 16 + py;
+          // thread (lx, ly) owns the quad at (2lx, 2ly); p00 + (0|1, 0|1)
+          const float p00x = (float)(tx * 16 + (pxl & ~1u)) + 0.5f;
+          const float p00y = (float)(ty * 16 + (py & ~1u)) + 0.5f;
+          float T = 1.0f, cr = 0, cg = 0, cb = 0;
+          float gdw = 0, gws = 0;
+          float mr = 0, mm = 0, nx = 0, ny = 0, nz = 0, gconf = 0, er = 0, eg = 0, eb = 0;
+          for (uint32_t i = 0; i < c; ++i) {
+            const uint32_t r = F.entries[s + i];
+            const uint32_t* w = &F.cache[(size_t)r * 14];
+            const float cx = u2f(w[0]), cy = u2f(w[1]);
+            const float ca = u2f(w[2]) * -0.5f, cc = u2f(w[4]) * -0.5f, cbx = -u2f(w[3]);
+            const float dx0 = p00x - cx, dy0 = p00y - cy;
+            const float dx = (pxl & 1u) ? dx0 + 1.0f : dx0;
+            const float dy = (py & 1u) ? dy0 + 1.0f : dy0;
+            // all four quad pixels <= -4 skips the splat for the whole quad (no state change),
+            // equivalent per pixel to 'valid' being false; T is unchanged either way.
+            const float power = (ca * dx * dx + cbx * dx * dy) + cc * dy * dy;
+            const float o = h2f(w[6] >> 16);
+            const float g = hdsr_expf(power);
+            const float alpha = std::fmin(0.99f, o * g);
+            const float newT = T * (1.0f - alpha);
+            const bool valid = power > -4.0f && alpha > (1.0f / 255.0f) && T >= 0.001f;
+            const float wgt = (alpha * T) * (valid ? 1.0f : 0.0f);
+            const float r_ = h2f(w[9]), g_ = h2f(w[9] >> 16), b_ = h2f(w[10]);
+            cr = cr + r_ * wgt;
+            cg = cg + g_ * wgt;
+            cb = cb + b_ * wgt;
+            if (valid) T = newT;
+            const float gw = wgt > 0.01f ? wgt : 0.0f;
+            gdw = gdw + gw * u2f(F.depth[r]);
+            gws = gws + gw;
+            if (gbuffer) {
+              mr = mr + h2f(w[5]) * gw;
+              mm = mm + h2f(w[5] >> 16) * gw;
+              // octDecode (gpu_tile_splat_composite.wgsl:143-149)
+              const float ex = h2f(w[8]), ey = h2f(w[8] >> 16);
+              float dnx = ex, dny = ey, dnz = (1.0f - std::fabs(ex)) - std::fabs(ey);
+              if (dnz < 0.0f) {
+                const float sx = dnx >= 0.0f ? 1.0f : -1.0f, sy = dny >= 0.0f ? 1.0f : -1.0f;
+                const float ax = (1.0f - std::fabs(dny)) * sx, ay = (1.0f - std::fabs(dnx)) * sy;
+                dnx = ax;
+                dny = ay;
+              }
+              const float dl = std::sqrt((dnx * dnx + dny * dny) + dnz * dnz);
+              nx = nx + (dnx / dl) * gw;
+              ny = ny + (dny / dl) * gw;
+              nz = nz + (dnz / dl) * gw;
+              gconf = gconf + h2f(w[13]) * gw;
+              er = er + h2f(w[11]) * wgt;
+              eg = eg + h2f(w[11] >> 16) * wgt;
+              eb = eb + h2f(w[12]) * wgt;
+            }
+          }
+          // The per-quad early exit (all four T < cutoff) never changes a pixel: once T < 0.001,
+          // 'valid' is false for every later splat.
+          if (X >= W || Y >= H) continue;
+          const size_t p = (size_t)Y * W + X;
+          F.rgba[p * 4 + 0] = cr + T * 0.02f;
+          F.rgba[p * 4 + 1] = cg + T * 0.02f;
+          F.rgba[p * 4 + 2] = cb + T * 0.04f;
+          F.rgba[p * 4 + 3] = 1.0f - T;
+          const float safe = std::fmax(gws, 0.0001f);
+          F.out_depth[p] = gdw / safe;
+          if (gbuffer) {
+            float ox, oy;
+            const float vx = nx, vy = ny + 0.0001f, vz = nz;
+            const float l = std::sqrt((vx * vx + vy * vy) + vz * vz);
+            oct_encode(v3<float>(vx / l, vy / l, vz / l), &ox, &oy);
+            F.gb_normal[p] = hdsr_pack2x16(ox, oy);
+            F.gb_material[p * 4 + 0] = hdsr_pack2x16(mr / safe, mm / safe);
+            F.gb_material[p * 4 + 1] = hdsr_pack2x16(er, eg);
+            F.gb_material[p * 4 + 2] = hdsr_pack2x16(eb, 0.0f);
+            F.gb_material[p * 4 + 3] = hdsr_pack2x16(gconf / safe, 0.0f);
+          }
+        }
+    }
+}
+
+// FP64 twin: projection and compositing in double over the fp32 pipeline's per-tile order.
+// Quantisation points of the reference stay (f16 colour/opacity in the cache, via f32).
+void composite64(Frame& F) {
+  const uint32_t W = F.width, H = F.height, n = F.n;
+  std::vector<Proj<double>> P(n);
+  for (uint32_t r = 0; r < n; ++r) {
+    P[r] = project_one<double>(F.S, F.U, r);
+    if (P[r].visible) {
+      P[r].opacity = h2f(hdsr_f16_bits((float)P[r].opacity));
+      P[r].r = h2f(hdsr_f16_bits((float)P[r].r));
+      P[r].g = h2f(hdsr_f16_bits((float)P[r].g));
+      P[r].b = h2f(hdsr_f16_bits((float)P[r].b));
+    }
+  }
+  F.rgba64.assign((size_t)W * H * 4, 0.0);
+  F.depth64.assign((size_t)W * H, 0.0);
+  for (uint32_t ty = 0; ty < F.tiles_y; ++ty)
+    for (uint32_t tx = 0; tx < F.tiles_x; ++tx) {
+      const uint32_t m = hdsr_morton2(tx, ty);
+      const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+      for (uint32_t py = 0; py < 16; ++py)
+        for (uint32_t pxl = 0; pxl < 16; ++pxl) {
+          const uint32_t X = tx * 16 + pxl, Y = ty * 16 + py;
+          if (X >= W || Y >= H) continue;
+          const double qx = (double)X + 0.5, qy = (double)Y + 0.5;
+          double T = 1, cr = 0, cg = 0, cb = 0, gdw = 0, gws = 0;
+          for (uint32_t i = 0; i < c; ++i) {
+            const Proj<double>& p = P[F.entries[s + i]];
+            if (!p.visible) continue;
+            const double dx = qx - p.px, dy = qy - p.py;
+            const double power = -0.5 * p.k0 * dx * dx - p.k1 * dx * dy - 0.5 * p.k2 * dy * dy;
+            const double alpha = std::fmin(0.99, p.opacity * std::exp(power));
+            const bool valid = power > -4.0 && alpha > 1.0 / 255.0 && T >= 0.001;
+            if (!valid) continue;
+            const double w = alpha * T;
+            cr += p.r * w;
+            cg += p.g * w;
+            cb += p.b * w;
+            T *= 1.0 - alpha;
+            const double gw = w > 0.01 ? w : 0.0;
+            gdw += gw * p.depth;
+            gws += gw;
+          }
+          const size_t q = (size_t)Y * W + X;
+          F.rgba64[q * 4 + 0] = cr + T * 0.02;
+          F.rgba64[q * 4 + 1] = cg + T * 0.02;
+          F.rgba64[q * 4 + 2] = cb + T * 0.04;
+          F.rgba64[q * 4 + 3] = 1.0 - T;
+          F.depth64[q] = gdw / std::fmax(gws, 0.0001);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+struct orc_splats {
+  const float* pos;
+  const float* logscale;
+  const float* rot;
+  const float* opacity;
+  const uint32_t* sorted;
+  const uint32_t* material;
+  const float* normals;
+  uint32_t normals_len;
+  const float* shdata;
+  uint32_t n;
+  uint32_t sh_degree;
+};
+
+enum {
+  ORC_CACHE = 0,
+  ORC_DEPTH,
+  ORC_LARGE_LIST,
+  ORC_TILE_COUNTS,
+  ORC_TILE_OFFSETS,
+  ORC_ENTRIES_CANON,
+  ORC_SMALL_TILES,
+  ORC_LARGE_TILES,
+  ORC_LARGE_BASES,
+  ORC_CHUNKS,
+  ORC_ENTRIES,
+  ORC_KEYS,
+  ORC_RGBA,
+  ORC_OUT_DEPTH,
+  ORC_GB_NORMAL,
+  ORC_GB_MATERIAL,
+  ORC_STATS,
+  ORC_RGBA64,
+  ORC_DEPTH64,
+  ORC_RAW_INVERSIONS,
+  ORC_NUM
+};
+
+#define ORC_FLAG_GBUFFER 1u
+#define ORC_FLAG_FP64 2u
+#define ORC_FLAG_NO_COMPOSITE 4u
+
+// Runs the whole path for one frame.  Returns an opaque handle (NULL on bad input).
+void* orc_run(const orc_splats* s, const void* uniforms256, uint32_t width, uint32_t height, uint32_t flags) {
+  if (!s || !uniforms256) return nullptr;
+  Frame* F = new Frame();
+  F->U = parse_uniforms(uniforms256);
+  F->S = Splats{s->pos, s->logscale, s->rot, s->opacity, s->sorted, s->material, s->normals, s->normals_len,
+                s->shdata, s->n, s->sh_degree};
+  F->n = s->n;
+  F->width = width;
+  F->height = height;
+  F->tiles_x = F->U.gx;
+  F->tiles_y = F->U.gy;
+  F->morton_count = (F->tiles_x && F->tiles_y) ? hdsr_morton2(F->tiles_x - 1, F->tiles_y - 1) + 1 : 0;
+  F->cache.assign((size_t)F->n * 14, 0);
+  F->depth.assign(F->n, 0);
+  for (uint32_t r = 0; r < F->n; ++r) {
+    const Proj<float> P = project_one<float>(F->S, F->U, r);
+    pack_record(P, &F->cache[(size_t)r * 14], &F->depth[r]);
+  }
+  bin_and_sort(*F);
+  if (!(flags & ORC_FLAG_NO_COMPOSITE)) composite32(*F, (flags & ORC_FLAG_GBUFFER) != 0);
+  if (flags & ORC_FLAG_FP64) composite64(*F);
+  return F;
+}
+
+// Bytes held for output 'which' (0 if absent).
+uint64_t orc_size(void* h, int which) {
+  Frame* F = static_cast<Frame*>(h);
+  switch (which) {
+    case ORC_CACHE: return F->cache.size() * 4;
+    case ORC_DEPTH: return F->depth.size() * 4;
+    case ORC_LARGE_LIST: return F->large_list.size() * 4;
+    case ORC_TILE_COUNTS: return F->tile_counts.size() * 4;
+    case ORC_TILE_OFFSETS: return F->tile_offsets.size() * 4;
+    case ORC_ENTRIES_CANON: return F->entries_canon.size() * 4;
+    case ORC_SMALL_TILES: return F->small_tiles.size() * 4;
+    case ORC_LARGE_TILES: return F->large_tiles.size() * 4;
+    case ORC_LARGE_BASES: return F->large_bases.size() * 4;
+    case ORC_CHUNKS: return F->chunks.size() * 4;
+    case ORC_ENTRIES: return F->entries.size() * 4;
+    case ORC_KEYS: return F->keys.size() * 4;
+    case ORC_RGBA: return F->rgba.size() * 4;
+    case ORC_OUT_DEPTH: return F->out_depth.size() * 4;
+    case ORC_GB_NORMAL: return F->gb_normal.size() * 4;
+    case ORC_GB_MATERIAL: return F->gb_material.size() * 4;
+    case ORC_STATS: return F->stats.size() * 4;
+    case ORC_RGBA64: return F->rgba64.size() * 8;
+    case ORC_DEPTH64: return F->depth64.size() * 8;
+    case ORC_RAW_INVERSIONS: return F->raw_order_inversions.size() * 4;
+    default: return 0;
+  }
+}
+
+int orc_get(void* h, int which, void* dst, uint64_t bytes) {
+  Frame* F = static_cast<Frame*>(h);
+  const uint64_t n = orc_size(h, which);
+  if (bytes < n) return -1;
+  const void* src = nullptr;
+  switch (which) {
+    case ORC_CACHE: src = F->cache.data(); break;
+    case ORC_DEPTH: src = F->depth.data(); break;
+    case ORC_LARGE_LIST: src = F->large_list.data(); break;
+    case ORC_TILE_COUNTS: src = F->tile_counts.data(); break;
+    case ORC_TILE_OFFSETS: src = F->tile_offsets.data(); break;
+    case ORC_ENTRIES_CANON: src = F->entries_canon.data(); break;
+    case ORC_SMALL_TILES: src = F->small_tiles.data(); break;
+    case ORC_LARGE_TILES: src = F->large_tiles.data(); break;
+    case ORC_LARGE_BASES: src = F->large_bases.data(); break;
+    case ORC_CHUNKS: src = F->chunks.data(); break;
+    case ORC_ENTRIES: src = F->entries.data(); break;
+    case ORC_KEYS: src = F->keys.data(); break;
+    case ORC_RGBA: src = F->rgba.data(); break;
+    case ORC_OUT_DEPTH: src = F->out_depth.data(); break;
+    case ORC_GB_NORMAL: src = F->gb_normal.data(); break;
+    case ORC_GB_MATERIAL: src = F->gb_material.data(); break;
+    case ORC_STATS: src = F->stats.data(); break;
+    case ORC_RGBA64: src = F->rgba64.data(); break;
+    case ORC_DEPTH64: src = F->depth64.data(); break;
+    case ORC_RAW_INVERSIONS: src = F->raw_order_inversions.data(); break;
+    default: return -2;
+  }
+  if (n) memcpy(dst, src, n);
+  return 0;
+}
+
+void orc_free(void* h) { delete static_cast<Frame*>(h); }
+
+// Scalar primitives, exported so tests can pin them against libm / float64.
+float orc_expf(float x) { return hdsr_expf(x); }
+float orc_logf(float x) { return hdsr_logf(x); }
+uint32_t orc_f16_bits(float x) { return hdsr_f16_bits(x); }
+float orc_f16_to_f32(uint32_t h) { return hdsr_f16_to_f32(h); }
+uint32_t orc_u32sat(float x) { return hdsr_u32sat(x); }
+uint32_t orc_morton2(uint32_t x, uint32_t y) { return hdsr_morton2(x, y); }
+
+void orc_expf_n(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_expf(x[i]);
+}
+void orc_logf_n(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_logf(x[i]);
+}
+void orc_f16_bits_n(const float* x, uint32_t* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_f16_bits(x[i]);
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+"""ctypes wrapper of the CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the benchmark's CPU-baseline leg,
+never by the product path.  See hdsr_oracle.cpp for what each output restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+
+OUT = {
+    "cache": 0, "depth": 1, "large_list": 2, "tile_counts": 3, "tile_offsets": 4, "entries_canon": 5,
+    "small_tiles": 6, "large_tiles": 7, "large_bases": 8, "chunks": 9, "entries": 10, "keys": 11, "rgba": 12,
+    "out_depth": 13, "gb_normal": 14, "gb_material": 15, "stats": 16, "rgba64": 17, "depth64": 18,
+    "raw_inversions": 19,
+}
+STATS = ["refs", "small_tiles", "large_tiles", "chunks", "large_splats", "visible", "cap_refs_tripped",
+         "cap_large_splats_tripped", "cap_chunks_tripped", "ref_cap"]
+FLAG_GBUFFER, FLAG_FP64, FLAG_NO_COMPOSITE = 1, 2, 4
+
+
+class OrcSplats(C.Structure):
+    _fields_ = [("pos", C.c_void_p), ("logscale", C.c_void_p), ("rot", C.c_void_p), ("opacity", C.c_void_p),
+                ("sorted", C.c_void_p), ("material", C.c_void_p), ("normals", C.c_void_p),
+                ("normals_len", C.c_uint32), ("shdata", C.c_void_p), ("n", C.c_uint32), ("sh_degree", C.c_uint32)]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    lib.orc_run.restype = C.c_void_p
+    lib.orc_run.argtypes = [C.POINTER(OrcSplats), C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.orc_size.restype = C.c_uint64
+    lib.orc_size.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_get.restype = C.c_int
+    lib.orc_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]
+    lib.orc_free.argtypes = [C.c_void_p]
+    lib.orc_free.restype = None
+    for name in ("orc_expf", "orc_logf"):
+        getattr(lib, name).restype = C.c_float
+        getattr(lib, name).argtypes = [C.c_float]
+    for name in ("orc_expf_n", "orc_logf_n"):
+        getattr(lib, name).restype = None
+        getattr(lib, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+    lib.orc_f16_bits_n.restype = None
+    lib.orc_f16_bits_n.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+    lib.orc_f16_bits.restype = C.c_uint32
+    lib.orc_f16_bits.argtypes = [C.c_float]
+    lib.orc_f16_to_f32.restype = C.c_float
+    lib.orc_f16_to_f32.argtypes = [C.c_uint32]
+    lib.orc_u32sat.restype = C.c_uint32
+    lib.orc_u32sat.argtypes = [C.c_float]
+    lib.orc_morton2.restype = C.c_uint32
+    lib.orc_morton2.argtypes = [C.c_uint32, C.c_uint32]
+    _lib = lib
+    return lib
+
+
+_DTYPES = {"rgba": np.float32, "out_depth": np.float32, "rgba64": np.float64, "depth64": np.float64}
+
+
+class OracleFrame:
+    """All outputs of one oracle frame as numpy arrays."""
+
+    def __init__(self, scene_arrays: dict, uniforms: np.ndarray, width: int, height: int, flags: int = 0):
+        lib = load()
+        self._arrays = {k: (v if v is None or k == "sh_degree" else np.ascontiguousarray(v))
+                        for k, v in scene_arrays.items()}
+        a = self._arrays
+        nrm = a.get("normals")
+        s = OrcSplats(a["positions"].ctypes.data, a["scales"].ctypes.data, a["rotations"].ctypes.data,
+                      a["opacities"].ctypes.data, a["sorted"].ctypes.data, a["materials"].ctypes.data,
+                      None if nrm is None else nrm.ctypes.data, 0 if nrm is None else int(nrm.size),
+                      a["sh_data"].ctypes.data, int(a["positions"].shape[0]), int(a["sh_degree"]))
+        u = np.ascontiguousarray(uniforms, dtype=np.float32)
+        h = lib.orc_run(C.byref(s), u.ctypes.data, width, height, flags)
+        if not h:
+            raise RuntimeError("oracle rejected the input")
+        self.out = {}
+        try:
+            for name, which in OUT.items():
+                n = lib.orc_size(h, which)
+                dt = _DTYPES.get(name, np.uint32)
+                arr = np.empty(n // np.dtype(dt).itemsize, dt)
+                if n:
+                    lib.orc_get(h, which, arr.ctypes.data, n)
+                self.out[name] = arr
+        finally:
+            lib.orc_free(h)
+        self.stats = dict(zip(STATS, self.out["stats"].tolist()))
+        self.width, self.height = width, height
+
+    def __getitem__(self, k):
+        return self.out[k]
+
+
+def scene_arrays(scene, sorted_indices) -> dict:
+    return {
+        "positions": np.ascontiguousarray(scene.positions, np.float32),
+        "scales": np.ascontiguousarray(scene.scales, np.float32),
+        "rotations": np.ascontiguousarray(scene.rotations, np.float32),
+        "opacities": np.ascontiguousarray(scene.opacities, np.float32),
+        "materials": np.ascontiguousarray(scene.materials, np.uint32),
+        "normals": None if scene.normals is None else np.ascontiguousarray(scene.normals, np.float32),
+        "sh_data": np.ascontiguousarray(scene.sh_data(), np.float32),
+        "sorted": np.ascontiguousarray(sorted_indices, np.uint32),
+        "sh_degree": scene.sh_degree,
+    }
+
+
+def run(scene, sorted_indices, uniforms, width, height, flags=0) -> OracleFrame:
+    return OracleFrame(scene_arrays(scene, sorted_indices), uniforms, width, height, flags)
