@@ -146,6 +146,7 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   }
   const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
   c->eng.set_min_capacity(base_cap);
+  c->eng.set_profile((flags & HDSR_PROFILE) != 0);
   *out = c.release();
   return HDSR_OK;
 }
@@ -329,6 +330,12 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->cap_large_splats_tripped = s.large_splats > 65535u;
   o->cap_chunks_tripped = s.chunks > 1024u;
   return HDSR_OK;
+}
+
+int hdsr_stage_times(hdsr_ctx* c, float* ms, uint32_t* renders) {
+  if (!c || !ms || !renders) return HDSR_E_INVALID_ARG;
+  if (!(c->flags & HDSR_PROFILE)) return fail(c, HDSR_E_STATE, "create the context with HDSR_PROFILE");
+  return c->eng.stage_times(ms, renders);
 }
 
 int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {

@@ -48,6 +48,10 @@ class Engine {
   int grow_for_overflow();
   void set_min_capacity(uint64_t cap) { min_cap_ = cap; }
 
+  // Per-stage HIP events (one set per run, kept in a ring until harvested).
+  void set_profile(bool on) { profile_ = on; }
+  int stage_times(float* ms, uint32_t* runs);
+
   const Bufs& bufs() const { return b_; }
   const Layout& last_layout() const { return lay_; }
   bool debug_buffers() const { return b_.keys20 != nullptr; }
@@ -78,6 +82,16 @@ class Engine {
   DevStats* hstats_ = nullptr;
   hipEvent_t done_ev_ = nullptr;
   bool pending_ = false;
+  // profiling
+  static constexpr int kStages = HDSR_STAGE_COUNT;
+  static constexpr int kProfRing = 128;
+  bool profile_ = false;
+  std::vector<hipEvent_t> prof_ev_;  // kProfRing * (kStages + 1)
+  int prof_head_ = 0, prof_count_ = 0;
+  double prof_acc_[kStages] = {};
+  uint32_t prof_runs_ = 0;
+  hipEvent_t* prof_slot();
+  void prof_harvest_one();
 };
 
 // Parses the 256-byte uniform block into a FrameDesc's camera fields

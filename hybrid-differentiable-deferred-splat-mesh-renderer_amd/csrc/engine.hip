@@ -86,6 +86,9 @@ void Engine::release() {
   hstats_ = nullptr;
   if (done_ev_) (void)hipEventDestroy(done_ev_);
   done_ev_ = nullptr;
+  for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
+  prof_ev_.clear();
+  prof_head_ = prof_count_ = 0;
   n_splat_ = n_blocks_ = n_scan4_ = n_cap_ = n_hist_ = n_hist_tmp_ = 0;
   n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = 0;
 }
@@ -279,18 +282,29 @@ int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool
     b.keys20 = nullptr;
     b.entries_canon = nullptr;
   }
+  hipEvent_t* ev = profile_ ? prof_slot() : nullptr;
   launch_clear_stats(b, stream);
+  if (ev) (void)hipEventRecord(ev[0], stream);
   launch_project(b, stream);
+  if (ev) (void)hipEventRecord(ev[1], stream);
   launch_scan_blocks(b, stream);
+  if (ev) (void)hipEventRecord(ev[2], stream);
   launch_emit_pairs(b, stream);
+  if (ev) (void)hipEventRecord(ev[3], stream);
   launch_group_pairs(b, stream);
+  if (ev) (void)hipEventRecord(ev[4], stream);
   launch_tile_ranges(b, stream);
+  if (ev) (void)hipEventRecord(ev[5], stream);
   launch_classify(b, stream);
+  if (ev) (void)hipEventRecord(ev[6], stream);
   if (debug && b.entries_canon)
     (void)hipMemcpyAsync(b.entries_canon, b.entries, (size_t)b.cap * 4, hipMemcpyDeviceToDevice, stream);
   launch_bucket(b, stream);
+  if (ev) (void)hipEventRecord(ev[7], stream);
   launch_segment_sort(b, stream);
+  if (ev) (void)hipEventRecord(ev[8], stream);
   launch_composite(b, stream, gbuffer);
+  if (ev) (void)hipEventRecord(ev[9], stream);
   if (hipMemcpyAsync(hstats_, b.stats, sizeof(DevStats), hipMemcpyDeviceToHost, stream) != hipSuccess ||
       hipEventRecord(done_ev_, stream) != hipSuccess) {
     err = "stats copy failed";
@@ -324,6 +338,45 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
     err = std::string("kernel launch failed: ") + hipGetErrorString(le);
     return HDSR_E_HIP;
   }
+  return 0;
+}
+
+hipEvent_t* Engine::prof_slot() {
+  if (prof_ev_.empty()) {
+    prof_ev_.resize((size_t)kProfRing * (kStages + 1));
+    for (hipEvent_t& e : prof_ev_)
+      if (hipEventCreate(&e) != hipSuccess) {
+        prof_ev_.clear();
+        profile_ = false;
+        return nullptr;
+      }
+  }
+  if (prof_count_ == kProfRing) prof_harvest_one();
+  const int slot = (prof_head_ + prof_count_) % kProfRing;
+  ++prof_count_;
+  return &prof_ev_[(size_t)slot * (kStages + 1)];
+}
+
+void Engine::prof_harvest_one() {
+  hipEvent_t* ev = &prof_ev_[(size_t)prof_head_ * (kStages + 1)];
+  (void)hipEventSynchronize(ev[kStages]);
+  for (int s = 0; s < kStages; ++s) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev[s], ev[s + 1]) == hipSuccess) prof_acc_[s] += ms;
+  }
+  ++prof_runs_;
+  prof_head_ = (prof_head_ + 1) % kProfRing;
+  --prof_count_;
+}
+
+int Engine::stage_times(float* ms, uint32_t* runs) {
+  while (prof_count_ > 0) prof_harvest_one();
+  for (int s = 0; s < kStages; ++s) {
+    ms[s] = prof_runs_ ? (float)(prof_acc_[s] / prof_runs_) : 0.0f;
+    prof_acc_[s] = 0.0;
+  }
+  *runs = prof_runs_;
+  prof_runs_ = 0;
   return 0;
 }
 

@@ -24,6 +24,8 @@ HDSR_REF_CAPS = 2
 HDSR_GBUFFER = 4
 HDSR_DEBUG = 8
 HDSR_OUT_RGBA16 = 16
+HDSR_PROFILE = 32
+STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "sort", "composite"]
 
 HDSR_MEM_HOST = 0
 HDSR_MEM_DEVICE = 1
@@ -84,6 +86,7 @@ SIGNATURES = [
     ("hdsr_debug_size", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     ("hdsr_debug_readback", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
     ("hdsr_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("hdsr_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     ("hdsr_last_error", C.c_char_p, [C.c_void_p]),
     ("hdsr_version", C.c_char_p, []),
 ]

@@ -115,6 +115,13 @@ class Resources:
         check(self._lib.hdsr_get_stats(self.handle, C.byref(s)), self.handle)
         return {name: getattr(s, name) for name, _ in _lib.Stats._fields_ if not name.startswith("reserved")}
 
+    def stage_times(self) -> tuple[dict, int]:
+        """Mean GPU milliseconds per render of each pipeline stage (needs profile=True)."""
+        ms = (C.c_float * len(_lib.STAGES))()
+        n = C.c_uint32()
+        check(self._lib.hdsr_stage_times(self.handle, ms, C.byref(n)), self.handle)
+        return dict(zip(_lib.STAGES, [float(x) for x in ms])), int(n.value)
+
     def outputs(self) -> dict:
         o = _lib.Outputs()
         check(self._lib.hdsr_get_outputs(self.handle, C.byref(o)), self.handle)
@@ -138,7 +145,7 @@ class Resources:
 
 
 def createTileSplatCompositor(device, plan: dict, options: dict | None = None, *, gbuffer=False, debug=False,
-                              rgba16=False) -> Resources:
+                              rgba16=False, profile=False) -> Resources:
     options = options or {}
     flags = 0
     if options.get("f16"):
@@ -151,6 +158,8 @@ def createTileSplatCompositor(device, plan: dict, options: dict | None = None, *
         flags |= _lib.HDSR_DEBUG
     if rgba16 or options.get("rgba16"):
         flags |= _lib.HDSR_OUT_RGBA16
+    if profile or options.get("profile"):
+        flags |= _lib.HDSR_PROFILE
     return Resources(plan, flags)
 
 

@@ -87,6 +87,15 @@ def pack_sh_data(colors, sh=None, emissive=None) -> np.ndarray:
     return out.reshape(-1)
 
 
+def sh_from_ply_f_rest(f_rest: np.ndarray) -> np.ndarray:
+    """3DGS PLY f_rest_* (channel-major: all R coefficients, then G, then B) -> (N, K, 3)
+    coefficient-major, RGB-interleaved (the splat_coeff_rgb layout of the SH sidecar)."""
+    f = np.asarray(f_rest, np.float32)
+    n, m = f.shape
+    k = m // 3
+    return np.ascontiguousarray(f.reshape(n, 3, k).transpose(0, 2, 1))
+
+
 def sort_back_to_front(positions: np.ndarray, view16: np.ndarray) -> np.ndarray:
     """cpuDepthSort = sortSplatIdsBackToFront (src/splatSort.ts:32-55): ascending view-space z
     (farthest first), ties by id; depth evaluated in float64 as the JS does."""

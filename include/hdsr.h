@@ -61,6 +61,7 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
 #define HDSR_GBUFFER 4u         /* also write the oct-normal and material/emissive/confidence words */
 #define HDSR_DEBUG 8u           /* keep the pre-sort entry list and the 20-bit keys for readback */
 #define HDSR_OUT_RGBA16 16u     /* also write binary16 RGBA (the reference's rgba16float texture) */
+#define HDSR_PROFILE 32u        /* record a HIP event between pipeline stages of every render */
 
 typedef struct hdsr_ctx hdsr_ctx;
 
@@ -170,6 +171,23 @@ typedef struct hdsr_stats {
   uint32_t reserved[2];
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
+
+/* Per-stage GPU time, from HIP events recorded on the render stream between stages (the
+ * reference timestamps no compute pass; its CPU stage timing is src/splatRenderer.ts:226-283).
+ * Needs HDSR_PROFILE.  Waits for outstanding renders, writes the mean milliseconds per render of
+ * each stage into ms[0..HDSR_STAGE_COUNT) and the number of renders averaged into *renders, then
+ * resets the accumulators. */
+#define HDSR_STAGE_PROJECT 0      /* project_splats + classify_large_splats + per-splat counts */
+#define HDSR_STAGE_SCAN 1         /* hierarchical scan of the per-workgroup counts */
+#define HDSR_STAGE_EMIT 2         /* pair scatter (scatter_tile_refs + large scatter) */
+#define HDSR_STAGE_GROUP 3        /* radix grouping on the Morton key (all passes) */
+#define HDSR_STAGE_RANGES 4       /* tile offsets / counts */
+#define HDSR_STAGE_CLASSIFY 5     /* classify_tiles */
+#define HDSR_STAGE_BUCKET 6       /* bucket_sort (large tiles) */
+#define HDSR_STAGE_SORT 7         /* tile_depth_sort + chunk_sort */
+#define HDSR_STAGE_COMPOSITE 8    /* composite */
+#define HDSR_STAGE_COUNT 9
+int hdsr_stage_times(hdsr_ctx* ctx, float* ms, uint32_t* renders);
 
 const char* hdsr_last_error(hdsr_ctx* ctx);
 const char* hdsr_version(void);

@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <vector>
 
 #include "../hybrid-differentiable-deferred-splat-mesh-renderer_amd/csrc/hdsr_fmath.h"
@@ -610,8 +611,10 @@ void bin_and_sort(Frame& F) {
   F.stats[ST_CAP_CHUNKS_TRIPPED] = F.chunks.size() / 2 > 1024u;
 }
 
-// composite (gpu_tile_splat_composite.wgsl:162-370), fp32, colour and depth.
-void composite32(Frame& F, bool gbuffer) {
+// composite (gpu_tile_splat_composite.wgsl:162-370), fp32, colour and depth.  Only tile rows
+// ty % row_step == 0 are composited (row_step 1 = the whole frame; larger values give a bounded
+// timing sample).
+void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
   const uint32_t W = F.width, H = F.height;
   F.rgba.assign((size_t)W * H * 4, 0.0f);
   F.out_depth.assign((size_t)W * H, 0.0f);
@@ -619,7 +622,7 @@ void composite32(Frame& F, bool gbuffer) {
     F.gb_normal.assign((size_t)W * H, 0);
     F.gb_material.assign((size_t)W * H * 4, 0);
   }
-  for (uint32_t ty = 0; ty < F.tiles_y; ++ty)
+  for (uint32_t ty = 0; ty < F.tiles_y; ty += row_step)
     for (uint32_t tx = 0; tx < F.tiles_x; ++tx) {
       const uint32_t m = hdsr_morton2(tx, ty);
       const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
@@ -802,6 +805,47 @@ enum {
 #define ORC_FLAG_GBUFFER 1u
 #define ORC_FLAG_FP64 2u
 #define ORC_FLAG_NO_COMPOSITE 4u
+
+static double now_s() {
+  timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+// As orc_run, compositing only every row_step-th tile row, and reporting seconds spent in
+// (projection, binning + per-tile sort, composite) into times[0..3).
+void* orc_run_timed(const orc_splats* s, const void* uniforms256, uint32_t width, uint32_t height, uint32_t row_step,
+                    double* times) {
+  if (!s || !uniforms256 || row_step == 0) return nullptr;
+  Frame* F = new Frame();
+  F->U = parse_uniforms(uniforms256);
+  F->S = Splats{s->pos, s->logscale, s->rot, s->opacity, s->sorted, s->material, s->normals, s->normals_len,
+                s->shdata, s->n, s->sh_degree};
+  F->n = s->n;
+  F->width = width;
+  F->height = height;
+  F->tiles_x = F->U.gx;
+  F->tiles_y = F->U.gy;
+  F->morton_count = (F->tiles_x && F->tiles_y) ? hdsr_morton2(F->tiles_x - 1, F->tiles_y - 1) + 1 : 0;
+  double t0 = now_s();
+  F->cache.assign((size_t)F->n * 14, 0);
+  F->depth.assign(F->n, 0);
+  for (uint32_t r = 0; r < F->n; ++r) {
+    const Proj<float> P = project_one<float>(F->S, F->U, r);
+    pack_record(P, &F->cache[(size_t)r * 14], &F->depth[r]);
+  }
+  double t1 = now_s();
+  bin_and_sort(*F);
+  double t2 = now_s();
+  composite32(*F, false, row_step);
+  double t3 = now_s();
+  if (times) {
+    times[0] = t1 - t0;
+    times[1] = t2 - t1;
+    times[2] = t3 - t2;
+  }
+  return F;
+}
 
 // Runs the whole path for one frame.  Returns an opaque handle (NULL on bad input).
 void* orc_run(const orc_splats* s, const void* uniforms256, uint32_t width, uint32_t height, uint32_t flags) {

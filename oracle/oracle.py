@@ -128,3 +128,25 @@ def scene_arrays(scene, sorted_indices) -> dict:
 
 def run(scene, sorted_indices, uniforms, width, height, flags=0) -> OracleFrame:
     return OracleFrame(scene_arrays(scene, sorted_indices), uniforms, width, height, flags)
+
+
+def time_frame(scene, sorted_indices, uniforms, width, height, row_step: int) -> dict:
+    """Times one single-threaded oracle frame, compositing every row_step-th tile row; returns
+    seconds per phase and the whole-frame estimate (composite time scaled by row_step)."""
+    lib = load()
+    lib.orc_run_timed.restype = C.c_void_p
+    lib.orc_run_timed.argtypes = [C.POINTER(OrcSplats), C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.POINTER(C.c_double)]
+    a = scene_arrays(scene, sorted_indices)
+    s = OrcSplats(a["positions"].ctypes.data, a["scales"].ctypes.data, a["rotations"].ctypes.data,
+                  a["opacities"].ctypes.data, a["sorted"].ctypes.data, a["materials"].ctypes.data, None, 0,
+                  a["sh_data"].ctypes.data, int(a["positions"].shape[0]), int(a["sh_degree"]))
+    u = np.ascontiguousarray(uniforms, dtype=np.float32)
+    t = (C.c_double * 3)()
+    h = lib.orc_run_timed(C.byref(s), u.ctypes.data, width, height, row_step, t)
+    if not h:
+        raise RuntimeError("oracle rejected the input")
+    lib.orc_free(h)
+    proj, binsort, comp = t[0], t[1], t[2]
+    return {"project_s": proj, "bin_sort_s": binsort, "composite_sample_s": comp, "row_step": row_step,
+            "frame_s_est": proj + binsort + comp * row_step, "measured_s": proj + binsort + comp}

@@ -1,0 +1,134 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every symbol the header
+declares, the plan arithmetic, and the reference's own known-answer tests for the host helpers
+(cpuDepthSort orders, the viewer flip, buffer layouts, material packing, SH coefficient layout)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from hdsr import _lib, camera, compositor, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "hdsr.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(hdsr_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(lib, n), f"missing export {n}"
+    declared = {n for n, _, _ in _lib.SIGNATURES}
+    assert declared == set(names), f"binding table differs from header: {declared ^ set(names)}"
+    assert _lib.load().hdsr_version().startswith(b"hdsr")
+
+
+def test_plan_matches_reference_arithmetic():
+    # planTileSplatCompositor (gpuTileSplatCompositor.ts:109-139)
+    cases = [((640, 480, 50_000), (40, 30, 1200, 1720, 800_000)),
+             ((1920, 1080, 1_000_000), (120, 68, 8160, 13600, 16_000_000)),
+             ((3840, 2160, 6_000_000), (240, 135, 32400, 54398, 65535 * 256)),
+             ((1280, 720, 500_000), (80, 45, 3600, 6390, 8_000_000))]
+    for (w, h, n), exp in cases:
+        p = compositor.planTileSplatCompositor({"viewportWidth": w, "viewportHeight": h, "tileSizePx": 16,
+                                                "splatCount": n})
+        assert (p["tileColumns"], p["tileRows"], p["tileCount"], p["mortonTileCount"], p["maxTotalTileRefs"]) == exp
+    p = compositor.planTileSplatCompositor({"viewportWidth": 100, "viewportHeight": 50, "tileSizePx": 16,
+                                            "splatCount": 10, "tileEntryMultiplier": 2.5})
+    assert p["maxTotalTileRefs"] == 25
+
+
+def test_plan_rejects_bad_input():
+    with pytest.raises(_lib.HdsrError):
+        compositor.planTileSplatCompositor({"viewportWidth": 0, "viewportHeight": 10, "tileSizePx": 16,
+                                            "splatCount": 1})
+
+
+def test_create_rejects_unsupported_options_without_gpu_work():
+    p = compositor.planTileSplatCompositor({"viewportWidth": 64, "viewportHeight": 64, "tileSizePx": 16,
+                                            "splatCount": 1})
+    h = C.c_void_p()
+    rc = _lib.load().hdsr_create(C.byref(compositor._plan_struct(p)), _lib.HDSR_F16_COMPOSITE, C.byref(h))
+    assert rc == _lib.HDSR_E_UNSUPPORTED and not h.value
+    bad = dict(p, tileSizePx=8)
+    rc = _lib.load().hdsr_create(C.byref(compositor._plan_struct(bad)), 0, C.byref(h))
+    assert rc == _lib.HDSR_E_INVALID_ARG
+
+
+IDENT = np.eye(4, dtype=np.float32).T.reshape(16)
+
+
+def test_cpu_depth_sort_kats():
+    # tests/cpuDepthSort.test.ts:18-84
+    pos = np.array([[0, 0, -2], [0, 0, -9], [0, 0, -5], [0, 0, -0.5]], np.float32)
+    assert scenes.sort_back_to_front(pos, IDENT).tolist() == [1, 2, 0, 3]
+    trans = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, -5, 1], np.float32)
+    pos = np.array([[0, 0, 0], [3, 2, 2], [-1, 1, -4]], np.float32)
+    assert scenes.sort_back_to_front(pos, trans).tolist() == [2, 0, 1]
+    xdepth = np.array([1, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1], np.float32)
+    pos = np.array([[-4, 0, 0], [-1, 0, 0], [-6, 0, 0]], np.float32)
+    assert scenes.sort_back_to_front(pos, xdepth).tolist() == [2, 0, 1]
+    pos = np.array([[0, 0, -4], [0, 0, -6], [1, 0, -4], [2, 0, -4]], np.float32)
+    assert scenes.sort_back_to_front(pos, IDENT).tolist() == [1, 0, 2, 3]
+    with pytest.raises(ValueError):
+        scenes.sort_back_to_front(np.array([[np.nan, 0, 0]], np.float32), IDENT)
+
+
+def test_viewer_flip_kat():
+    # tests/renderer/realSmokeScene.test.mjs:149-163
+    vp = camera.view_projection(IDENT, IDENT)
+    assert vp.tolist() == [1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]
+
+
+def test_camera_conventions():
+    cam = camera.Camera((0, 0, 3), (0, 0, 0), 525, 525, 640, 480)
+    V = camera.col_major(cam.view)
+    assert np.allclose(V @ [0, 0, 0, 1], [0, 0, -3, 1])       # looks down -z
+    assert np.allclose(V @ [1, 0, 0, 1], [1, 0, -3, 1])       # +x right
+    P = camera.col_major(cam.proj)
+    clip = P @ [0, 0, -camera.NEAR, 1]
+    assert abs(clip[2] / clip[3]) < 1e-6                      # near plane -> depth 0
+    clip = P @ [0, 0, -camera.FAR, 1]
+    assert abs(clip[2] / clip[3] - 1) < 1e-6                  # far plane -> depth 1
+    plan = compositor.planTileSplatCompositor({"viewportWidth": 640, "viewportHeight": 480, "tileSizePx": 16,
+                                               "splatCount": 7})
+    u = camera.frame_uniforms_for(cam, plan, 3)
+    assert u.nbytes == compositor.TILE_SPLAT_FRAME_UNIFORM_BYTES == 256
+    w = u.view(np.uint32)
+    assert (u[16], u[17], u[18]) == (640, 480, 16)
+    assert (w[20], w[21], w[22], w[23], w[25]) == (40, 30, 7, plan["maxTotalTileRefs"], 3)
+    assert (u[48], u[49]) == (525.0, 525.0)                  # focal = W * P[0] / 2
+    assert u[52] == 1 and u[57] == 1 and u[62] == 1 and u[55] == 0  # identity normalMatrix, vec4 columns
+    np.testing.assert_array_equal(u[28:31], np.float32([0, 0, 3]))
+
+
+def test_material_and_sh_packing_kats():
+    # splats.test.ts:375-431: default material pack2x16float(0.75, 0), shData stride 3 + 3K + 3
+    m = scenes.pack_material(3)
+    assert np.all(m == 0x3A00)
+    # truncating f32ToF16 flushes below 2^-14 and truncates the mantissa (splats.ts:431-442)
+    assert scenes.f32_to_f16_trunc(np.float32([1e-5]))[0] == 0
+    assert scenes.f32_to_f16_trunc(np.float32([1.0009765]))[0] == 0x3C00
+    assert scenes.f32_to_f16_trunc(np.float32([70000.0]))[0] == 0x7C00
+    colors = np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6]], np.float32)
+    sd = scenes.pack_sh_data(colors)
+    assert sd.tolist() == pytest.approx([0.1, 0.2, 0.3, 0, 0, 0, 0.4, 0.5, 0.6, 0, 0, 0])
+    sh = np.arange(2 * 3 * 3, dtype=np.float32).reshape(2, 3, 3)
+    em = np.ones((2, 3), np.float32)
+    sd = scenes.pack_sh_data(colors, sh, em).reshape(2, -1)
+    assert sd.shape[1] == 3 + 9 + 3
+    np.testing.assert_array_equal(sd[1, 3:12], sh[1].reshape(-1))
+    np.testing.assert_array_equal(sd[:, 12:], em)
+
+
+def test_ply_f_rest_to_coefficient_major_kat():
+    # preprocessing/tests/test_sh_payload_export.py:15-33: PLY f_rest is channel-major
+    f_rest = np.array([[10, 11, 12, 20, 21, 22, 30, 31, 32]], np.float32)
+    sh = scenes.sh_from_ply_f_rest(f_rest)
+    np.testing.assert_array_equal(sh, np.array([[[10, 20, 30], [11, 21, 31], [12, 22, 32]]], np.float32))
