@@ -128,8 +128,8 @@ def main() -> int:
     k = (scene.sh_degree + 1) ** 2 - 1
     mean_st = {key: float(np.mean([s[key] for s in view_stats])) for key in view_stats[0]}
     pixels = W * H
-    # refs the composite must load: bounded by R (early exits make it smaller)
-    comp_refs = mean_st["refs"]
+    # refs the rasterizer loaded: up to the batch in which each tile's last pixel saturated
+    comp_refs = mean_st["composite_refs"]
     per_stage = {}
     for name, ms in stage_ms.items():
         byts = algorithmic_bytes(name, scene.count, k, mean_st, pixels, comp_refs)

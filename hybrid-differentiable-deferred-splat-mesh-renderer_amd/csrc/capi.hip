@@ -329,6 +329,8 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->cap_refs_tripped = s.refs > c->plan.max_total_tile_refs;
   o->cap_large_splats_tripped = s.large_splats > 65535u;
   o->cap_chunks_tripped = s.chunks > 1024u;
+  o->composite_refs = s.composite_refs;
+  o->composite_max_refs = s.composite_max_refs;
   return HDSR_OK;
 }
 

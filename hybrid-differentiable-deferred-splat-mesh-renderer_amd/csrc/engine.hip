@@ -67,7 +67,7 @@ Layout Engine::layout(FrameDesc* frames, uint32_t V) {
 Engine::~Engine() { release(); }
 
 void Engine::release() {
-  void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
+  void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
                   b_.pairs_a, b_.pairs_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
@@ -125,6 +125,8 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.depth, &have, ns))) return rc;
     have = n_splat_;
     if ((rc = grow(&b_.large_list, &have, ns))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.dlog, &have, ns))) return rc;
     n_splat_ = ns;
   }
   if (L.splat_blocks > n_blocks_ || !b_.block_cnt) {

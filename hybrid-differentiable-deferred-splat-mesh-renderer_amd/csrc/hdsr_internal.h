@@ -75,7 +75,9 @@ struct DevStats {
   uint32_t segments;      // small tiles with >= 1 entry + chunks of large tiles
   uint32_t chunks;
   uint32_t scratch_used;
-  uint32_t pad[5];
+  uint32_t composite_refs;      // entries the rasterizer loaded (up to each tile's saturation)
+  uint32_t composite_max_refs;  // largest per-tile count of those
+  uint32_t pad[3];
 };
 
 struct Seg {
@@ -107,6 +109,7 @@ struct Bufs {
   uint4* cold;               // [splat_blocks*256*2]
   uint32_t* bounds;          // [splat_blocks*256]
   uint32_t* depth;           // [splat_blocks*256]
+  uint2* dlog;               // [splat_blocks*256] (depth bits, ln(max(depth, 1e-6)) bits) for the sorts
   uint4* block_cnt;          // [splat_blocks] per block (small pairs, large pairs, large splats, visible)
   uint4* block_pre;          // [splat_blocks] exclusive prefix of block_cnt
   uint4* scan_tmp;           // scratch of the uint4 scan

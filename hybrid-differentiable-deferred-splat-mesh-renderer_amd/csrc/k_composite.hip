@@ -2,13 +2,16 @@
 //
 // Reference: composite (src/shaders/gpu_tile_splat_composite.wgsl:123-370).
 //
-// One 64-thread workgroup (= one wave64) per 16x16 tile; lane (lx, ly) = (lane & 7, lane >> 3)
-// owns the 2x2 pixel quad at (16tx + 2lx, 16ty + 2ly).  Each batch stages 64 entries' records
-// into LDS (one 32-byte record gather per lane); every lane then walks the batch in order.  The
-// wave leaves the tile as soon as every pixel's transmittance is below the cutoff: from then on
-// no splat can pass the 'T >= 0.001' gate, so the output is the one of the full walk.
+// One 256-thread workgroup (four wave64s) per 16x16 tile, one pixel per thread.  The reference
+// gives each of 64 threads a 2x2 quad; on CDNA the tile's serial chain (every splat of a tile is
+// walked in order by the same lanes) is what bounds the kernel once all tiles are resident, so
+// spreading the 256 pixels over 256 lanes shortens the heaviest tile's chain four-fold.  Each
+// batch stages 256 entries' 32-byte records into LDS (one gather per thread); a wave whose 64
+// pixels are all saturated skips the batch, and the workgroup stops once all four are.  From
+// saturation on no splat can pass the 'T >= 0.001' gate, so the output equals the full walk.
 //
-// Per pixel and splat (fp32, no contraction, exp from hdsr_fmath.h):
+// Per pixel and splat (fp32, no contraction, exp from hdsr_fmath.h), with the reference's quad
+// arithmetic kept: d = p00 - centre, plus 1 for the odd column / row of the quad:
 //   power = (a*dx*dx + b*dx*dy) + c*dy*dy       a = -k0/2, b = -k1, c = -k2/2
 //   alpha = min(0.99, opacity * exp(power))
 //   valid = power > -4 && alpha > 1/255 && T >= 0.001
@@ -20,6 +23,8 @@
 
 namespace hdsr {
 namespace {
+
+constexpr uint32_t kBatch = 256;
 
 __device__ __forceinline__ float h16(uint32_t bits) {
   _Float16 h;
@@ -104,14 +109,38 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
   }
 }
 
+// The y-interval outside which the splat's power is <= -4 at every pixel of a tile whose pixel
+// centres lie in [x0, x1] x [y0, y1], or an empty interval (lo > hi) when the splat cannot
+// reach the tile at all.  Exact arithmetic: power = -q/2 with q = d^T K d, K = [[k0,k1],[k1,k2]],
+// so power > -4 needs q < 8, an ellipse with half extents sqrt(8 K^-1_xx), sqrt(8 K^-1_yy).
+// The kernel's fp32 power differs from the exact one by at most ~4 ulp of the sum of the
+// magnitudes of its three terms (bounded here over the tile), so the level 8 is raised by
+// twice that bound: culling then never drops a splat that the fp32 evaluation would accept.
+__device__ __forceinline__ float2 reach_y(float cx, float cy, float k0, float k1, float k2, float x0, float x1,
+                                          float y0, float y1) {
+  // lower bound of det(K): the two products and the difference each round once
+  const float p02 = k0 * k2, p11 = k1 * k1;
+  const float det = (p02 - p11) - 4.0f * 5.9604645e-08f * (p02 + p11);
+  if (!(det > 0.0f) || !(k0 > 0.0f) || !(k2 > 0.0f)) return make_float2(-INFINITY, INFINITY);
+  const float DX = fmaxf(fabsf(x0 - cx), fabsf(x1 - cx)), DY = fmaxf(fabsf(y0 - cy), fabsf(y1 - cy));
+  const float sabs = 0.5f * (k0 * DX * DX + 2.0f * fabsf(k1) * DX * DY + k2 * DY * DY);
+  const float level = 8.0f + 2.0f * (8.0f * 5.9604645e-08f * sabs * 2.0f + 1e-4f);
+  const float hx = sqrtf(level * k2 / det) * 1.0001f + 1e-3f;
+  const float hy = sqrtf(level * k0 / det) * 1.0001f + 1e-3f;
+  if (cx + hx < x0 || cx - hx > x1 || cy + hy < y0 || cy - hy > y1) return make_float2(1.0f, -1.0f);
+  return make_float2(cy - hy, cy + hy);
+}
+
 template <bool kG>
-__global__ __launch_bounds__(64) void k_composite(Bufs b) {
-  __shared__ float4 s_a[HDSR_BATCH];  // cx, cy, a, b
-  __shared__ float4 s_b[HDSR_BATCH];  // c, opacity, r, g
-  __shared__ float2 s_c[HDSR_BATCH];  // b, depth
-  __shared__ float4 s_ga[kG ? HDSR_BATCH : 1];  // roughness, metalness, confidence, emissive r
-  __shared__ float4 s_gb[kG ? HDSR_BATCH : 1];  // normal xyz, emissive g
-  __shared__ float s_gc[kG ? HDSR_BATCH : 1];   // emissive b
+__global__ __launch_bounds__(256) void k_composite(Bufs b) {
+  __shared__ float4 s_a[kBatch];  // cx, cy, a, b
+  __shared__ float4 s_b[kBatch];  // c, opacity, r, g
+  __shared__ float4 s_c[kBatch];  // b, depth, reach y0, reach y1
+  __shared__ float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
+  __shared__ float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
+  __shared__ float s_gc[kG ? kBatch : 1];   // emissive b
+  __shared__ int s_done[4];
+  __shared__ uint32_t s_keep[4];
 
   const uint32_t t = blockIdx.x;
   const uint32_t f = frame_of(b.tile_frame_start, b.num_frames, t);
@@ -121,27 +150,46 @@ __global__ __launch_bounds__(64) void k_composite(Bufs b) {
   const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
   const uint32_t start = b.tile_offsets[m];
   const uint32_t count = b.tile_counts[m];
-  const uint32_t lane = threadIdx.x;
-  const uint32_t X0 = tx * 16 + (lane & 7) * 2, Y0 = ty * 16 + (lane >> 3) * 2;
-  const float p00x = (float)X0 + 0.5f, p00y = (float)Y0 + 0.5f;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lx = tid & 15, ly = tid >> 4;
+  const uint32_t X = tx * 16 + lx, Y = ty * 16 + ly;
+  // the reference's quad origin and the +1 offsets of the quad's odd column / row
+  const float p00x = (float)(X & ~1u) + 0.5f, p00y = (float)(Y & ~1u) + 0.5f;
+  const bool odd_x = (lx & 1u) != 0, odd_y = (ly & 1u) != 0;
 
-  Px p[4];
-  Gb q[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    p[k] = Px{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    q[k] = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
-  }
+  Px p = Px{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t loaded = 0;
 
-  for (uint32_t base = 0; base < count; base += HDSR_BATCH) {
-    const uint32_t i = base + lane;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  const float tile_x0 = (float)(tx * 16) + 0.5f, tile_y0 = (float)(ty * 16) + 0.5f;
+  // pixel-centre rows of this wave
+  const float band_y0 = tile_y0 + (float)(wave * 4), band_y1 = band_y0 + 3.0f;
+  for (uint32_t base = 0; base < count; base += kBatch) {
+    const uint32_t i = base + tid;
+    bool keep = false;
+    uint4 h0 = make_uint4(0, 0, 0, 0), h1 = make_uint4(0, 0, 0, 0);
+    uint32_t g = 0;
+    float2 reach = make_float2(0.0f, 0.0f);
     if (i < count) {
-      const uint32_t g = b.entries[start + i];
-      const uint4 h0 = b.hot[(size_t)g * 2];
-      const uint4 h1 = b.hot[(size_t)g * 2 + 1];
-      s_a[lane] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
-      s_b[lane] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
-      s_c[lane] = make_float2(h16(h1.w), hdsr_u2f(h1.y));
+      g = b.entries[start + i];
+      h0 = b.hot[(size_t)g * 2];
+      h1 = b.hot[(size_t)g * 2 + 1];
+      reach = reach_y(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
+                      tile_x0 + 15.0f, tile_y0, tile_y0 + 15.0f);
+      keep = !(reach.x > reach.y);
+    }
+    // compact the kept splats, in order
+    const unsigned long long km = __ballot(keep);
+    if (lane == 0) s_keep[wave] = (uint32_t)__popcll(km);
+    __syncthreads();
+    uint32_t slot = (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
+    for (uint32_t q = 0; q < wave; ++q) slot += s_keep[q];
+    const uint32_t nb = s_keep[0] + s_keep[1] + s_keep[2] + s_keep[3];
+    if (keep) {
+      s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
+      s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
+      s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), reach.x, reach.y);
       if (kG) {
         const uint4 c0 = b.cold[(size_t)g * 2];
         const uint4 c1 = b.cold[(size_t)g * 2 + 1];
@@ -155,44 +203,45 @@ __global__ __launch_bounds__(64) void k_composite(Bufs b) {
           dny = ay;
         }
         const float dl = sqrtf((dnx * dnx + dny * dny) + dnz * dnz);
-        s_ga[lane] = make_float4(h16(c0.x), h16(c0.x >> 16), h16(c0.w), h16(c1.x));
-        s_gb[lane] = make_float4(dnx / dl, dny / dl, dnz / dl, h16(c1.x >> 16));
-        s_gc[lane] = h16(c1.y);
+        s_ga[slot] = make_float4(h16(c0.x), h16(c0.x >> 16), h16(c0.w), h16(c1.x));
+        s_gb[slot] = make_float4(dnx / dl, dny / dl, dnz / dl, h16(c1.x >> 16));
+        s_gc[slot] = h16(c1.y);
       }
     }
     __syncthreads();
-    const uint32_t nb = min(HDSR_BATCH, count - base);
-    for (uint32_t j = 0; j < nb; ++j) {
-      const float4 A = s_a[j];
-      const float4 B = s_b[j];
-      const float2 C = s_c[j];
-      const float dx = p00x - A.x, dy = p00y - A.y;
-      const float dx1 = dx + 1.0f, dy1 = dy + 1.0f;
-      const float pw0 = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
-      const float pw1 = (A.z * dx1 * dx1 + A.w * dx1 * dy) + B.x * dy * dy;
-      const float pw2 = (A.z * dx * dx + A.w * dx * dy1) + B.x * dy1 * dy1;
-      const float pw3 = (A.z * dx1 * dx1 + A.w * dx1 * dy1) + B.x * dy1 * dy1;
-      if (pw0 <= -4.0f && pw1 <= -4.0f && pw2 <= -4.0f && pw3 <= -4.0f) continue;
-      float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
-      float gc = 0.0f;
-      if (kG) {
-        ga = s_ga[j];
-        gb = s_gb[j];
-        gc = s_gc[j];
+    loaded += min(kBatch, count - base);
+    if (!__all(p.T < 0.001f)) {
+      for (uint32_t j = 0; j < nb; ++j) {
+        if ((j & 31u) == 31u && __all(p.T < 0.001f)) break;  // wave saturated
+        const float4 C = s_c[j];
+        if (C.z > band_y1 || C.w < band_y0) continue;  // uniform: the splat misses this wave's rows
+        const float4 A = s_a[j];
+        const float4 B = s_b[j];
+        float dx = p00x - A.x, dy = p00y - A.y;
+        dx = odd_x ? dx + 1.0f : dx;
+        dy = odd_y ? dy + 1.0f : dy;
+        const float pw = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
+        if (pw <= -4.0f) continue;
+        float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
+        float gc = 0.0f;
+        if (kG) {
+          ga = s_ga[j];
+          gb = s_gb[j];
+          gc = s_gc[j];
+        }
+        blend<kG>(p, q, pw, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
       }
-      blend<kG>(p[0], q[0], pw0, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
-      blend<kG>(p[1], q[1], pw1, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
-      blend<kG>(p[2], q[2], pw2, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
-      blend<kG>(p[3], q[3], pw3, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
     }
+    const int wave_done = __all(p.T < 0.001f);
+    if ((tid & 63) == 0) s_done[tid >> 6] = wave_done;
     __syncthreads();
-    const bool done = p[0].T < 0.001f && p[1].T < 0.001f && p[2].T < 0.001f && p[3].T < 0.001f;
-    if (__all(done)) break;
+    if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) break;
   }
-  store_px<kG>(F, X0, Y0, p[0], q[0]);
-  store_px<kG>(F, X0 + 1, Y0, p[1], q[1]);
-  store_px<kG>(F, X0, Y0 + 1, p[2], q[2]);
-  store_px<kG>(F, X0 + 1, Y0 + 1, p[3], q[3]);
+  store_px<kG>(F, X, Y, p, q);
+  if (tid == 0 && b.stats) {
+    atomicAdd(&b.stats->composite_refs, loaded);
+    atomicMax(&b.stats->composite_max_refs, loaded);
+  }
 }
 
 }  // namespace
@@ -200,9 +249,9 @@ __global__ __launch_bounds__(64) void k_composite(Bufs b) {
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer) {
   if (!b.total_tiles) return;
   if (gbuffer)
-    hipLaunchKernelGGL(k_composite<true>, dim3(b.total_tiles), dim3(64), 0, stream, b);
+    hipLaunchKernelGGL(k_composite<true>, dim3(b.total_tiles), dim3(256), 0, stream, b);
   else
-    hipLaunchKernelGGL(k_composite<false>, dim3(b.total_tiles), dim3(64), 0, stream, b);
+    hipLaunchKernelGGL(k_composite<false>, dim3(b.total_tiles), dim3(256), 0, stream, b);
 }
 
 }  // namespace hdsr

@@ -267,6 +267,9 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   b.cold[(size_t)g * 2 + 1] = c1;
   b.bounds[g] = bounds;
   b.depth[g] = depth_bits;
+  // ln(max(depth, 1e-6)) once per splat; the per-tile keys and buckets subtract the tile's
+  // logMin from it (hdsr_log_pos evaluates the same expression)
+  b.dlog[g] = make_uint2(depth_bits, hdsr_f2u(hdsr_logf(fmaxf(hdsr_u2f(depth_bits), 1e-6f))));
 
   // workgroup reduction of the four counts
   __shared__ uint32_t red[4][4];

@@ -1,5 +1,5 @@
-// k_sort.hip - per-tile depth order: bitonic sort on 20-bit log-depth keys, with the 128-bucket
-// log-depth cascade for tiles holding more than 4096 entries.
+// k_sort.hip - per-tile depth order on 20-bit log-depth keys, with the 128-bucket log-depth
+// cascade for tiles holding more than 4096 entries.
 //
 // Reference: tile_depth_sort (src/shaders/gpu_tile_depth_sort.wgsl:28-79), bucket_sort
 // (gpu_tile_bucket_sort.wgsl:41-144), chunk_sort / write_chunk_indirect
@@ -9,21 +9,30 @@
 //   y   = (ln(max(depth, 1e-6)) - logMin) * (128 / logRange)       (the bucket coordinate)
 //   q20 = min(u32(y * 2^13), 2^20 - 1),  key = (q20 << 12) | i
 // with logMin / logRange from the min and max depth bits of the whole tile.  Keys are unique, so
-// the bitonic network's result equals a stable sort by q20.  The bucket of an entry is
-// min(u32(y), 127) = q20 >> 13.
+// the reference's bitonic network orders them exactly as a stable sort on q20 does; that sort is
+// done here as an LSD radix sort in LDS (digits of 8, 8 and 4 bits, a pass skipped when all keys
+// share its digit).  Equal digits are ranked inside a wave with eight ballots and a wave-private
+// running count, so a pass needs three workgroup barriers instead of the bitonic network's 78.
+// The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
-// Large tiles: one 1024-thread workgroup finds the depth range, histograms the 128 buckets and
-// writes the tile's entries in bucket order to scratch, stably (ranks inside a wave come from
-// seven ballots, across waves from a per-bucket prefix).  The bucket-ordered list is then cut
-// into consecutive 4096-entry chunks (the reference's greedy packing splits buckets exactly
-// there) and each chunk is sorted like a small tile, reading scratch and writing the tile's
-// range of the entry list.  The reference's thread-0 serial phases and atomic cursors are gone.
+// Large tiles: one 1024-thread workgroup finds the depth range and writes the tile's entries in
+// stable bucket order to scratch (ranks by ballots, per-bucket prefix across waves).  The
+// bucket-ordered list is then cut into consecutive 4096-entry chunks (the reference's greedy
+// packing splits buckets exactly there) and each chunk is sorted like a small tile, reading
+// scratch and writing the tile's range of the entry list.
+//
+// ln(max(depth, 1e-6)) is computed once per splat by the projection (dlog[]), bit-identical to
+// evaluating it here.
 #include "hdsr_internal.h"
 
 namespace hdsr {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ float log_pos(uint32_t ldepth_bits, float log_min, float scale) {
+  return (hdsr_u2f(ldepth_bits) - log_min) * scale;
+}
 
 __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   __shared__ uint32_t red_min[16], red_max[16];
@@ -37,7 +46,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     const uint32_t* src = b.entries + L.start;
     uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
     for (uint32_t i = t; i < L.count; i += 1024) {
-      const uint32_t d = b.depth[src[i]];
+      const uint32_t d = b.dlog[src[i]].x;
       dmn = min(dmn, d);
       dmx = max(dmx, d);
     }
@@ -59,16 +68,22 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     float lmin, scale;
     hdsr_log_frame(dmn, dmx, &lmin, &scale);
     for (uint32_t i = t; i < L.count; i += 1024) {
-      const uint32_t k = hdsr_bucket_of(hdsr_log_pos(b.depth[src[i]], lmin, scale));
+      const uint32_t k = hdsr_bucket_of(log_pos(b.dlog[src[i]].y, lmin, scale));
       atomicAdd(&bcnt[k], 1u);
     }
     __syncthreads();
-    if (t == 0) {
-      uint32_t run = 0;
-      for (uint32_t k = 0; k < HDSR_NUM_BUCKETS; ++k) {
-        bbase[k] = run;
-        run += bcnt[k];
+    if (t < 64) {
+      // exclusive scan of the 128 bucket counts by one wave (two per lane)
+      const uint32_t c0 = bcnt[2 * t], c1 = bcnt[2 * t + 1];
+      uint32_t inc = c0 + c1;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = __shfl_up(inc, d, 64);
+        if (t >= (uint32_t)d) inc += v;
       }
+      const uint32_t exc = inc - c0 - c1;
+      bbase[2 * t] = exc;
+      bbase[2 * t + 1] = exc + c0;
     }
     if (t < HDSR_NUM_BUCKETS) cursor[t] = 0;
     __syncthreads();
@@ -76,7 +91,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
       const uint32_t i = r0 + t;
       const bool valid = i < L.count;
       const uint32_t g = valid ? src[i] : 0u;
-      const uint32_t k = valid ? hdsr_bucket_of(hdsr_log_pos(b.depth[g], lmin, scale)) : 0u;
+      const uint32_t k = valid ? hdsr_bucket_of(log_pos(b.dlog[g].y, lmin, scale)) : 0u;
       unsigned long long m = __ballot(valid);
 #pragma unroll
       for (uint32_t bit = 0; bit < 7; ++bit) {
@@ -113,79 +128,179 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   }
 }
 
-__device__ __forceinline__ uint32_t insert_zero_bit(uint32_t v, uint32_t bit) {
-  const uint32_t mask = (1u << bit) - 1u;
-  return ((v >> bit) << (bit + 1u)) | (v & mask);
+constexpr uint32_t kSortThreads = 1024;
+constexpr uint32_t kSortWaves = kSortThreads / 64;
+
+__device__ __forceinline__ uint32_t wave_and(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+  return v;
 }
 
-__global__ __launch_bounds__(256) void k_segsort(Bufs b) {
-  __shared__ uint32_t keys[HDSR_MAX_TILE_ENTRIES];
+// One LSD pass over cur[0..n): stable by the digit (key >> shift) & (2^bits - 1), into nxt.
+// Wave w owns positions [w*S, min((w+1)*S, n)) (S <= 256) and walks them in rounds of 64;
+// equal digits inside a round are ranked by ballots, across rounds by a wave-private count.
+// Every thread of the workgroup must call it (it contains barriers).
+__device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
+                                               uint32_t shift, uint32_t bits, uint32_t (*wc)[256],
+                                               uint32_t* wsum) {
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t mask = (1u << bits) - 1u;
+  for (uint32_t q = t; q < kSortWaves * 256; q += kSortThreads) wc[q >> 8][q & 255] = 0;
+  __syncthreads();
+  const uint32_t lo = w * S, hi = min(lo + S, n);
+  uint32_t keys[4], dig[4], pos[4];
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t p = lo + r * 64 + lane;
+    const bool valid = p < hi;
+    const uint32_t key = valid ? cur[p] : 0u;
+    const uint32_t d = (key >> shift) & mask;
+    unsigned long long m = __ballot(valid);
+    for (uint32_t bit = 0; bit < bits; ++bit) {
+      const bool s = (d >> bit) & 1u;
+      const unsigned long long bm = __ballot(s);
+      m &= s ? bm : ~bm;
+    }
+    const uint32_t rank = __popcll(m & below);
+    uint32_t old = 0;
+    if (valid) old = wc[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (m >> lane) == 1ull) wc[w][d] = old + rank + 1;
+    __builtin_amdgcn_wave_barrier();
+    keys[r] = key;
+    dig[r] = d;
+    pos[r] = valid ? old + rank : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  // digit-major exclusive scan over (digit, wave): digit t's base, then each wave's offset
+  uint32_t tot = 0, inc = 0;
+  if (t < 256) {
+#pragma unroll
+    for (uint32_t q = 0; q < kSortWaves; ++q) tot += wc[q][t];
+    inc = tot;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t v = __shfl_up(inc, d, 64);
+      if (lane >= (uint32_t)d) inc += v;
+    }
+    if (lane == 63) wsum[w] = inc;
+  }
+  __syncthreads();
+  if (t < 256) {
+    uint32_t run = inc - tot;
+    for (uint32_t q = 0; q < w; ++q) run += wsum[q];
+#pragma unroll
+    for (uint32_t q = 0; q < kSortWaves; ++q) {
+      const uint32_t c = wc[q][t];
+      wc[q][t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r)
+    if (pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
+  __shared__ uint32_t ka[HDSR_MAX_TILE_ENTRIES];
+  __shared__ uint32_t kb[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t vals[HDSR_MAX_TILE_ENTRIES];
-  __shared__ uint32_t red_min[4], red_max[4];
+  __shared__ uint32_t wc[kSortWaves][256];
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t red[4][kSortWaves];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const uint32_t nseg = b.stats->segments;
   for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
     const Seg S = b.segs[s];
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t count = S.count;
-    uint32_t d[16];
+    uint2 dl[4];
     uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 16; ++j) {
-      const uint32_t i = j * 256 + t;
-      d[j] = 0;
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      dl[r] = make_uint2(0, 0);
       if (i < count) {
         const uint32_t g = src[i];
         vals[i] = g;
-        d[j] = b.depth[g];
-        dmn = min(dmn, d[j]);
-        dmx = max(dmx, d[j]);
+        dl[r] = b.dlog[g];
+        dmn = min(dmn, dl[r].x);
+        dmx = max(dmx, dl[r].x);
       }
     }
-    float lmin, scale;
     if (S.large == kNone) {
       dmn = wave_min(dmn);
       dmx = wave_max(dmx);
       if (lane == 0) {
-        red_min[w] = dmn;
-        red_max[w] = dmx;
+        red[0][w] = dmn;
+        red[1][w] = dmx;
       }
       __syncthreads();
-      dmn = min(min(red_min[0], red_min[1]), min(red_min[2], red_min[3]));
-      dmx = max(max(red_max[0], red_max[1]), max(red_max[2], red_max[3]));
+      dmn = red[0][0];
+      dmx = red[1][0];
+#pragma unroll
+      for (uint32_t q = 1; q < kSortWaves; ++q) {
+        dmn = min(dmn, red[0][q]);
+        dmx = max(dmx, red[1][q]);
+      }
     } else {
       dmn = b.large_tiles[S.large].dmin;
       dmx = b.large_tiles[S.large].dmax;
     }
+    float lmin, scale;
     hdsr_log_frame(dmn, dmx, &lmin, &scale);
-    uint32_t P = 1;
-    while (P < count) P <<= 1;
+    uint32_t kand = 0xFFFFFFFFu, kor = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 16; ++j) {
-      const uint32_t i = j * 256 + t;
-      if (i < count) keys[i] = (hdsr_q20_of(hdsr_log_pos(d[j], lmin, scale)) << 12) | i;
-      else if (i < P) keys[i] = 0xFFFFFFFFu;
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      if (i < count) {
+        const uint32_t key = (hdsr_q20_of(log_pos(dl[r].y, lmin, scale)) << 12) | i;
+        ka[i] = key;
+        kand &= key;
+        kor |= key;
+      }
+    }
+    kand = wave_and(kand);
+    kor = wave_or(kor);
+    if (lane == 0) {
+      red[2][w] = kand;
+      red[3][w] = kor;
     }
     __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        const uint32_t bit = __builtin_ctz(j);
-        for (uint32_t c = t; c < (P >> 1); c += 256) {
-          const uint32_t l = insert_zero_bit(c, bit);
-          const uint32_t r = l | j;
-          const uint32_t a = keys[l], bb = keys[r];
-          const bool asc = (l & k) == 0;
-          if ((a > bb) == asc) {
-            keys[l] = bb;
-            keys[r] = a;
-          }
-        }
-        __syncthreads();
+    kand = red[2][0];
+    kor = red[3][0];
+#pragma unroll
+    for (uint32_t q = 1; q < kSortWaves; ++q) {
+      kand &= red[2][q];
+      kor |= red[3][q];
+    }
+    const uint32_t differ = kand ^ kor;  // bits that are not the same in every key
+    const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
+    uint32_t* cur = ka;
+    uint32_t* nxt = kb;
+    const uint32_t shifts[3] = {12, 20, 28}, widths[3] = {8, 8, 4};
+#pragma unroll
+    for (uint32_t ps = 0; ps < 3; ++ps) {
+      const uint32_t dmask = ((1u << widths[ps]) - 1u) << shifts[ps];
+      if (count > 1 && (differ & dmask)) {  // uniform across the workgroup
+        lds_radix_pass(cur, nxt, count, Sw, shifts[ps], widths[ps], wc, wsum);
+        uint32_t* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
       }
     }
     uint32_t* dst = b.entries + S.dst;
-    for (uint32_t i = t; i < count; i += 256) {
-      const uint32_t key = keys[i];
+    for (uint32_t i = t; i < count; i += kSortThreads) {
+      const uint32_t key = cur[i];
       dst[i] = vals[key & 0xFFFu];
       if (b.keys20) b.keys20[S.dst + i] = key;
     }
@@ -203,8 +318,8 @@ void launch_bucket(const Bufs& b, hipStream_t stream) {
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   if (!b.seg_cap) return;
-  const uint32_t grid = b.seg_cap < 16384u ? b.seg_cap : 16384u;
-  hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(256), 0, stream, b);
+  const uint32_t grid = b.seg_cap < 2048u ? b.seg_cap : 2048u;
+  hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
 }
 
 }  // namespace hdsr

@@ -168,7 +168,8 @@ typedef struct hdsr_stats {
   uint32_t cap_refs_tripped;      /* R > maxTotalTileRefs (gpu_tile_splat_composite.wgsl:114) */
   uint32_t cap_large_splats_tripped; /* > 65535 large splats (gpu_classify_large_splats.wgsl:52) */
   uint32_t cap_chunks_tripped;    /* > 1024 chunks (gpu_tile_bucket_sort.wgsl:126) */
-  uint32_t reserved[2];
+  uint32_t composite_refs;        /* entries the rasterizer loaded (up to each tile's saturation) */
+  uint32_t composite_max_refs;    /* the largest per-tile count of those */
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 
