@@ -68,7 +68,7 @@ Engine::~Engine() { release(); }
 
 void Engine::release() {
   void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
-                  b_.pairs_a, b_.pairs_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch,
+                  b_.keys_a, b_.vals_a, b_.keys_b, b_.vals_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch, b_.ekey, b_.ekey_s,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
   for (void* p : ptrs)
@@ -141,17 +141,25 @@ int Engine::ensure(const Layout& L, bool debug) {
   // pair capacity
   uint64_t cap = std::max<uint64_t>({min_cap_, want_cap_, 1});
   cap = std::min<uint64_t>(cap, 0xFFFFFFF0ull);
-  if (cap > n_cap_ || !b_.pairs_a) {
+  if (cap > n_cap_ || !b_.keys_a) {
     have = n_cap_;
-    if ((rc = grow(&b_.pairs_a, &have, cap))) return rc;
+    if ((rc = grow(&b_.keys_a, &have, cap))) return rc;
     have = n_cap_;
-    if ((rc = grow(&b_.pairs_b, &have, cap))) return rc;
+    if ((rc = grow(&b_.vals_a, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.keys_b, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.vals_b, &have, cap))) return rc;
     have = n_cap_;
     if ((rc = grow(&b_.entries, &have, cap))) return rc;
     have = n_cap_;
     if ((rc = grow(&b_.tkeys, &have, cap))) return rc;
     have = n_cap_;
     if ((rc = grow(&b_.scratch, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.ekey, &have, cap))) return rc;
+    have = n_cap_;
+    if ((rc = grow(&b_.ekey_s, &have, cap))) return rc;
     n_cap_ = cap;
     n_debug_ = 0;  // debug arrays follow the capacity
   }

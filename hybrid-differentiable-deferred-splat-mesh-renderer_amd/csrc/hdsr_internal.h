@@ -114,13 +114,17 @@ struct Bufs {
   uint4* block_pre;          // [splat_blocks] exclusive prefix of block_cnt
   uint4* scan_tmp;           // scratch of the uint4 scan
   uint32_t* large_list;      // [splat_blocks*256] sort ranks (global g) of large splats
-  unsigned long long* pairs_a;  // [cap] (key << 32) | g
-  unsigned long long* pairs_b;  // [cap]
+  uint32_t* keys_a;          // [cap] pair keys (global Morton key), emission order
+  uint32_t* vals_a;          // [cap] pair values (global splat index g)
+  uint32_t* keys_b;          // [cap] radix ping-pong
+  uint32_t* vals_b;          // [cap]
   uint32_t* entries;         // [cap] final per-tile order (global g)
   uint32_t* tkeys;           // [cap] Morton key of each entry after grouping
   uint32_t* keys20;          // [cap] (q20 << 12) | local index, or null
   uint32_t* entries_canon;   // [cap] entries before the per-tile sort, or null
   uint32_t* scratch;         // [cap] bucket-ordered copies of large tiles
+  uint2* ekey;               // [cap] dlog[] of each grouped entry (same order as entries)
+  uint2* ekey_s;             // [cap] dlog[] of each scratch entry
   uint32_t cap;              // pair capacity
   uint32_t* radix_hist;      // [256 * radix blocks] digit-major
   uint32_t* radix_tmp;       // scratch of the u32 scan

@@ -9,10 +9,11 @@
 // Here the order is canonical by construction: every visible splat's pairs (Morton key, rank)
 // are written at an offset from an exclusive scan over the per-splat tile counts (small splats
 // in ascending rank, then large splats in ascending rank), and a stable LSD radix sort on the
-// Morton key groups them by tile while keeping that order.  The radix digit is 8 bits (two
-// passes cover a single frame's 16-bit Morton range); each pass is a per-workgroup digit
-// histogram, a hierarchical scan of the digit-major histogram, and a stable scatter that ranks
-// equal digits inside each wave with eight ballots.  All of it is deterministic.
+// Morton key groups them by tile while keeping that order.  The key width decides the passes
+// (at most 8 bits each: a 1080p frame's 14-bit Morton range takes two 7-bit passes, as the
+// reference's 4 x 4-bit passes over 16 bits give the same stable order); each pass is a
+// per-workgroup digit histogram, a hierarchical scan of the digit-major histogram, and a stable
+// scatter that ranks equal digits inside each wave with ballots.  All of it is deterministic.
 #include "hdsr_internal.h"
 
 namespace hdsr {
@@ -177,29 +178,66 @@ __global__ __launch_bounds__(256) void k_emit(Bufs b) {
   const uint4 c = counts_of(bd);
   uint4 tot;
   const uint4 pre = block_excl_256<uint4>(c, &tot) + b.block_pre[blk];
-  if (bd == HDSR_SENTINEL) return;
   const uint32_t small_total = b.stats->small_pairs;
-  uint32_t off = c.z ? small_total + pre.y : pre.x;
   if (c.z) b.large_list[pre.z] = g;
-  uint32_t x0, y0, x1, y1;
-  box_of(bd, &x0, &y0, &x1, &y1);
+  // A wave's small-splat pairs occupy one contiguous range of the output and its large-splat
+  // pairs another (both classes are in ascending rank), so the wave writes each range with
+  // consecutive lanes: pair p of the range belongs to the lane whose running start is the last
+  // one <= p, and is that splat's (p - start)-th tile in row-major order of its box.
+  const uint32_t lane = __lane_id();
   const uint32_t cap = b.cap;
-  for (uint32_t ty = y0; ty <= y1; ++ty)
-    for (uint32_t tx = x0; tx <= x1; ++tx) {
-      if (off < cap) {
-        const unsigned long long key = mbase + hdsr_morton2(tx, ty);
-        b.pairs_a[off] = (key << 32) | g;
-      }
-      ++off;
+  const uint32_t g0 = g - lane;
+#pragma unroll
+  for (uint32_t cls = 0; cls < 2; ++cls) {
+    const uint32_t n = cls ? c.y : c.x;
+    uint32_t inc = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t x = __shfl_up(inc, d, 64);
+      if (lane >= (uint32_t)d) inc += x;
     }
+    const uint32_t loc = inc - n;
+    const uint32_t total = __shfl(inc, 63, 64);
+    const uint32_t off = cls ? small_total + pre.y : pre.x;
+    for (uint32_t p0 = 0; p0 < total; p0 += 64) {
+      const uint32_t p = p0 + lane;
+      uint32_t lo = 0, hi = 64;
+#pragma unroll
+      for (int it = 0; it < 6; ++it) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (__shfl(loc, mid, 64) <= p) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t o_loc = __shfl(loc, lo, 64);
+      const uint32_t o_off = __shfl(off, lo, 64);
+      const uint32_t o_bd = __shfl(bd, lo, 64);
+      if (p < total) {
+        const uint32_t j = p - o_loc;
+        uint32_t x0, y0, x1, y1;
+        box_of(o_bd, &x0, &y0, &x1, &y1);
+        const uint32_t sx = x1 - x0 + 1u;
+        const uint32_t ty = y0 + j / sx, tx = x0 + j % sx;
+        const uint32_t dst = o_off + j;
+        if (dst < cap) {
+          b.keys_a[dst] = mbase + hdsr_morton2(tx, ty);
+          b.vals_a[dst] = g0 + lo;
+        }
+      }
+    }
+  }
 }
 
-// ---- stable LSD radix passes on the Morton key (the high 32 bits of a pair) -----------------
+// ---- stable LSD radix passes on the Morton key ---------------------------------------------
+//
+// A pass over blocks of kRadixBlockItems pairs: k_radix_hist counts the digit of each key per
+// block (digit-major table, then scanned), k_radix_scatter ranks each pair inside its block and
+// writes it to its global slot.  In the scatter each wave owns 1024 consecutive pairs of the
+// block and ranks equal digits with ballots plus a wave-private running count (no workgroup
+// barrier per round); the block's pairs are then reordered in LDS so that the global writes of
+// each digit run are contiguous.
 
-__device__ __forceinline__ uint32_t radix_blocks(uint32_t cap) { return (cap + kRadixBlockItems - 1) / kRadixBlockItems; }
-
-__global__ __launch_bounds__(256) void k_radix_hist(const unsigned long long* pairs, uint32_t* hist,
-                                                    const DevStats* st, uint32_t nblocks, uint32_t shift) {
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t* hist, const DevStats* st,
+                                                    uint32_t nblocks, uint32_t shift, uint32_t mask) {
   __shared__ uint32_t cnt[kRadixBuckets];
   cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -209,70 +247,100 @@ __global__ __launch_bounds__(256) void k_radix_hist(const unsigned long long* pa
 #pragma unroll 4
     for (uint32_t j = 0; j < 16; ++j) {
       const uint32_t i = base + j * 256 + threadIdx.x;
-      if (i < n) {
-        const uint32_t d = (uint32_t)(pairs[i] >> (32 + shift)) & (kRadixBuckets - 1);
-        atomicAdd(&cnt[d], 1u);
-      }
+      if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & mask], 1u);
     }
   }
   __syncthreads();
   hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-template <bool kLast>
-__global__ __launch_bounds__(256) void k_radix_scatter(const unsigned long long* in, unsigned long long* out,
-                                                       uint32_t* entries, uint32_t* tkeys, const uint32_t* hist,
-                                                       const DevStats* st, uint32_t nblocks, uint32_t shift) {
-  __shared__ uint32_t digit_base[kRadixBuckets];
-  __shared__ uint32_t wcnt[4][kRadixBuckets];
+// With 'dlog' set (the last pass), also writes each moved pair's (depth bits, log-depth) to
+// ekey_out at the same position, so the per-tile sorts read them as a stream.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                       uint32_t* vout, const uint32_t* hist, const DevStats* st,
+                                                       uint32_t nblocks, uint32_t shift, uint32_t bits,
+                                                       const uint2* dlog, uint2* ekey_out) {
+  __shared__ uint32_t sk[kRadixBlockItems];
+  __shared__ uint32_t sv[kRadixBlockItems];
+  __shared__ uint32_t wc[4][kRadixBuckets];
+  __shared__ uint32_t dbase[kRadixBuckets];
+  __shared__ uint32_t gbase[kRadixBuckets];
+  __shared__ uint32_t wsum[4];
   const uint32_t n = st->refs;
   const uint32_t base = blockIdx.x * kRadixBlockItems;
   if (base >= n) return;  // uniform per workgroup
-  const uint32_t t = threadIdx.x;
-  digit_base[t] = hist[t * nblocks + blockIdx.x];
-  const uint32_t lane = __lane_id();
-  const uint32_t w = t >> 6;
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  for (uint32_t j = 0; j < 16; ++j) {
-    const uint32_t i = base + j * 256 + t;
-    const bool valid = i < n;
-    const unsigned long long v = valid ? in[i] : 0ull;
-    const uint32_t d = (uint32_t)(v >> (32 + shift)) & (kRadixBuckets - 1);
-    unsigned long long m = __ballot(valid);
+  const uint32_t mask = (1u << bits) - 1u;
 #pragma unroll
-    for (uint32_t bit = 0; bit < kRadixBits; ++bit) {
+  for (uint32_t q = 0; q < 4; ++q) wc[q][t] = 0;
+  gbase[t] = hist[t * nblocks + blockIdx.x];
+  __syncthreads();
+  uint32_t k[16], v[16], pos[16];
+#pragma unroll
+  for (uint32_t r = 0; r < 16; ++r) {  // all loads first: 32 in flight per lane
+    const uint32_t i = base + w * 1024 + r * 64 + lane;
+    const bool valid = i < n;
+    k[r] = valid ? kin[i] : 0u;
+    v[r] = valid ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 16; ++r) {
+    const uint32_t i = base + w * 1024 + r * 64 + lane;
+    const bool valid = i < n;
+    const uint32_t d = (k[r] >> shift) & mask;
+    unsigned long long m = __ballot(valid);
+    for (uint32_t bit = 0; bit < bits; ++bit) {
       const bool s = (d >> bit) & 1u;
       const unsigned long long bm = __ballot(s);
       m &= s ? bm : ~bm;
     }
     const uint32_t rank = __popcll(m & below);
-    const bool last_of_group = valid && ((m >> lane) == 1ull);
+    uint32_t old = 0;
+    if (valid) old = wc[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (m >> lane) == 1ull) wc[w][d] = old + rank + 1;
+    __builtin_amdgcn_wave_barrier();
+    pos[r] = valid ? old + rank : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  // block-local digit starts (exclusive scan over digits) and each wave's start inside a digit
+  const uint32_t c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
+  const uint32_t tot = c0 + c1 + c2 + c3;
+  uint32_t inc = tot;
 #pragma unroll
-    for (uint32_t q = 0; q < 4; ++q) wcnt[q][t] = 0;
-    __syncthreads();
-    if (last_of_group) wcnt[w][d] = rank + 1;
-    __syncthreads();
-    {
-      uint32_t run = digit_base[t];
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t x = __shfl_up(inc, d, 64);
+    if (lane >= (uint32_t)d) inc += x;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t db = inc - tot;
+  for (uint32_t q = 0; q < w; ++q) db += wsum[q];
+  dbase[t] = db;
+  wc[0][t] = db;
+  wc[1][t] = db + c0;
+  wc[2][t] = db + c0 + c1;
+  wc[3][t] = db + c0 + c1 + c2;
+  __syncthreads();
 #pragma unroll
-      for (uint32_t q = 0; q < 4; ++q) {
-        const uint32_t c = wcnt[q][t];
-        wcnt[q][t] = run;
-        run += c;
-      }
-      digit_base[t] = run;
+  for (uint32_t r = 0; r < 16; ++r) {
+    if (pos[r] != 0xFFFFFFFFu) {
+      const uint32_t lp = wc[w][(k[r] >> shift) & mask] + pos[r];
+      sk[lp] = k[r];
+      sv[lp] = v[r];
     }
-    __syncthreads();
-    if (valid) {
-      const uint32_t dst = wcnt[w][d] + rank;
-      if (kLast) {
-        entries[dst] = (uint32_t)v;
-        tkeys[dst] = (uint32_t)(v >> 32);
-      } else {
-        out[dst] = v;
-      }
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+  const uint32_t cnt = min(kRadixBlockItems, n - base);
+  for (uint32_t q = t; q < cnt; q += 256) {
+    const uint32_t kk = sk[q];
+    const uint32_t d = (kk >> shift) & mask;
+    const uint32_t dst = gbase[d] + (q - dbase[d]);
+    const uint32_t vv = sv[q];
+    kout[dst] = kk;
+    vout[dst] = vv;
+    if (dlog) ekey_out[dst] = dlog[vv];
   }
 }
 
@@ -385,23 +453,24 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
   const uint32_t nblocks = (b.cap + kRadixBlockItems - 1) / kRadixBlockItems;
   if (!nblocks) return;
   const uint32_t passes = b.key_bits == 0 ? 1u : (b.key_bits + kRadixBits - 1) / kRadixBits;
-  const unsigned long long* src = b.pairs_a;
-  unsigned long long* dst = b.pairs_b;
+  const uint32_t bits = b.key_bits == 0 ? 1u : (b.key_bits + passes - 1) / passes;
+  const uint32_t mask = (1u << bits) - 1u;
+  const uint32_t* ks = b.keys_a;
+  const uint32_t* vs = b.vals_a;
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t shift = p * kRadixBits;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, src, b.radix_hist, b.stats, nblocks,
-                       shift);
+    const uint32_t shift = p * bits;
+    const bool last = p + 1 == passes;
+    // the last pass writes the grouped entries and their keys; earlier ones ping-pong
+    uint32_t* kd = last ? b.tkeys : (ks == b.keys_a ? b.keys_b : b.keys_a);
+    uint32_t* vd = last ? b.entries : (vs == b.vals_a ? b.vals_b : b.vals_a);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, ks, b.radix_hist, b.stats, nblocks, shift,
+                       mask);
     scan_u32_exclusive(b.radix_hist, b.radix_hist, nblocks * kRadixBuckets, b.radix_tmp, stream);
-    if (p + 1 == passes) {
-      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(nblocks), dim3(256), 0, stream, src, dst, b.entries, b.tkeys,
-                         b.radix_hist, b.stats, nblocks, shift);
-    } else {
-      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(nblocks), dim3(256), 0, stream, src, dst, b.entries, b.tkeys,
-                         b.radix_hist, b.stats, nblocks, shift);
-      const unsigned long long* t = dst;
-      dst = const_cast<unsigned long long*>(src);
-      src = t;
-    }
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, stream, ks, vs, kd, vd, b.radix_hist,
+                       b.stats, nblocks, shift, bits, last ? (const uint2*)b.dlog : (const uint2*)nullptr,
+                       last ? b.ekey : (uint2*)nullptr);
+    ks = kd;
+    vs = vd;
   }
 }
 

@@ -165,16 +165,20 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const float tile_x0 = (float)(tx * 16) + 0.5f, tile_y0 = (float)(ty * 16) + 0.5f;
   // pixel-centre rows of this wave
   const float band_y0 = tile_y0 + (float)(wave * 4), band_y1 = band_y0 + 3.0f;
+  // Two-stage prefetch: while batch k is blended, the records of batch k+1 and the entry
+  // indices of batch k+2 are in flight.
+  const uint4 z4 = make_uint4(0, 0, 0, 0);
+  uint32_t g_cur = tid < count ? b.entries[start + tid] : 0u;
+  uint4 h0n = tid < count ? b.hot[(size_t)g_cur * 2] : z4;
+  uint4 h1n = tid < count ? b.hot[(size_t)g_cur * 2 + 1] : z4;
+  uint32_t g_next = kBatch + tid < count ? b.entries[start + kBatch + tid] : 0u;
   for (uint32_t base = 0; base < count; base += kBatch) {
     const uint32_t i = base + tid;
     bool keep = false;
-    uint4 h0 = make_uint4(0, 0, 0, 0), h1 = make_uint4(0, 0, 0, 0);
-    uint32_t g = 0;
+    const uint4 h0 = h0n, h1 = h1n;
+    const uint32_t g = g_cur;
     float2 reach = make_float2(0.0f, 0.0f);
     if (i < count) {
-      g = b.entries[start + i];
-      h0 = b.hot[(size_t)g * 2];
-      h1 = b.hot[(size_t)g * 2 + 1];
       reach = reach_y(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
                       tile_x0 + 15.0f, tile_y0, tile_y0 + 15.0f);
       keep = !(reach.x > reach.y);
@@ -209,6 +213,13 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
       }
     }
     __syncthreads();
+    {
+      const uint32_t i1 = base + kBatch + tid, i2 = base + 2 * kBatch + tid;
+      g_cur = g_next;
+      h0n = i1 < count ? b.hot[(size_t)g_cur * 2] : z4;
+      h1n = i1 < count ? b.hot[(size_t)g_cur * 2 + 1] : z4;
+      g_next = i2 < count ? b.entries[start + i2] : 0u;
+    }
     loaded += min(kBatch, count - base);
     if (!__all(p.T < 0.001f)) {
       for (uint32_t j = 0; j < nb; ++j) {

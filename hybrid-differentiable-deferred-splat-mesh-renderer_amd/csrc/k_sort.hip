@@ -21,8 +21,9 @@
 // packing splits buckets exactly there) and each chunk is sorted like a small tile, reading
 // scratch and writing the tile's range of the entry list.
 //
-// ln(max(depth, 1e-6)) is computed once per splat by the projection (dlog[]), bit-identical to
-// evaluating it here.
+// ln(max(depth, 1e-6)) is computed once per splat by the projection (dlog[], bit-identical to
+// evaluating it here) and gathered once, in grouped order, by the last radix pass (ekey[]), so
+// these kernels read depths as coalesced streams.
 #include "hdsr_internal.h"
 
 namespace hdsr {
@@ -34,19 +35,28 @@ __device__ __forceinline__ float log_pos(uint32_t ldepth_bits, float log_min, fl
   return (hdsr_u2f(ldepth_bits) - log_min) * scale;
 }
 
+// Large tiles.  Wave w of the 1024-thread workgroup owns the contiguous slice [w*S, (w+1)*S)
+// of the tile's entries.  Pass 1: depth min/max.  Pass 2: per-wave bucket counts (ranks inside a
+// round from seven ballots, a wave-private running count across rounds).  Then one scan over
+// (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 3 walks the slice in
+// the same order and writes each entry at slot + its running rank: a stable bucket order with
+// three workgroup barriers per tile.
 __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   __shared__ uint32_t red_min[16], red_max[16];
-  __shared__ uint32_t bbase[HDSR_NUM_BUCKETS], bcnt[HDSR_NUM_BUCKETS], cursor[HDSR_NUM_BUCKETS];
-  __shared__ uint32_t wcnt[16][HDSR_NUM_BUCKETS];
+  __shared__ uint32_t wc[16][HDSR_NUM_BUCKETS];
+  __shared__ uint32_t wsum[2];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t nl = b.stats->large_tiles;
   for (uint32_t l = blockIdx.x; l < nl; l += gridDim.x) {
     const LargeTile L = b.large_tiles[l];
     const uint32_t* src = b.entries + L.start;
+    const uint2* esrc = b.ekey + L.start;
+    const uint32_t S = ((L.count + 1023u) / 1024u) * 64u;
+    const uint32_t lo = w * S, hi = min(lo + S, L.count);
     uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
-    for (uint32_t i = t; i < L.count; i += 1024) {
-      const uint32_t d = b.dlog[src[i]].x;
+    for (uint32_t i = lo + lane; i < hi; i += 64) {
+      const uint32_t d = esrc[i].x;
       dmn = min(dmn, d);
       dmx = max(dmx, d);
     }
@@ -56,7 +66,8 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
       red_min[w] = dmn;
       red_max[w] = dmx;
     }
-    if (t < HDSR_NUM_BUCKETS) bcnt[t] = 0;
+    wc[t >> 7][t & 127] = 0;
+    wc[8 + (t >> 7)][t & 127] = 0;
     __syncthreads();
     dmn = red_min[0];
     dmx = red_max[0];
@@ -67,58 +78,86 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     }
     float lmin, scale;
     hdsr_log_frame(dmn, dmx, &lmin, &scale);
-    for (uint32_t i = t; i < L.count; i += 1024) {
-      const uint32_t k = hdsr_bucket_of(log_pos(b.dlog[src[i]].y, lmin, scale));
-      atomicAdd(&bcnt[k], 1u);
+    // pass 2: wave-private bucket counts, four rounds of loads in flight
+    for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
+      uint32_t kk[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t i = r0 + u * 64 + lane;
+        kk[u] = i < hi ? hdsr_bucket_of(log_pos(esrc[i].y, lmin, scale)) : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const bool valid = r0 + u * 64 + lane < hi;
+        const uint32_t k = kk[u];
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (uint32_t bit = 0; bit < 7; ++bit) {
+          const bool s = (k >> bit) & 1u;
+          const unsigned long long bm = __ballot(s);
+          m &= s ? bm : ~bm;
+        }
+        if (valid && (m >> lane) == 1ull) wc[w][k] += (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+      }
     }
     __syncthreads();
-    if (t < 64) {
-      // exclusive scan of the 128 bucket counts by one wave (two per lane)
-      const uint32_t c0 = bcnt[2 * t], c1 = bcnt[2 * t + 1];
-      uint32_t inc = c0 + c1;
+    // scan over (bucket, wave), bucket-major: threads 0..127 own one bucket each
+    uint32_t tot = 0, inc = 0;
+    if (t < HDSR_NUM_BUCKETS) {
+#pragma unroll
+      for (uint32_t q = 0; q < 16; ++q) tot += wc[q][t];
+      inc = tot;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t v = __shfl_up(inc, d, 64);
-        if (t >= (uint32_t)d) inc += v;
+        const uint32_t x = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += x;
       }
-      const uint32_t exc = inc - c0 - c1;
-      bbase[2 * t] = exc;
-      bbase[2 * t + 1] = exc + c0;
+      if (lane == 63) wsum[w] = inc;
     }
-    if (t < HDSR_NUM_BUCKETS) cursor[t] = 0;
     __syncthreads();
-    for (uint32_t r0 = 0; r0 < L.count; r0 += 1024) {
-      const uint32_t i = r0 + t;
-      const bool valid = i < L.count;
-      const uint32_t g = valid ? src[i] : 0u;
-      const uint32_t k = valid ? hdsr_bucket_of(log_pos(b.dlog[g].y, lmin, scale)) : 0u;
-      unsigned long long m = __ballot(valid);
+    if (t < HDSR_NUM_BUCKETS) {
+      uint32_t run = inc - tot + (w == 1 ? wsum[0] : 0u);
 #pragma unroll
-      for (uint32_t bit = 0; bit < 7; ++bit) {
-        const bool s = (k >> bit) & 1u;
-        const unsigned long long bm = __ballot(s);
-        m &= s ? bm : ~bm;
+      for (uint32_t q = 0; q < 16; ++q) {
+        const uint32_t c = wc[q][t];
+        wc[q][t] = run;
+        run += c;
       }
-      const uint32_t rank = __popcll(m & below);
-      const bool last = valid && ((m >> lane) == 1ull);
-      wcnt[t >> 7][t & 127] = 0;
-      wcnt[8 + (t >> 7)][t & 127] = 0;
-      __syncthreads();
-      if (last) wcnt[w][k] = rank + 1;
-      __syncthreads();
-      if (t < HDSR_NUM_BUCKETS) {
-        uint32_t run = cursor[t];
+    }
+    __syncthreads();
+    // pass 3: stable scatter in bucket order (entries and their depth pairs)
+    for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
+      uint32_t gg[4];
+      uint2 ee[4];
 #pragma unroll
-        for (uint32_t q = 0; q < 16; ++q) {
-          const uint32_t c = wcnt[q][t];
-          wcnt[q][t] = run;
-          run += c;
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t i = r0 + u * 64 + lane;
+        gg[u] = i < hi ? src[i] : 0u;
+        ee[u] = i < hi ? esrc[i] : make_uint2(0, 0);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const bool valid = r0 + u * 64 + lane < hi;
+        const uint32_t k = valid ? hdsr_bucket_of(log_pos(ee[u].y, lmin, scale)) : 0u;
+        unsigned long long m = __ballot(valid);
+#pragma unroll
+        for (uint32_t bit = 0; bit < 7; ++bit) {
+          const bool s = (k >> bit) & 1u;
+          const unsigned long long bm = __ballot(s);
+          m &= s ? bm : ~bm;
         }
-        cursor[t] = run;
+        const uint32_t rank = __popcll(m & below);
+        uint32_t old = 0;
+        if (valid) old = wc[w][k];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (m >> lane) == 1ull) wc[w][k] = old + rank + 1;
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+          b.scratch[L.scratch + old + rank] = gg[u];
+          b.ekey_s[L.scratch + old + rank] = ee[u];
+        }
       }
-      __syncthreads();
-      if (valid) b.scratch[L.scratch + bbase[k] + wcnt[w][k] + rank] = g;
-      __syncthreads();
     }
     if (t == 0) {
       b.large_tiles[l].dmin = dmn;
@@ -222,6 +261,7 @@ __global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
   for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
     const Seg S = b.segs[s];
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
+    const uint2* esrc = (S.large == kNone ? b.ekey : b.ekey_s) + S.src;
     const uint32_t count = S.count;
     uint2 dl[4];
     uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
@@ -230,9 +270,8 @@ __global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
       const uint32_t i = r * kSortThreads + t;
       dl[r] = make_uint2(0, 0);
       if (i < count) {
-        const uint32_t g = src[i];
-        vals[i] = g;
-        dl[r] = b.dlog[g];
+        vals[i] = src[i];
+        dl[r] = esrc[i];
         dmn = min(dmn, dl[r].x);
         dmx = max(dmx, dl[r].x);
       }
