@@ -177,18 +177,38 @@ def main() -> int:
 
 
 def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:
-    """Single-threaded CPU restatement of the same path (the oracle), one view, composite on a
-    sample of tile rows chosen to keep the run near budget_s seconds."""
-    from oracle import oracle
-    tiles_y = plan["tileRows"]
-    # one composited tile row of this workload costs ~5-10 s on one core: sample two rows
-    step = max(1, tiles_y // 2)
-    t = oracle.time_frame(scene, sorted_idx, u, W, H, step)
-    frame_s = t["project_s"] + t["bin_sort_s"] + t["composite_sample_s"] * (tiles_y / len(range(0, tiles_y, step)))
+    """The reference's CPU path (cpuDepthSort + CPU tile-ref population + JS compositing),
+    restated in plain JavaScript (oracle/js/cpu_path.js) and timed under Node on this host, one
+    thread, on one whole frame of view 0 (about 15-20 s).  Falls back to compositing a sample of
+    tile rows if a probe frame would exceed the budget."""
+    import json as _json
+    import shutil
+    import subprocess
+    import tempfile
+    node = shutil.which("node")
+    if node is None:
+        return None
+    js = os.path.join(ROOT, "oracle", "js", "cpu_path.js")
+    with tempfile.TemporaryDirectory() as d:
+        for k, v in {"positions.bin": scene.positions, "scales.bin": scene.scales, "rotations.bin": scene.rotations,
+                     "opacities.bin": scene.opacities, "shdata.bin": scene.sh_data(), "uniforms.bin": u}.items():
+            np.ascontiguousarray(v).tofile(os.path.join(d, k))
+        with open(os.path.join(d, "meta.json"), "w") as fh:
+            _json.dump({"count": scene.count, "width": W, "height": H, "shDegree": scene.sh_degree}, fh)
+        step = 1 if budget_s >= 15 else max(1, plan["tileRows"] // 4)
+        r = subprocess.run([node, "--max-old-space-size=16384", js, d, str(step)], capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            return {"error": r.stderr[-500:]}
+        t = _json.loads(r.stdout.strip().splitlines()[-1])
+    frame_s = t["frame_s_est"]
     return {"value": round(1.0 / frame_s, 5), "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"view 0, full projection + binning + per-tile sort, composite on every {step}th tile row "
-                      f"({len(range(0, tiles_y, step))}/{tiles_y}) scaled to the frame",
-            "frame_s": round(frame_s, 2), "phases_s": {k: round(v, 3) for k, v in t.items()}}
+            "sample": (f"view 0 of the same workload: cpuDepthSort + tile-ref population over all 1M splats, "
+                       f"JS compositing of {t['composite_rows']}/{t['tile_rows']} tile rows"
+                       + (" (the whole frame)" if t["composite_rows"] == t["tile_rows"] else ", scaled to the frame")
+                       + ", Node single thread"),
+            "frame_s": round(frame_s, 3),
+            "phases_s": {k: round(v, 3) for k, v in t.items() if k.endswith("_s")}}
 
 
 if __name__ == "__main__":

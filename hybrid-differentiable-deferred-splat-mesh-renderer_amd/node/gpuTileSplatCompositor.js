@@ -1,0 +1,168 @@
+'use strict';
+// gpuTileSplatCompositor.js - drop-in facade of the reference's src/gpuTileSplatCompositor.ts
+// over the MI355X compositor (hdsr_napi.node -> libhdsr.so).  Node >= 12, CommonJS, no optional
+// chaining or nullish coalescing.
+//
+// Exports the reference's names with the same argument meaning:
+//   TILE_SPLAT_FRAME_UNIFORM_BYTES                                   (:18)
+//   planTileSplatCompositor(input)                                   (:109-139)
+//   createTileSplatCompositor(device, plan, options)                 (:141-728)
+//   writeTileSplatFrameUniforms(target, viewProj, plan, splatScale, shDegree, cameraPos,
+//                               viewMatrix, projMatrix, normalMatrix) (:730-774)
+//   createTileSplatBindGroups(device, resources, splatBuffers, outputTexture, gbufferTextures)
+//                                                                    (:792-995)
+//   encodeFullComputeCompositorPipeline(encoder, resources, bindGroups) (:1023-1179)
+//   encodeCompositeOnly(encoder, resources, bindGroups)              (:1001-1014)
+// GPU objects become typed arrays: splat buffers are Float32Array / Uint32Array in the layouts of
+// src/splats.ts:376-500, the frame uniform buffer is written with
+// device.queue.writeBuffer(resources.frameUniformBuffer, 0, float32x64) as the reference's caller
+// does (src/splatRenderer.ts:1030), and the output textures are read with resources.readRGBA() /
+// readDepth().  createDevice() returns the minimal device object these calls need.
+
+var path = require('path');
+var addon = require(path.join(__dirname, 'hdsr_napi.node'));
+
+var TILE_SPLAT_FRAME_UNIFORM_BYTES = 256;
+
+var FLAGS = { f16: 1, refCaps: 2, gbuffer: 4, debug: 8, rgba16: 16, profile: 32 };
+var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4 };
+var DBG = {
+  projCache: 0, depth: 1, tileCounts: 2, tileOffsets: 3, entries: 4, entriesCanon: 5, smallTileList: 6,
+  largeTileList: 7, largeTileOverflowBases: 8, chunkRanges: 9, keys20: 10, largeSplatList: 11,
+};
+
+function planTileSplatCompositor(input) {
+  return addon.planCompute({
+    viewportWidth: input.viewportWidth,
+    viewportHeight: input.viewportHeight,
+    tileSizePx: input.tileSizePx,
+    splatCount: input.splatCount,
+    tileEntryMultiplier: input.tileEntryMultiplier === undefined ? 0 : input.tileEntryMultiplier,
+  });
+}
+
+function FrameUniformBuffer(resources) {
+  this.label = 'tile_splat_frame_uniforms';
+  this.size = TILE_SPLAT_FRAME_UNIFORM_BYTES;
+  this._resources = resources;
+}
+FrameUniformBuffer.prototype._write = function (data) {
+  var view = data instanceof Float32Array ? data :
+    new Float32Array(data.buffer, data.byteOffset, TILE_SPLAT_FRAME_UNIFORM_BYTES / 4);
+  addon.setFrame(this._resources.handle, view);
+};
+
+function createTileSplatCompositor(device, plan, options) {
+  var opts = options || {};
+  var flags = 0;
+  Object.keys(FLAGS).forEach(function (k) { if (opts[k]) flags |= FLAGS[k]; });
+  var handle = addon.create(plan, flags);
+  var px = plan.viewportWidth * plan.viewportHeight;
+  var resources = {
+    plan: plan,
+    handle: handle,
+    flags: flags,
+    destroy: function () { addon.destroy(handle); },
+    sync: function () { addon.sync(handle); },
+    readRGBA: function () { var a = new Float32Array(px * 4); addon.readback(handle, OUT.rgba, a); return a; },
+    readDepth: function () { var a = new Float32Array(px); addon.readback(handle, OUT.depth, a); return a; },
+    readOutput: function (name, array) { addon.readback(handle, OUT[name], array); return array; },
+    debug: function (name) { return addon.debugReadback(handle, DBG[name]); },
+    stats: function () { return addon.stats(handle); },
+    stageTimes: function () { return addon.stageTimes(handle); },
+    updateSortedIndices: function (sorted) { addon.updateSortedIndices(handle, sorted); },
+  };
+  resources.frameUniformBuffer = new FrameUniformBuffer(resources);
+  return resources;
+}
+
+// Byte-identical 256-byte layout of the reference (gpu_project_splats.wgsl:27-44).
+function writeTileSplatFrameUniforms(target, viewProj, plan, splatScale, shDegree, cameraPos, viewMatrix,
+                                     projMatrix, normalMatrix) {
+  if (splatScale === undefined) splatScale = 1.0;
+  if (shDegree === undefined) shDegree = 0;
+  target.set(viewProj, 0);
+  target[16] = plan.viewportWidth;
+  target[17] = plan.viewportHeight;
+  target[18] = plan.tileSizePx;
+  target[19] = 0;
+  var u32 = new Uint32Array(target.buffer, target.byteOffset, target.length);
+  u32[20] = plan.tileColumns;
+  u32[21] = plan.tileRows;
+  u32[22] = plan.splatCount;
+  u32[23] = plan.maxTotalTileRefs;
+  target[24] = splatScale;
+  u32[25] = shDegree;
+  if (cameraPos) {
+    target[28] = cameraPos[0];
+    target[29] = cameraPos[1];
+    target[30] = cameraPos[2];
+  }
+  if (viewMatrix) target.set(viewMatrix, 32);
+  if (projMatrix) {
+    target[48] = plan.viewportWidth * projMatrix[0] * 0.5;
+    target[49] = plan.viewportHeight * projMatrix[5] * 0.5;
+  }
+  var n = normalMatrix || [1, 0, 0, 0, 1, 0, 0, 0, 1];
+  target[52] = n[0]; target[53] = n[1]; target[54] = n[2]; target[55] = 0;
+  target[56] = n[3]; target[57] = n[4]; target[58] = n[5]; target[59] = 0;
+  target[60] = n[6]; target[61] = n[7]; target[62] = n[8]; target[63] = 0;
+}
+
+function createTileSplatBindGroups(device, resources, splatBuffers, outputTexture, gbufferTextures) {
+  addon.bindSplats(resources.handle, {
+    positions: splatBuffers.positionBuffer,
+    scales: splatBuffers.scaleBuffer,
+    rotations: splatBuffers.rotationBuffer,
+    opacities: splatBuffers.opacityBuffer,
+    materials: splatBuffers.materialBuffer,
+    normals: splatBuffers.normalBuffer,
+    shData: splatBuffers.shDataBuffer,
+    sortedIndices: splatBuffers.sortedIndexBuffer,
+    shDegree: splatBuffers.shDegree === undefined ? 0 : splatBuffers.shDegree,
+  });
+  return { resources: resources, outputTexture: outputTexture, gbufferTextures: gbufferTextures };
+}
+
+function encodeFullComputeCompositorPipeline(encoder, resources, bindGroups) {
+  addon.render(resources.handle);
+}
+
+function encodeCompositeOnly(encoder, resources, bindGroups) {
+  addon.compositeOnly(resources.handle);
+}
+
+// The device object the calls above need: queue.writeBuffer for the frame uniforms and a
+// queue.submit that waits for the enqueued work.
+function createDevice() {
+  var pending = [];
+  return {
+    queue: {
+      writeBuffer: function (buffer, offset, data) {
+        if (!(buffer instanceof FrameUniformBuffer) || offset !== 0) {
+          throw new Error('only the compositor frame uniform buffer can be written');
+        }
+        buffer._write(data);
+      },
+      submit: function (commandBuffers) {
+        (commandBuffers || []).forEach(function (cb) { if (cb && cb.resources) pending.push(cb.resources); });
+        while (pending.length) pending.pop().sync();
+      },
+    },
+    createCommandEncoder: function () {
+      return { finish: function () { return {}; } };
+    },
+  };
+}
+
+module.exports = {
+  TILE_SPLAT_FRAME_UNIFORM_BYTES: TILE_SPLAT_FRAME_UNIFORM_BYTES,
+  planTileSplatCompositor: planTileSplatCompositor,
+  createTileSplatCompositor: createTileSplatCompositor,
+  writeTileSplatFrameUniforms: writeTileSplatFrameUniforms,
+  createTileSplatBindGroups: createTileSplatBindGroups,
+  encodeFullComputeCompositorPipeline: encodeFullComputeCompositorPipeline,
+  encodeCompositeOnly: encodeCompositeOnly,
+  createDevice: createDevice,
+  version: addon.version,
+};

@@ -1,0 +1,443 @@
+/* hdsr_napi.c - Node N-API addon over the C ABI of include/hdsr.h.
+ *
+ * Thin: every function converts JS values (plain objects, typed arrays, an external handle) to
+ * the C structs and calls exactly one hdsr_* entry point; a nonzero return becomes a thrown JS
+ * Error carrying hdsr_last_error().  The handle's finalizer calls hdsr_destroy, so a compositor
+ * that is garbage-collected releases its device memory.  Built with N-API version 8 headers
+ * (Node >= 12.22).  The JavaScript facade with the reference's names is gpuTileSplatCompositor.js.
+ */
+#define NAPI_VERSION 8
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "hdsr.h"
+
+#define CALL(env, call)                                        \
+  do {                                                         \
+    if ((call) != napi_ok) {                                   \
+      napi_throw_error((env), NULL, "N-API call failed: " #call); \
+      return NULL;                                             \
+    }                                                          \
+  } while (0)
+
+static napi_value throw_rc(napi_env env, int rc, hdsr_ctx* ctx, const char* what) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s failed (%d): %s", what, rc, ctx ? hdsr_last_error(ctx) : "");
+  napi_throw_error(env, NULL, buf);
+  return NULL;
+}
+
+static uint32_t get_u32(napi_env env, napi_value obj, const char* key, uint32_t dflt) {
+  bool has = false;
+  napi_value v;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return dflt;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return dflt;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_number) return dflt;
+  uint32_t out = dflt;
+  napi_get_value_uint32(env, v, &out);
+  return out;
+}
+
+static double get_f64(napi_env env, napi_value obj, const char* key, double dflt) {
+  bool has = false;
+  napi_value v;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return dflt;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return dflt;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_number) return dflt;
+  double out = dflt;
+  napi_get_value_double(env, v, &out);
+  return out;
+}
+
+static void set_u32(napi_env env, napi_value obj, const char* key, uint32_t v) {
+  napi_value x;
+  napi_create_uint32(env, v, &x);
+  napi_set_named_property(env, obj, key, x);
+}
+
+static void set_f64(napi_env env, napi_value obj, const char* key, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, key, x);
+}
+
+/* Pointer and byte length of a typed array property (NULL when absent or undefined). */
+static const void* typed_prop(napi_env env, napi_value obj, const char* key, size_t* bytes) {
+  bool has = false;
+  napi_value v;
+  *bytes = 0;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return NULL;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return NULL;
+  bool is_ta = false;
+  napi_is_typedarray(env, v, &is_ta);
+  if (!is_ta) return NULL;
+  napi_typedarray_type type;
+  size_t length, offset;
+  void* data;
+  napi_value ab;
+  if (napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &offset) != napi_ok) return NULL;
+  size_t elem = 1;
+  switch (type) {
+    case napi_int16_array:
+    case napi_uint16_array: elem = 2; break;
+    case napi_int32_array:
+    case napi_uint32_array:
+    case napi_float32_array: elem = 4; break;
+    case napi_float64_array:
+    case napi_bigint64_array:
+    case napi_biguint64_array: elem = 8; break;
+    default: elem = 1;
+  }
+  *bytes = length * elem;
+  return data;
+}
+
+static void plan_to_js(napi_env env, const hdsr_plan* p, napi_value out) {
+  set_u32(env, out, "viewportWidth", p->viewport_width);
+  set_u32(env, out, "viewportHeight", p->viewport_height);
+  set_u32(env, out, "tileSizePx", p->tile_size_px);
+  set_u32(env, out, "tileColumns", p->tile_columns);
+  set_u32(env, out, "tileRows", p->tile_rows);
+  set_u32(env, out, "tileCount", p->tile_count);
+  set_u32(env, out, "mortonTileCount", p->morton_tile_count);
+  set_u32(env, out, "splatCount", p->splat_count);
+  set_u32(env, out, "maxTotalTileRefs", p->max_total_tile_refs);
+}
+
+static void plan_from_js(napi_env env, napi_value obj, hdsr_plan* p) {
+  p->viewport_width = get_u32(env, obj, "viewportWidth", 0);
+  p->viewport_height = get_u32(env, obj, "viewportHeight", 0);
+  p->tile_size_px = get_u32(env, obj, "tileSizePx", 0);
+  p->tile_columns = get_u32(env, obj, "tileColumns", 0);
+  p->tile_rows = get_u32(env, obj, "tileRows", 0);
+  p->tile_count = get_u32(env, obj, "tileCount", 0);
+  p->morton_tile_count = get_u32(env, obj, "mortonTileCount", 0);
+  p->splat_count = get_u32(env, obj, "splatCount", 0);
+  p->max_total_tile_refs = get_u32(env, obj, "maxTotalTileRefs", 0);
+}
+
+/* The external holds a box so that destroy() can release the context early and the finalizer
+ * (run at garbage collection) then only frees the box. */
+typedef struct {
+  hdsr_ctx* ctx;
+} ctx_box;
+
+static ctx_box* box_arg(napi_env env, napi_value v) {
+  void* p = NULL;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, NULL, "expected a compositor handle");
+    return NULL;
+  }
+  return (ctx_box*)p;
+}
+
+static hdsr_ctx* ctx_arg(napi_env env, napi_value v) {
+  ctx_box* b = box_arg(env, v);
+  if (!b) return NULL;
+  if (!b->ctx) {
+    napi_throw_error(env, NULL, "compositor handle already destroyed");
+    return NULL;
+  }
+  return b->ctx;
+}
+
+static void finalize_ctx(napi_env env, void* data, void* hint) {
+  (void)env;
+  (void)hint;
+  ctx_box* b = (ctx_box*)data;
+  if (b && b->ctx) hdsr_destroy(b->ctx);
+  free(b);
+}
+
+/* planCompute({viewportWidth, viewportHeight, tileSizePx, splatCount, tileEntryMultiplier?}) */
+static napi_value js_plan(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1) {
+    napi_throw_type_error(env, NULL, "planCompute(input)");
+    return NULL;
+  }
+  hdsr_plan_in in;
+  in.viewport_width = get_u32(env, argv[0], "viewportWidth", 0);
+  in.viewport_height = get_u32(env, argv[0], "viewportHeight", 0);
+  in.tile_size_px = get_u32(env, argv[0], "tileSizePx", 0);
+  in.splat_count = get_u32(env, argv[0], "splatCount", 0);
+  in.tile_entry_multiplier = (float)get_f64(env, argv[0], "tileEntryMultiplier", 0.0);
+  hdsr_plan p;
+  const int rc = hdsr_plan_compute(&in, &p);
+  if (rc) return throw_rc(env, rc, NULL, "hdsr_plan_compute");
+  napi_value out;
+  CALL(env, napi_create_object(env, &out));
+  plan_to_js(env, &p, out);
+  return out;
+}
+
+/* create(plan, flags) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 1) {
+    napi_throw_type_error(env, NULL, "create(plan, flags)");
+    return NULL;
+  }
+  hdsr_plan p;
+  plan_from_js(env, argv[0], &p);
+  uint32_t flags = 0;
+  if (argc > 1) napi_get_value_uint32(env, argv[1], &flags);
+  hdsr_ctx* ctx = NULL;
+  const int rc = hdsr_create(&p, flags, &ctx);
+  if (rc) return throw_rc(env, rc, NULL, "hdsr_create");
+  ctx_box* box = (ctx_box*)malloc(sizeof(ctx_box));
+  if (!box) {
+    hdsr_destroy(ctx);
+    napi_throw_error(env, NULL, "out of host memory");
+    return NULL;
+  }
+  box->ctx = ctx;
+  napi_value h;
+  if (napi_create_external(env, box, finalize_ctx, NULL, &h) != napi_ok) {
+    hdsr_destroy(ctx);
+    free(box);
+    napi_throw_error(env, NULL, "napi_create_external failed");
+    return NULL;
+  }
+  return h;
+}
+
+/* destroy(handle): releases now; the handle must not be used again. */
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  ctx_box* b = box_arg(env, argv[0]);
+  if (!b) return NULL;
+  if (b->ctx) hdsr_destroy(b->ctx);
+  b->ctx = NULL;
+  return NULL;
+}
+
+/* bindSplats(handle, {positions, scales, rotations, opacities, materials, normals?, shData,
+ *                     sortedIndices, shDegree})  -- host typed arrays, copied to the device */
+static napi_value js_bind(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  size_t nb = 0, tmp = 0;
+  hdsr_splats s;
+  memset(&s, 0, sizeof s);
+  s.positions = (const float*)typed_prop(env, argv[1], "positions", &nb);
+  s.count = (uint32_t)(nb / 12);
+  s.scales = (const float*)typed_prop(env, argv[1], "scales", &tmp);
+  s.rotations = (const float*)typed_prop(env, argv[1], "rotations", &tmp);
+  s.opacities = (const float*)typed_prop(env, argv[1], "opacities", &tmp);
+  s.materials = (const uint32_t*)typed_prop(env, argv[1], "materials", &tmp);
+  s.normals = (const float*)typed_prop(env, argv[1], "normals", &tmp);
+  s.normals_len = tmp / 4;
+  s.sh_data = (const float*)typed_prop(env, argv[1], "shData", &tmp);
+  s.sorted_indices = (const uint32_t*)typed_prop(env, argv[1], "sortedIndices", &tmp);
+  s.sh_degree = get_u32(env, argv[1], "shDegree", 0);
+  s.memory = HDSR_MEM_HOST;
+  const int rc = hdsr_bind_splats(ctx, &s);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_bind_splats");
+  return NULL;
+}
+
+static napi_value js_update_sorted(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  napi_value holder;
+  CALL(env, napi_create_object(env, &holder));
+  CALL(env, napi_set_named_property(env, holder, "a", argv[1]));
+  size_t bytes = 0;
+  const void* p = typed_prop(env, holder, "a", &bytes);
+  const int rc = hdsr_update_sorted_indices(ctx, (const uint32_t*)p, HDSR_MEM_HOST);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_update_sorted_indices");
+  return NULL;
+}
+
+/* setFrame(handle, Float32Array(64) | ArrayBuffer view of 256 bytes) */
+static napi_value js_set_frame(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  napi_value holder;
+  CALL(env, napi_create_object(env, &holder));
+  CALL(env, napi_set_named_property(env, holder, "a", argv[1]));
+  size_t bytes = 0;
+  const void* p = typed_prop(env, holder, "a", &bytes);
+  if (!p || bytes < HDSR_FRAME_UNIFORM_BYTES) {
+    napi_throw_type_error(env, NULL, "frame uniforms must be a 256-byte typed array");
+    return NULL;
+  }
+  const int rc = hdsr_set_frame(ctx, p);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_set_frame");
+  return NULL;
+}
+
+static napi_value js_render(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  const int rc = hdsr_render(ctx, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_render");
+  return NULL;
+}
+
+static napi_value js_composite_only(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  const int rc = hdsr_composite_only(ctx, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_composite_only");
+  return NULL;
+}
+
+static napi_value js_sync(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  const int rc = hdsr_sync(ctx);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_sync");
+  return NULL;
+}
+
+/* readback(handle, which, typedArray) copies an output into the array */
+static napi_value js_readback(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  int32_t which = 0;
+  napi_get_value_int32(env, argv[1], &which);
+  napi_value holder;
+  CALL(env, napi_create_object(env, &holder));
+  CALL(env, napi_set_named_property(env, holder, "a", argv[2]));
+  size_t bytes = 0;
+  void* p = (void*)typed_prop(env, holder, "a", &bytes);
+  const int rc = hdsr_readback(ctx, which, p, bytes);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_readback");
+  return NULL;
+}
+
+/* debugReadback(handle, which) -> Uint32Array */
+static napi_value js_debug(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  int32_t which = 0;
+  napi_get_value_int32(env, argv[1], &which);
+  uint64_t bytes = 0;
+  int rc = hdsr_debug_size(ctx, which, &bytes);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_debug_size");
+  void* data = NULL;
+  napi_value ab, ta;
+  CALL(env, napi_create_arraybuffer(env, (size_t)bytes, &data, &ab));
+  rc = hdsr_debug_readback(ctx, which, data, bytes);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_debug_readback");
+  CALL(env, napi_create_typedarray(env, napi_uint32_array, (size_t)(bytes / 4), ab, 0, &ta));
+  return ta;
+}
+
+static napi_value js_stats(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  hdsr_stats s;
+  const int rc = hdsr_get_stats(ctx, &s);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_get_stats");
+  napi_value o;
+  CALL(env, napi_create_object(env, &o));
+  set_u32(env, o, "refs", s.refs);
+  set_u32(env, o, "smallPairs", s.small_pairs);
+  set_u32(env, o, "largePairs", s.large_pairs);
+  set_u32(env, o, "largeSplats", s.large_splats);
+  set_u32(env, o, "visibleSplats", s.visible_splats);
+  set_u32(env, o, "smallTiles", s.small_tiles);
+  set_u32(env, o, "largeTiles", s.large_tiles);
+  set_u32(env, o, "chunks", s.chunks);
+  set_u32(env, o, "segments", s.segments);
+  set_u32(env, o, "pairCapacity", s.pair_capacity);
+  set_u32(env, o, "overflowReruns", s.overflow_reruns);
+  set_u32(env, o, "capRefsTripped", s.cap_refs_tripped);
+  set_u32(env, o, "capLargeSplatsTripped", s.cap_large_splats_tripped);
+  set_u32(env, o, "capChunksTripped", s.cap_chunks_tripped);
+  set_u32(env, o, "compositeRefs", s.composite_refs);
+  set_u32(env, o, "compositeMaxRefs", s.composite_max_refs);
+  return o;
+}
+
+static napi_value js_stage_times(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  float ms[HDSR_STAGE_COUNT];
+  uint32_t n = 0;
+  const int rc = hdsr_stage_times(ctx, ms, &n);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_stage_times");
+  static const char* names[HDSR_STAGE_COUNT] = {"project", "scan", "emit", "group", "ranges",
+                                                "classify", "bucket", "sort", "composite"};
+  napi_value o;
+  CALL(env, napi_create_object(env, &o));
+  for (int i = 0; i < HDSR_STAGE_COUNT; ++i) set_f64(env, o, names[i], ms[i]);
+  set_u32(env, o, "renders", n);
+  return o;
+}
+
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value s;
+  CALL(env, napi_create_string_utf8(env, hdsr_version(), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+static napi_value init(napi_env env, napi_value exports) {
+  const napi_property_descriptor props[] = {
+      {"planCompute", NULL, js_plan, NULL, NULL, NULL, napi_default, NULL},
+      {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
+      {"destroy", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
+      {"bindSplats", NULL, js_bind, NULL, NULL, NULL, napi_default, NULL},
+      {"updateSortedIndices", NULL, js_update_sorted, NULL, NULL, NULL, napi_default, NULL},
+      {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_default, NULL},
+      {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},
+      {"compositeOnly", NULL, js_composite_only, NULL, NULL, NULL, napi_default, NULL},
+      {"sync", NULL, js_sync, NULL, NULL, NULL, napi_default, NULL},
+      {"readback", NULL, js_readback, NULL, NULL, NULL, napi_default, NULL},
+      {"debugReadback", NULL, js_debug, NULL, NULL, NULL, napi_default, NULL},
+      {"stats", NULL, js_stats, NULL, NULL, NULL, napi_default, NULL},
+      {"stageTimes", NULL, js_stage_times, NULL, NULL, NULL, napi_default, NULL},
+      {"version", NULL, js_version, NULL, NULL, NULL, napi_default, NULL},
+  };
+  napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

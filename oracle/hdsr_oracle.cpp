@@ -636,6 +636,7 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
           float gdw = 0, gws = 0;
           float mr = 0, mm = 0, nx = 0, ny = 0, nz = 0, gconf = 0, er = 0, eg = 0, eb = 0;
           for (uint32_t i = 0; i < c; ++i) {
+            if (T < 0.001f) break;  // no later splat passes 'T >= 0.001': nothing changes
             const uint32_t r = F.entries[s + i];
             const uint32_t* w = &F.cache[(size_t)r * 14];
             const float cx = u2f(w[0]), cy = u2f(w[1]);
@@ -734,6 +735,7 @@ void composite64(Frame& F) {
           const double qx = (double)X + 0.5, qy = (double)Y + 0.5;
           double T = 1, cr = 0, cg = 0, cb = 0, gdw = 0, gws = 0;
           for (uint32_t i = 0; i < c; ++i) {
+            if (T < 0.001) break;
             const Proj<double>& p = P[F.entries[s + i]];
             if (!p.visible) continue;
             const double dx = qx - p.px, dy = qy - p.py;
