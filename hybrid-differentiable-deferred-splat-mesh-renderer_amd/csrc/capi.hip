@@ -4,7 +4,7 @@
 #include <memory>
 #include <string>
 
-#include "../../include/hdsr.h"
+#include "hdsr.h"
 #include "engine.h"
 
 using namespace hdsr;

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hdsr.h"
+#include "hdsr.h"
 #include "hdsr_internal.h"
 
 namespace hdsr {
@@ -39,6 +39,12 @@ class Engine {
   int run(const FrameDesc* frames, uint32_t V, hipStream_t stream, bool gbuffer, bool debug);
   // Re-runs only the rasterizer on the previous run's tile lists (same frames, new outputs ok).
   int composite_only(const FrameDesc* frames, uint32_t V, hipStream_t stream, bool gbuffer);
+  // As run(), through a hipGraph: the stage sequence is captured once per batch shape (the
+  // totals that set grid sizes) and replayed afterwards.  Per-frame data (scene and output
+  // pointers, cameras, block / tile tables) lives in the device frame table, which every replay
+  // re-uploads from a fixed pinned buffer, so one graph serves any mix with the same totals.
+  // *replayed tells whether a cached graph was used.  The stream must not be the null stream.
+  int run_graph(const FrameDesc* frames, uint32_t V, hipStream_t stream, bool gbuffer, bool* replayed);
 
   // Host copy of the device counters of the last run (valid after wait()).
   const DevStats& host_stats() const { return *hstats_; }
@@ -60,6 +66,16 @@ class Engine {
  private:
   int ensure(const Layout& L, bool debug);
   int upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t stream);
+  void enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, bool debug, hipEvent_t* ev);
+  void drop_graphs();
+  struct GraphEntry {
+    uint64_t sig[6];
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs_;
+  uint64_t epoch_ = 0;  // bumped by every (re)allocation: graphs bake buffer addresses
+  void* gstage_ = nullptr;
+  size_t gstage_bytes_ = 0;
   void release();
   template <typename T>
   int grow(T** p, size_t* have, size_t need);

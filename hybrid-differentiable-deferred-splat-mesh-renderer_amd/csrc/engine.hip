@@ -67,6 +67,10 @@ Layout Engine::layout(FrameDesc* frames, uint32_t V) {
 Engine::~Engine() { release(); }
 
 void Engine::release() {
+  drop_graphs();
+  if (gstage_) (void)hipHostFree(gstage_);
+  gstage_ = nullptr;
+  gstage_bytes_ = 0;
   void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
                   b_.keys_a, b_.vals_a, b_.keys_b, b_.vals_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch, b_.ekey, b_.ekey_s,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
@@ -99,6 +103,7 @@ int Engine::grow(T** p, size_t* have, size_t need) {
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *have = 0;
+  ++epoch_;
   const size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
   if (hipMalloc((void**)p, bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -276,23 +281,7 @@ int Engine::upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t 
   return 0;
 }
 
-int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool gbuffer, bool debug) {
-  if (V == 0 || V > kMaxFrames) {
-    err = "frame count out of range";
-    return HDSR_E_INVALID_ARG;
-  }
-  std::vector<FrameDesc> frames(frames_in, frames_in + V);
-  Layout L = layout(frames.data(), V);
-  int rc = ensure(L, debug);
-  if (rc) return rc;
-  lay_ = L;
-  if ((rc = upload_frames(frames.data(), L, stream))) return rc;
-  Bufs b = b_;
-  if (!debug) {
-    b.keys20 = nullptr;
-    b.entries_canon = nullptr;
-  }
-  hipEvent_t* ev = profile_ ? prof_slot() : nullptr;
+void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, bool debug, hipEvent_t* ev) {
   launch_clear_stats(b, stream);
   if (ev) (void)hipEventRecord(ev[0], stream);
   launch_project(b, stream);
@@ -315,6 +304,116 @@ int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool
   if (ev) (void)hipEventRecord(ev[8], stream);
   launch_composite(b, stream, gbuffer);
   if (ev) (void)hipEventRecord(ev[9], stream);
+}
+
+void Engine::drop_graphs() {
+  for (GraphEntry& g : graphs_) (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+}
+
+int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool gbuffer, bool* replayed) {
+  *replayed = false;
+  if (V == 0 || V > kMaxFrames) {
+    err = "frame count out of range";
+    return HDSR_E_INVALID_ARG;
+  }
+  std::vector<FrameDesc> frames(frames_in, frames_in + V);
+  Layout L = layout(frames.data(), V);
+  const uint64_t epoch0 = epoch_;
+  int rc = ensure(L, false);
+  if (rc) return rc;
+  lay_ = L;
+  const size_t fb = sizeof(FrameDesc) * V, sb = 8 * (size_t)(V + 1);
+  if (fb + sb > gstage_bytes_) {
+    if (gstage_) (void)hipHostFree(gstage_);
+    gstage_ = nullptr;
+    gstage_bytes_ = 0;
+    if (hipHostMalloc(&gstage_, fb + sb, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      err = "pinned host allocation failed";
+      return HDSR_E_OOM;
+    }
+    gstage_bytes_ = fb + sb;
+    ++epoch_;
+  }
+  if (epoch_ != epoch0) drop_graphs();
+  // the previous replay on this engine must be done reading the staging buffer
+  if (done_ev_ && hipEventSynchronize(done_ev_) != hipSuccess) {
+    err = "hipEventSynchronize failed";
+    return HDSR_E_HIP;
+  }
+  char* st = static_cast<char*>(gstage_);
+  memcpy(st, frames.data(), fb);
+  uint32_t* tab = reinterpret_cast<uint32_t*>(st + fb);
+  for (uint32_t v = 0; v <= V; ++v) {
+    tab[v] = L.block_start[v];
+    tab[V + 1 + v] = L.tile_start[v];
+  }
+  const uint64_t sig[6] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.key_bits, (uint64_t)gbuffer};
+  hipGraphExec_t exec = nullptr;
+  for (const GraphEntry& g : graphs_)
+    if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
+  if (!exec) {
+    Bufs b = b_;
+    b.keys20 = nullptr;
+    b.entries_canon = nullptr;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      err = "hipStreamBeginCapture failed";
+      return HDSR_E_HIP;
+    }
+    (void)hipMemcpyAsync(d_frames_, st, fb, hipMemcpyHostToDevice, stream);
+    (void)hipMemcpyAsync(d_starts_, tab, sb, hipMemcpyHostToDevice, stream);
+    enqueue_stages(b, stream, gbuffer, false, nullptr);
+    (void)hipMemcpyAsync(hstats_, b.stats, sizeof(DevStats), hipMemcpyDeviceToHost, stream);
+    if (hipStreamEndCapture(stream, &graph) != hipSuccess || !graph) {
+      (void)hipGetLastError();
+      err = "graph capture failed";
+      return HDSR_E_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      (void)hipGetLastError();
+      err = "hipGraphInstantiate failed";
+      return HDSR_E_HIP;
+    }
+    if (graphs_.size() >= 8) {
+      (void)hipGraphExecDestroy(graphs_.front().exec);
+      graphs_.erase(graphs_.begin());
+    }
+    GraphEntry e;
+    memcpy(e.sig, sig, sizeof sig);
+    e.exec = exec;
+    graphs_.push_back(e);
+  } else {
+    *replayed = true;
+  }
+  if (hipGraphLaunch(exec, stream) != hipSuccess || hipEventRecord(done_ev_, stream) != hipSuccess) {
+    err = std::string("graph launch failed: ") + hipGetErrorString(hipGetLastError());
+    return HDSR_E_HIP;
+  }
+  pending_ = true;
+  return 0;
+}
+
+int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool gbuffer, bool debug) {
+  if (V == 0 || V > kMaxFrames) {
+    err = "frame count out of range";
+    return HDSR_E_INVALID_ARG;
+  }
+  std::vector<FrameDesc> frames(frames_in, frames_in + V);
+  Layout L = layout(frames.data(), V);
+  int rc = ensure(L, debug);
+  if (rc) return rc;
+  lay_ = L;
+  if ((rc = upload_frames(frames.data(), L, stream))) return rc;
+  Bufs b = b_;
+  if (!debug) {
+    b.keys20 = nullptr;
+    b.entries_canon = nullptr;
+  }
+  enqueue_stages(b, stream, gbuffer, debug, profile_ ? prof_slot() : nullptr);
   if (hipMemcpyAsync(hstats_, b.stats, sizeof(DevStats), hipMemcpyDeviceToHost, stream) != hipSuccess ||
       hipEventRecord(done_ev_, stream) != hipSuccess) {
     err = "stats copy failed";

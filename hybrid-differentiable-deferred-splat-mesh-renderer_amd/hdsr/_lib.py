@@ -70,6 +70,29 @@ class Stats(C.Structure):
         "cap_chunks_tripped", "composite_refs", "composite_max_refs")]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_splats_per_batch", C.c_uint64),
+                ("max_views_per_batch", C.c_uint32), ("slots", C.c_uint32), ("use_graphs", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene_id", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("reserved", C.c_uint32),
+                ("uniforms256", C.c_void_p)]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [("done", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("batch_views", C.c_uint32),
+                ("latency_ms", C.c_double), ("rgba", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("frames", C.c_uint64), ("batches", C.c_uint64), ("graph_replays", C.c_uint64),
+                ("uploads", C.c_uint64), ("evictions", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("resident_scenes", C.c_uint32), ("queued", C.c_uint32), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_batch_views", C.c_double), ("busy_s", C.c_double)]
+
+
 # Every symbol include/hdsr.h declares: (name, restype, argtypes).
 SIGNATURES = [
     ("hdsr_plan_compute", C.c_int, [C.POINTER(PlanIn), C.POINTER(Plan)]),
@@ -89,6 +112,19 @@ SIGNATURES = [
     ("hdsr_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     ("hdsr_last_error", C.c_char_p, [C.c_void_p]),
     ("hdsr_version", C.c_char_p, []),
+    ("hdsr_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    ("hdsr_server_destroy", None, [C.c_void_p]),
+    ("hdsr_server_add_scene", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Splats)]),
+    ("hdsr_server_remove_scene", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("hdsr_server_submit", C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    ("hdsr_server_wait", C.c_int, [C.c_void_p, C.c_uint64, C.c_double]),
+    ("hdsr_server_result", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(ResultInfo)]),
+    ("hdsr_server_read", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]),
+    ("hdsr_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("hdsr_server_drain", C.c_int, [C.c_void_p]),
+    ("hdsr_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    ("hdsr_server_reset_stats", C.c_int, [C.c_void_p]),
+    ("hdsr_server_last_error", C.c_char_p, [C.c_void_p]),
 ]
 
 _lib = None

@@ -193,6 +193,80 @@ int hdsr_stage_times(hdsr_ctx* ctx, float* ms, uint32_t* renders);
 const char* hdsr_last_error(hdsr_ctx* ctx);
 const char* hdsr_version(void);
 
+/* ---- Serving form -------------------------------------------------------------------------
+ * No counterpart in the reference (its only frame scheduler is the browser's
+ * requestAnimationFrame loop, src/main.ts:846; the closest host logic is src/renderDemand.ts).
+ * A server keeps many scenes resident in device memory under a byte budget (least recently used
+ * scenes are evicted; a scene is uploaded again when a request needs it), and a worker thread
+ * batches the queued camera requests - any mix of scenes and resolutions - into one launch per
+ * pipeline stage (up to max_views_per_batch frames), on one of 'slots' in-flight streams, so
+ * requests that arrive while a batch runs are admitted into the next one as soon as a slot is
+ * free.  Repeated batch shapes replay a captured hipGraph.  Each request's result is the frame
+ * the plain path renders for the same scene (with the identity sortedIndices unless the scene
+ * was registered with its own) and camera: batching changes no output bit. */
+typedef struct hdsr_server hdsr_server;
+
+typedef struct hdsr_server_config {
+  uint64_t hbm_budget_bytes;      /* resident scene bytes; 0 = no limit */
+  uint64_t max_splats_per_batch;  /* sum of scene sizes of one batch; 0 = 32M */
+  uint32_t max_views_per_batch;   /* <= 64; 0 = 64 */
+  uint32_t slots;                 /* batches in flight; 0 = 2 */
+  uint32_t use_graphs;            /* replay captured hipGraphs for repeated batch shapes */
+  uint32_t reserved;
+} hdsr_server_config;
+
+typedef struct hdsr_request {
+  uint32_t scene_id;
+  uint32_t width;
+  uint32_t height;
+  uint32_t reserved;
+  const void* uniforms256;        /* the frame's 256-byte uniform block (copied at submit) */
+} hdsr_request;
+
+typedef struct hdsr_result_info {
+  uint32_t done;                  /* 1 once the frame is complete */
+  uint32_t width;
+  uint32_t height;
+  uint32_t batch_views;           /* frames in the batch this request ran in */
+  double latency_ms;              /* submit -> completion observed */
+  float* rgba;                    /* device H*W*4 f32 (valid until hdsr_server_release) */
+  float* depth;                   /* device H*W f32 */
+} hdsr_result_info;
+
+typedef struct hdsr_server_stats {
+  uint64_t frames;                /* completed requests since the last reset */
+  uint64_t batches;
+  uint64_t graph_replays;
+  uint64_t uploads;               /* scene uploads (first admission or after eviction) */
+  uint64_t evictions;
+  uint64_t resident_bytes;
+  uint32_t resident_scenes;
+  uint32_t queued;
+  double p50_ms;                  /* latency percentiles over the completed requests */
+  double p99_ms;
+  double mean_batch_views;
+  double busy_s;                  /* wall time from the first submit to the last completion */
+} hdsr_server_stats;
+
+int hdsr_server_create(const hdsr_server_config* cfg, hdsr_server** out);
+void hdsr_server_destroy(hdsr_server* srv);
+/* Registers a scene from host memory (HDSR_MEM_HOST; the arrays must stay valid while the scene
+ * is registered: eviction drops the device copy only).  sorted_indices may be NULL (identity). */
+int hdsr_server_add_scene(hdsr_server* srv, uint32_t scene_id, const hdsr_splats* splats);
+int hdsr_server_remove_scene(hdsr_server* srv, uint32_t scene_id);
+int hdsr_server_submit(hdsr_server* srv, const hdsr_request* req, uint64_t* ticket);
+/* Blocks until the ticket's frame is complete (timeout_s <= 0: no limit). */
+int hdsr_server_wait(hdsr_server* srv, uint64_t ticket, double timeout_s);
+int hdsr_server_result(hdsr_server* srv, uint64_t ticket, hdsr_result_info* info);
+int hdsr_server_read(hdsr_server* srv, uint64_t ticket, int which, void* host, uint64_t bytes);
+/* Returns the ticket's output buffers to the pool; the ticket is forgotten. */
+int hdsr_server_release(hdsr_server* srv, uint64_t ticket);
+/* Waits until every submitted request is complete. */
+int hdsr_server_drain(hdsr_server* srv);
+int hdsr_server_get_stats(hdsr_server* srv, hdsr_server_stats* out);
+int hdsr_server_reset_stats(hdsr_server* srv);
+const char* hdsr_server_last_error(hdsr_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

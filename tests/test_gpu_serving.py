@@ -1,0 +1,123 @@
+"""Serving form: a request's frame is the same alone and inside a continuous batch.
+
+Scenes of different sizes and SH degrees, requests at mixed resolutions and poses.  Every
+request is rendered alone by the plain C-ABI path (identity sortedIndices, as the server uses) and
+then submitted together to a batching server (several slots, graph replay, eviction under a small
+byte budget); each result must equal its alone image bit for bit.
+"""
+import numpy as np
+import pytest
+
+from hdsr import camera, compositor, scenes
+from hdsr.serving import Server, poisson_run
+from scenes_for_tests import dense_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def make_scenes():
+    a = scenes.scene_config0(n=9000, seed=101)
+    b = dense_scene(n=14000, seed=102, sh_degree=3, core_fraction=0.4)
+    c = scenes.scene_config1(n=20000, seed=103)
+    return {1: a, 2: b, 3: c}
+
+
+def make_requests(sc, k=18, seed=5):
+    rng = np.random.default_rng(seed)
+    res = [(320, 240), (640, 480), (480, 272)]
+    out = []
+    for i in range(k):
+        sid = [1, 2, 3][i % 3]
+        w, h = res[int(rng.integers(0, 3))]
+        cam = camera.Camera(camera.orbit_eye(float(rng.uniform(2.5, 3.5)), float(rng.uniform(0, 6.28)),
+                                             float(rng.uniform(-0.3, 0.5))), (0, 0, 0), 500.0 * h / 480,
+                            500.0 * h / 480, w, h)
+        out.append((sid, cam))
+    return out
+
+
+def alone(scene, cam):
+    plan = compositor.planTileSplatCompositor({"viewportWidth": cam.width, "viewportHeight": cam.height,
+                                               "tileSizePx": 16, "splatCount": scene.count})
+    res = compositor.createTileSplatCompositor(None, plan)
+    compositor.bind_scene(res, scene, np.arange(scene.count, dtype=np.uint32))
+    res.set_frame(camera.frame_uniforms_for(cam, plan, scene.sh_degree))
+    compositor.encodeFullComputeCompositorPipeline(None, res)
+    out = (res.read_rgba(), res.read_depth())
+    res.destroy()
+    return out
+
+
+@pytest.fixture(scope="module")
+def workload(hip_device):
+    sc = make_scenes()
+    reqs = make_requests(sc)
+    ref = [alone(sc[sid], cam) for sid, cam in reqs]
+    return sc, reqs, ref
+
+
+@pytest.mark.parametrize("graphs,budget", [(True, 0), (False, 0), (True, 6 << 20)])
+def test_batched_equals_alone(workload, graphs, budget):
+    sc, reqs, ref = workload
+    srv = Server(hbm_budget_bytes=budget, max_views_per_batch=64, slots=2, use_graphs=graphs)
+    for sid, s in sc.items():
+        srv.add_scene(sid, s)
+    for rep in range(2):  # the second round replays captured graphs
+        tickets = [srv.submit(sid, cam) for sid, cam in reqs]
+        srv.drain()
+        views = []
+        for t, (rgba, depth) in zip(tickets, ref):
+            info = srv.result(t)
+            assert info["done"]
+            views.append(info["batch_views"])
+            np.testing.assert_array_equal(srv.read_rgba(t), rgba)
+            np.testing.assert_array_equal(srv.read_depth(t), depth)
+            srv.release(t)
+        assert max(views) > 1, "requests were never batched"
+    st = srv.stats()
+    if budget:
+        assert st["evictions"] > 0 and st["uploads"] > 3
+    srv.close()
+
+
+def test_graph_replay_equals_alone(workload):
+    # batch shape depends on arrival timing; waiting on each request makes every batch one frame
+    # of the same shape, so every batch after the first replays the captured graph
+    sc, reqs, ref = workload
+    srv = Server(max_views_per_batch=64, slots=1, use_graphs=True)
+    for sid, s in sc.items():
+        srv.add_scene(sid, s)
+    picks = [i for i, (sid, cam) in enumerate(reqs) if sid == reqs[0][0] and cam.width == reqs[0][1].width]
+    for i in picks * 2:
+        t = srv.submit(*reqs[i])
+        srv.wait(t)
+        np.testing.assert_array_equal(srv.read_rgba(t), ref[i][0])
+        srv.release(t)
+    assert srv.stats()["graph_replays"] >= len(picks) * 2 - 1
+    srv.close()
+
+
+def test_unbatched_server_equals_alone(workload):
+    sc, reqs, ref = workload
+    srv = Server(max_views_per_batch=1, slots=1, use_graphs=False)
+    for sid, s in sc.items():
+        srv.add_scene(sid, s)
+    for (sid, cam), (rgba, _) in list(zip(reqs, ref))[:6]:
+        t = srv.submit(sid, cam)
+        srv.wait(t)
+        assert srv.result(t)["batch_views"] == 1
+        np.testing.assert_array_equal(srv.read_rgba(t), rgba)
+        srv.release(t)
+    srv.close()
+
+
+def test_poisson_stream_completes(workload):
+    sc, reqs, _ = workload
+    srv = Server(max_views_per_batch=64, slots=2)
+    for sid, s in sc.items():
+        srv.add_scene(sid, s)
+    pre = [(sid, cam.width, cam.height, srv.uniforms(sid, cam)) for sid, cam in reqs]
+    st = poisson_run(srv, pre, rate_hz=400.0, duration_s=0.5)
+    assert st["frames"] == st["submitted"] > 50
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
