@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1_000_000, help="splats (default: the 1M SH3 workload)")
     ap.add_argument("--views", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0, help="target CPU seconds for the sample")
     return ap.parse_args()
 
@@ -139,6 +140,8 @@ def main() -> int:
     dom_bytes = algorithmic_bytes(dom, scene.count, k, mean_st, pixels, comp_refs)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
 
+    res.destroy()
+    serving = None if a.no_serving else serving_bench()
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, sorted_host[0], uniforms[0], W, H, plan, a.cpu_budget_s)
@@ -170,10 +173,67 @@ def main() -> int:
         "vs_cpu": None if not cpu else round(fps / cpu["value"], 1),
         "setup_s": round(setup_s, 1),
         "profiled_renders": renders,
+        "serving": serving,
     }
     print(json.dumps(out))
-    res.destroy()
     return 0
+
+
+def serving_bench(rates=(100, 200, 400, 600, 800, 1200, 3000, 5000), duration_s=1.0, max_requests=1500, seed=3) -> dict:
+    """Serving workload: 32 resident scenes with sizes log-uniform in [50k, 4M] splats, SH0 or SH3
+    by coin (each a distinct slice of a 4M-splat seeded scene of that degree), 256 requests of
+    random scene / orbit pose / resolution in {640x480, 1280x720, 1920x1080}, submitted as a Poisson
+    stream at each rate.  The batching server (64 views per launch, 2 slots, graph replay, 200 GB
+    budget) is compared with the plain per-request drop-in (one frame per launch, one slot)."""
+    import math
+
+    from hdsr import scenes
+    from hdsr.camera import Camera, orbit_eye
+    from hdsr.serving import Server, poisson_run
+    rng = np.random.default_rng(seed)
+    base = {3: scenes.scene_config1(4_000_000, seed=31, sh_degree=3), 0: scenes.scene_config1(4_000_000, seed=30,
+                                                                                              sh_degree=0)}
+    sizes = np.exp(rng.uniform(math.log(50_000), math.log(4_000_000), 32)).astype(np.int64)
+    degs = rng.integers(0, 2, 32) * 3
+    scene_list = []
+    for i in range(32):
+        b = base[int(degs[i])]
+        off = int(rng.integers(0, b.count - sizes[i] + 1))
+        scene_list.append(b.subset(np.arange(off, off + sizes[i])))
+    res = [(640, 480), (1280, 720), (1920, 1080)]
+    out = {"scenes": 32, "total_splats": int(sizes.sum()), "rates_hz": list(rates), "duration_s": duration_s}
+    for mode, kw in (("batched", dict(max_views_per_batch=64, slots=2, use_graphs=True)),
+                     ("per_request", dict(max_views_per_batch=1, slots=1, use_graphs=False))):
+        srv = Server(hbm_budget_bytes=200 << 30, **kw)
+        for i, s in enumerate(scene_list):
+            srv.add_scene(i, s)
+        pool = []
+        for k in range(256):
+            sid = int(rng.integers(0, 32))
+            w, h = res[int(rng.integers(0, 3))]
+            cam = Camera(orbit_eye(float(rng.uniform(2.5, 3.5)), float(rng.uniform(0, 2 * math.pi)),
+                                   float(rng.uniform(-0.3, 0.6))), (0, 0, 0), 1500.0 * h / 1080, 1500.0 * h / 1080, w, h)
+            pool.append((sid, w, h, srv.uniforms(sid, cam)))
+        # admit every scene and settle capacities before measuring
+        poisson_run(srv, [(i, 640, 480, srv.uniforms(i, Camera(orbit_eye(3.0, 0.0, 0.2), (0, 0, 0), 700, 700, 640,
+                                                                     480))) for i in range(32)],
+                    rate_hz=1e6, duration_s=10.0, max_requests=32)
+        poisson_run(srv, pool, rate_hz=1e6, duration_s=10.0, max_requests=256)
+        runs = []
+        for r in rates:
+            st = poisson_run(srv, pool, rate_hz=float(r), duration_s=duration_s, max_requests=max_requests)
+            runs.append({"rate_hz": r, "fps": round(st["fps"], 1), "p50_ms": round(st["p50_ms"], 2),
+                         "p99_ms": round(st["p99_ms"], 2), "mean_batch_views": round(st["mean_batch_views"], 2),
+                         "frames": st["frames"], "graph_replays": st["graph_replays"]})
+        out[mode] = runs
+        srv.close()
+    # sustained frames/s at matched p99: the best rate of each mode whose p99 stays under the bound
+    bound = max(min(r["p99_ms"] for r in out["per_request"]) * 2.0, 50.0)
+    best = {m: max([r["fps"] for r in out[m] if r["p99_ms"] <= bound] or [0.0]) for m in ("batched", "per_request")}
+    out["p99_bound_ms"] = round(bound, 1)
+    out["fps_at_p99_bound"] = best
+    out["speedup_at_p99_bound"] = round(best["batched"] / best["per_request"], 2) if best["per_request"] else None
+    return out
 
 
 def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:

@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,7 @@ class Engine {
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs_;
+  std::vector<std::array<uint64_t, 6>> seen_sigs_;
   uint64_t epoch_ = 0;  // bumped by every (re)allocation: graphs bake buffer addresses
   void* gstage_ = nullptr;
   size_t gstage_bytes_ = 0;

@@ -354,6 +354,18 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
   for (const GraphEntry& g : graphs_)
     if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
   if (!exec) {
+    // capture costs milliseconds: a shape is captured only once it has been seen before
+    bool seen = false;
+    for (const auto& s : seen_sigs_) seen |= memcmp(s.data(), sig, sizeof sig) == 0;
+    if (!seen) {
+      std::array<uint64_t, 6> a;
+      memcpy(a.data(), sig, sizeof sig);
+      if (seen_sigs_.size() >= 1024) seen_sigs_.erase(seen_sigs_.begin());
+      seen_sigs_.push_back(a);
+      return run(frames_in, V, stream, gbuffer, false);
+    }
+  }
+  if (!exec) {
     Bufs b = b_;
     b.keys20 = nullptr;
     b.entries_canon = nullptr;
@@ -378,7 +390,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
       err = "hipGraphInstantiate failed";
       return HDSR_E_HIP;
     }
-    if (graphs_.size() >= 8) {
+    if (graphs_.size() >= 128) {
       (void)hipGraphExecDestroy(graphs_.front().exec);
       graphs_.erase(graphs_.begin());
     }

@@ -60,6 +60,11 @@ struct FrameDesc {
   uint32_t* gb_normal;   // width*height, or null
   uint32_t* gb_material; // width*height*4, or null
   uint16_t* rgba16;      // width*height*4 binary16, or null
+  // differentiable mode: opacity holds logits, the rasterizer reads the fp32 record (drec) and
+  // keeps each pixel's final transmittance and one-past-last contributing entry for the backward
+  uint32_t diff;
+  float* t_final;        // width*height, or null
+  uint32_t* last_idx;    // width*height, or null
 };
 
 // Counters of one batch, written by the kernels (device) and copied to pinned host memory.
@@ -107,6 +112,9 @@ struct Bufs {
   uint32_t key_bits;         // bits needed for a global Morton key
   uint4* hot;                // [splat_blocks*256*2]
   uint4* cold;               // [splat_blocks*256*2]
+  float4* drec;              // [splat_blocks*256*3] fp32 record of the differentiable mode, or null:
+                             //   {px, py, k0, k1 | k2, opacity, r, g | b, depth, 0, 0}
+  float* gacc;               // [splat_blocks*256*9] dL/d(px, py, k0, k1, k2, opacity, r, g, b), or null
   uint32_t* bounds;          // [splat_blocks*256]
   uint32_t* depth;           // [splat_blocks*256]
   uint2* dlog;               // [splat_blocks*256] (depth bits, ln(max(depth, 1e-6)) bits) for the sorts
@@ -150,6 +158,17 @@ void launch_classify(const Bufs& b, hipStream_t stream);
 void launch_bucket(const Bufs& b, hipStream_t stream);
 void launch_segment_sort(const Bufs& b, hipStream_t stream);
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer);
+// Differentiable mode (single frame): backward of the rasterizer into gacc, then of the
+// projection into the parameter gradients (accumulated, indexed by splat id).
+struct DiffGrads {
+  float* means;           // 3N
+  float* log_scales;      // 3N
+  float* rotations;       // 4N
+  float* opacity_logits;  // N
+  float* sh_data;         // N * sh_stride (the emissive tail gets no gradient)
+};
+void launch_backward(const Bufs& b, const FrameDesc& F, const float* grad_rgba, const DiffGrads& out,
+                     hipStream_t stream);
 // Rebuilds the reference's 14-word projection cache of frame f into dst (debug readback).
 void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream);
 
