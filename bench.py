@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--views", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0, help="target CPU seconds for the sample")
     return ap.parse_args()
 
@@ -142,6 +143,7 @@ def main() -> int:
 
     res.destroy()
     serving = None if a.no_serving else serving_bench()
+    diff = None if a.no_diff else diff_bench()
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, sorted_host[0], uniforms[0], W, H, plan, a.cpu_budget_s)
@@ -174,6 +176,7 @@ def main() -> int:
         "setup_s": round(setup_s, 1),
         "profiled_renders": renders,
         "serving": serving,
+        "diff": diff,
     }
     print(json.dumps(out))
     return 0
@@ -234,6 +237,61 @@ def serving_bench(rates=(100, 200, 400, 600, 800, 1200, 3000, 5000), duration_s=
     out["fps_at_p99_bound"] = best
     out["speedup_at_p99_bound"] = round(best["batched"] / best["per_request"], 2) if best["per_request"] else None
     return out
+
+
+def diff_bench(n=500_000, views=16, steps=3, seed=4) -> dict:
+    """Differentiation workload: 500k SH3 splats (the 1M scene's recipe), 16 orbit views at
+    1280x720.  Targets are the views rendered from the seeded perturbation of the scene (means
+    + N(0, 0.01^2), log-scales + N(0, 0.1^2), opacity logits + N(0, 0.3^2), SH + N(0, 0.02^2)); the
+    loss is L2 per view.  One step = every view once: forward (differentiable mode), dL/d(rgba) =
+    2 (rgba - target), backward accumulating into the five gradient buffers.  Reports ms per
+    view of the forward alone and of forward + backward."""
+    import torch
+
+    from hdsr import scenes
+    from hdsr.diff import DiffRenderer
+    sc = scenes.scene_config4(n, seed)
+    cams = scenes.cameras_config4()[:views]
+    W, H = cams[0].width, cams[0].height
+    rng = np.random.default_rng(seed + 1)
+    op = sc.opacities.astype(np.float64)
+    logits = np.log(op / (1.0 - op)).astype(np.float32)
+    sh = sc.sh_data().reshape(n, -1)
+    base = [sc.positions, sc.scales, sc.rotations, logits, sh]
+    pert = [sc.positions + rng.normal(0, 0.01, sc.positions.shape), sc.scales + rng.normal(0, 0.1, sc.scales.shape),
+            sc.rotations, logits + rng.normal(0, 0.3, logits.shape), sh.copy()]
+    pert[4][:, :-3] += rng.normal(0, 0.02, sh[:, :-3].shape)
+    dev = torch.device("cuda:0")
+    to = lambda arrs: [torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev) for x in arrs]  # noqa: E731
+    P, Q = to(base), to(pert)
+    r = DiffRenderer(n, W, H, 3)
+    us = [r.uniforms(c) for c in cams]
+    targets = [r.render(u, *Q)[0] for u in us]
+    grads = [torch.zeros_like(t) for t in P]
+
+    def step(backward: bool) -> None:
+        for v, u in enumerate(us):
+            img, _ = r.render(u, *P)
+            if backward:
+                r.backward(2.0 * (img - targets[v]), *P, out=grads)
+
+    step(True)  # warm: capacities, allocator
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(False)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / (steps * views) * 1e3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(True)
+    torch.cuda.synchronize()
+    fb = (time.perf_counter() - t0) / (steps * views) * 1e3
+    st = r.stats()
+    r.close()
+    return {"workload": f"{n} SH3 splats, {views} views at {W}x{H}, L2 to perturbed-scene targets",
+            "fwd_ms_per_view": round(fwd, 3), "fwd_bwd_ms_per_view": round(fb, 3),
+            "fwd_bwd_over_fwd": round(fb / fwd, 2), "refs_per_view": st["refs"]}
 
 
 def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:

@@ -28,9 +28,13 @@ struct hdsr_ctx {
   // outputs
   float* rgba = nullptr;
   float* depth = nullptr;
+  float* user_rgba = nullptr;     // caller-bound outputs (hdsr_bind_outputs), or null
+  float* user_depth = nullptr;
   uint32_t* gbn = nullptr;
   uint32_t* gbm = nullptr;
   uint16_t* rgba16 = nullptr;
+  float* t_final = nullptr;       // differentiable mode
+  uint32_t* last_idx = nullptr;
   hipStream_t last_stream = nullptr;
   bool rendered = false;
   uint32_t reruns = 0;
@@ -89,11 +93,14 @@ void fill_frame_scene(hdsr_ctx* c) {
   F.sh_stride = sh_stride(c->sh_degree);
   F.width = c->plan.viewport_width;
   F.height = c->plan.viewport_height;
-  F.rgba = c->rgba;
-  F.out_depth = c->depth;
+  F.rgba = c->user_rgba ? c->user_rgba : c->rgba;
+  F.out_depth = c->user_depth ? c->user_depth : c->depth;
   F.gb_normal = c->gbn;
   F.gb_material = c->gbm;
   F.rgba16 = c->rgba16;
+  F.diff = (c->flags & HDSR_DIFFERENTIABLE) ? 1u : 0u;
+  F.t_final = c->t_final;
+  F.last_idx = c->last_idx;
 }
 
 }  // namespace
@@ -127,6 +134,7 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   if (!plan || !out) return HDSR_E_INVALID_ARG;
   *out = nullptr;
   if (flags & (HDSR_F16_COMPOSITE | HDSR_REF_CAPS)) return HDSR_E_UNSUPPORTED;
+  if ((flags & HDSR_DIFFERENTIABLE) && (flags & HDSR_GBUFFER)) return HDSR_E_UNSUPPORTED;
   if (plan->tile_size_px != HDSR_TILE_PX) return HDSR_E_INVALID_ARG;
   if (plan->tile_columns == 0 || plan->tile_rows == 0 || plan->tile_columns > 255 || plan->tile_rows > 255)
     return HDSR_E_INVALID_ARG;  // u8 tile-box packing (gpu_project_splats.wgsl:229-231)
@@ -144,6 +152,11 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   if (flags & HDSR_OUT_RGBA16) {
     if (hipMalloc((void**)&c->rgba16, px * 8) != hipSuccess) return HDSR_E_OOM;
   }
+  if (flags & HDSR_DIFFERENTIABLE) {
+    if (hipMalloc((void**)&c->t_final, px * 4) != hipSuccess) return HDSR_E_OOM;
+    if (hipMalloc((void**)&c->last_idx, px * 4) != hipSuccess) return HDSR_E_OOM;
+    c->eng.set_diff(true);
+  }
   const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
   c->eng.set_min_capacity(base_cap);
   c->eng.set_profile((flags & HDSR_PROFILE) != 0);
@@ -155,7 +168,7 @@ void hdsr_destroy(hdsr_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   free_scene(c);
-  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16};
+  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete c;
@@ -276,10 +289,17 @@ int hdsr_sync(hdsr_ctx* c) {
   return fail(c, HDSR_E_CAPACITY, "pair capacity still overflowing after regrowth");
 }
 
+int hdsr_bind_outputs(hdsr_ctx* c, float* rgba, float* depth) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  c->user_rgba = rgba;
+  c->user_depth = depth;
+  return HDSR_OK;
+}
+
 int hdsr_get_outputs(hdsr_ctx* c, hdsr_outputs* o) {
   if (!c || !o) return HDSR_E_INVALID_ARG;
-  o->rgba = c->rgba;
-  o->depth = c->depth;
+  o->rgba = c->user_rgba ? c->user_rgba : c->rgba;
+  o->depth = c->user_depth ? c->user_depth : c->depth;
   o->gbuffer_normal = c->gbn;
   o->gbuffer_material = c->gbm;
   o->rgba16 = c->rgba16;
@@ -296,16 +316,32 @@ int hdsr_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
   const void* src = nullptr;
   size_t need = 0;
   switch (which) {
-    case HDSR_OUT_RGBA: src = c->rgba; need = px * 16; break;
-    case HDSR_OUT_DEPTH: src = c->depth; need = px * 4; break;
+    case HDSR_OUT_RGBA: src = c->user_rgba ? c->user_rgba : c->rgba; need = px * 16; break;
+    case HDSR_OUT_DEPTH: src = c->user_depth ? c->user_depth : c->depth; need = px * 4; break;
     case HDSR_OUT_GB_NORMAL: src = c->gbn; need = px * 4; break;
     case HDSR_OUT_GB_MATERIAL: src = c->gbm; need = px * 16; break;
     case HDSR_OUT_RGBA16_BUF: src = c->rgba16; need = px * 8; break;
+    case HDSR_OUT_T_FINAL: src = c->t_final; need = px * 4; break;
+    case HDSR_OUT_LAST_INDEX: src = c->last_idx; need = px * 4; break;
     default: return fail(c, HDSR_E_INVALID_ARG, "unknown output");
   }
   if (!src) return fail(c, HDSR_E_STATE, "output not enabled by the create flags");
   if (bytes < need) return fail(c, HDSR_E_INVALID_ARG, "host buffer too small");
   HIPX(c, hipMemcpy(host, src, need, hipMemcpyDeviceToHost));
+  return HDSR_OK;
+}
+
+int hdsr_backward(hdsr_ctx* c, const float* grad_rgba, const hdsr_grads* g, void* stream) {
+  if (!c || !grad_rgba || !g) return HDSR_E_INVALID_ARG;
+  if (!(c->flags & HDSR_DIFFERENTIABLE)) return fail(c, HDSR_E_STATE, "create the context with HDSR_DIFFERENTIABLE");
+  if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
+  if (c->n && (!g->means || !g->log_scales || !g->rotations || !g->opacity_logits || !g->sh_data))
+    return fail(c, HDSR_E_INVALID_ARG, "missing gradient buffer");
+  int rc = hdsr_sync(c);
+  if (rc) return rc;
+  DiffGrads d{g->means, g->log_scales, g->rotations, g->opacity_logits, g->sh_data};
+  rc = c->eng.backward(c->n, grad_rgba, d, static_cast<hipStream_t>(stream));
+  if (rc) return fail(c, rc, c->eng.err);
   return HDSR_OK;
 }
 

@@ -54,6 +54,10 @@ class Engine {
   // Grows the pair capacity to hold the last run's references (with headroom).
   int grow_for_overflow();
   void set_min_capacity(uint64_t cap) { min_cap_ = cap; }
+  // Differentiable mode: keeps the fp32 record table and the per-splat gradient slots.
+  void set_diff(bool on) { diff_ = on; }
+  // Backward of the last run (one frame, n splats); see launch_backward.
+  int backward(uint32_t n, const float* grad_rgba, const DiffGrads& g, hipStream_t stream);
 
   // Per-stage HIP events (one set per run, kept in a ring until harvested).
   void set_profile(bool on) { profile_ = on; }
@@ -86,6 +90,8 @@ class Engine {
   Layout lay_;
   uint64_t min_cap_ = 1u << 20;
   uint64_t want_cap_ = 0;
+  bool diff_ = false;
+  size_t n_diff_ = 0;
   // allocation sizes (elements)
   size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;
   size_t n_morton_ = 0, n_tiles_ = 0, n_seg_ = 0, n_frames_ = 0, n_debug_ = 0;

@@ -71,7 +71,7 @@ void Engine::release() {
   if (gstage_) (void)hipHostFree(gstage_);
   gstage_ = nullptr;
   gstage_bytes_ = 0;
-  void* ptrs[] = {b_.hot, b_.cold, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
+  void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
                   b_.keys_a, b_.vals_a, b_.keys_b, b_.vals_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch, b_.ekey, b_.ekey_s,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
@@ -94,7 +94,7 @@ void Engine::release() {
   prof_ev_.clear();
   prof_head_ = prof_count_ = 0;
   n_splat_ = n_blocks_ = n_scan4_ = n_cap_ = n_hist_ = n_hist_tmp_ = 0;
-  n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = 0;
+  n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = n_diff_ = 0;
 }
 
 template <typename T>
@@ -133,6 +133,13 @@ int Engine::ensure(const Layout& L, bool debug) {
     have = n_splat_;
     if ((rc = grow(&b_.dlog, &have, ns))) return rc;
     n_splat_ = ns;
+  }
+  if (diff_ && (ns > n_diff_ || !b_.drec)) {
+    have = n_diff_;
+    if ((rc = grow(&b_.drec, &have, ns * 3))) return rc;
+    have = n_diff_;
+    if ((rc = grow(&b_.gacc, &have, ns * 9))) return rc;
+    n_diff_ = ns;
   }
   if (L.splat_blocks > n_blocks_ || !b_.block_cnt) {
     have = n_blocks_;
@@ -406,6 +413,20 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     return HDSR_E_HIP;
   }
   pending_ = true;
+  return 0;
+}
+
+int Engine::backward(uint32_t n, const float* grad_rgba, const DiffGrads& g, hipStream_t stream) {
+  if (!diff_ || !b_.drec || !b_.gacc || lay_.num_frames != 1) {
+    err = "backward needs a differentiable-mode render of one frame";
+    return HDSR_E_STATE;
+  }
+  launch_backward(b_, n, grad_rgba, g, stream);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    err = std::string("backward launch failed: ") + hipGetErrorString(e);
+    return HDSR_E_HIP;
+  }
   return 0;
 }
 

@@ -158,8 +158,9 @@ void launch_classify(const Bufs& b, hipStream_t stream);
 void launch_bucket(const Bufs& b, hipStream_t stream);
 void launch_segment_sort(const Bufs& b, hipStream_t stream);
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer);
-// Differentiable mode (single frame): backward of the rasterizer into gacc, then of the
-// projection into the parameter gradients (accumulated, indexed by splat id).
+// Differentiable mode (one frame, the batch's frame 0, with n splats): backward of the
+// rasterizer into gacc, then of the projection into the parameter gradients, which are
+// accumulated (added to) and indexed by splat id.
 struct DiffGrads {
   float* means;           // 3N
   float* log_scales;      // 3N
@@ -167,8 +168,7 @@ struct DiffGrads {
   float* opacity_logits;  // N
   float* sh_data;         // N * sh_stride (the emissive tail gets no gradient)
 };
-void launch_backward(const Bufs& b, const FrameDesc& F, const float* grad_rgba, const DiffGrads& out,
-                     hipStream_t stream);
+void launch_backward(const Bufs& b, uint32_t n, const float* grad_rgba, const DiffGrads& out, hipStream_t stream);
 // Rebuilds the reference's 14-word projection cache of frame f into dst (debug readback).
 void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream);
 

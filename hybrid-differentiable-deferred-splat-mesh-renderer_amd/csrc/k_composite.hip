@@ -20,6 +20,7 @@
 // Output: rgba = (C + T * (0.02, 0.02, 0.04), 1 - T), depth = D / max(W, 1e-4).
 // With the G-buffer enabled, also the voted material / normal / confidence / emissive words.
 #include "hdsr_internal.h"
+#include "raster_common.h"
 
 namespace hdsr {
 namespace {
@@ -56,7 +57,7 @@ struct Gb {
 };
 
 template <bool kG>
-__device__ __forceinline__ void blend(Px& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
+__device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
                                       const float4& ga, const float4& gb, float geb) {
   const float gauss = hdsr_expf(power);
   const float alpha = fminf(0.99f, o * gauss);
@@ -81,6 +82,7 @@ __device__ __forceinline__ void blend(Px& p, Gb& q, float power, float o, float 
     q.eg = q.eg + gb.w * w;
     q.eb = q.eb + geb * w;
   }
+  return valid;
 }
 
 template <bool kG>
@@ -109,29 +111,10 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
   }
 }
 
-// The y-interval outside which the splat's power is <= -4 at every pixel of a tile whose pixel
-// centres lie in [x0, x1] x [y0, y1], or an empty interval (lo > hi) when the splat cannot
-// reach the tile at all.  Exact arithmetic: power = -q/2 with q = d^T K d, K = [[k0,k1],[k1,k2]],
-// so power > -4 needs q < 8, an ellipse with half extents sqrt(8 K^-1_xx), sqrt(8 K^-1_yy).
-// The kernel's fp32 power differs from the exact one by at most ~4 ulp of the sum of the
-// magnitudes of its three terms (bounded here over the tile), so the level 8 is raised by
-// twice that bound: culling then never drops a splat that the fp32 evaluation would accept.
-__device__ __forceinline__ float2 reach_y(float cx, float cy, float k0, float k1, float k2, float x0, float x1,
-                                          float y0, float y1) {
-  // lower bound of det(K): the two products and the difference each round once
-  const float p02 = k0 * k2, p11 = k1 * k1;
-  const float det = (p02 - p11) - 4.0f * 5.9604645e-08f * (p02 + p11);
-  if (!(det > 0.0f) || !(k0 > 0.0f) || !(k2 > 0.0f)) return make_float2(-INFINITY, INFINITY);
-  const float DX = fmaxf(fabsf(x0 - cx), fabsf(x1 - cx)), DY = fmaxf(fabsf(y0 - cy), fabsf(y1 - cy));
-  const float sabs = 0.5f * (k0 * DX * DX + 2.0f * fabsf(k1) * DX * DY + k2 * DY * DY);
-  const float level = 8.0f + 2.0f * (8.0f * 5.9604645e-08f * sabs * 2.0f + 1e-4f);
-  const float hx = sqrtf(level * k2 / det) * 1.0001f + 1e-3f;
-  const float hy = sqrtf(level * k0 / det) * 1.0001f + 1e-3f;
-  if (cx + hx < x0 || cx - hx > x1 || cy + hy < y0 || cy - hy > y1) return make_float2(1.0f, -1.0f);
-  return make_float2(cy - hy, cy + hy);
-}
-
-template <bool kG>
+// kD: differentiable mode.  Records come from the fp32 table (drec) instead of the f16-packed
+// one, and each pixel also keeps its final transmittance and one past the index (in its tile's
+// list) of the last entry that blended, which bound the backward walk.
+template <bool kG, bool kD>
 __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_a[kBatch];  // cx, cy, a, b
   __shared__ float4 s_b[kBatch];  // c, opacity, r, g
@@ -139,6 +122,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
   __shared__ float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
   __shared__ float s_gc[kG ? kBatch : 1];   // emissive b
+  __shared__ uint32_t s_idx[kD ? kBatch : 1];  // entry index of each compacted slot
   __shared__ int s_done[4];
   __shared__ uint32_t s_keep[4];
 
@@ -158,6 +142,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const bool odd_x = (lx & 1u) != 0, odd_y = (ly & 1u) != 0;
 
   Px p = Px{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  uint32_t last = 0;
   Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t loaded = 0;
 
@@ -168,14 +153,17 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   // Two-stage prefetch: while batch k is blended, the records of batch k+1 and the entry
   // indices of batch k+2 are in flight.
   const uint4 z4 = make_uint4(0, 0, 0, 0);
+  const uint4* rec = kD ? reinterpret_cast<const uint4*>(b.drec) : b.hot;
+  constexpr uint32_t kRec = kD ? 3 : 2;  // 16-byte words per record
   uint32_t g_cur = tid < count ? b.entries[start + tid] : 0u;
-  uint4 h0n = tid < count ? b.hot[(size_t)g_cur * 2] : z4;
-  uint4 h1n = tid < count ? b.hot[(size_t)g_cur * 2 + 1] : z4;
+  uint4 h0n = tid < count ? rec[(size_t)g_cur * kRec] : z4;
+  uint4 h1n = tid < count ? rec[(size_t)g_cur * kRec + 1] : z4;
+  uint4 h2n = (kD && tid < count) ? rec[(size_t)g_cur * kRec + 2] : z4;
   uint32_t g_next = kBatch + tid < count ? b.entries[start + kBatch + tid] : 0u;
   for (uint32_t base = 0; base < count; base += kBatch) {
     const uint32_t i = base + tid;
     bool keep = false;
-    const uint4 h0 = h0n, h1 = h1n;
+    const uint4 h0 = h0n, h1 = h1n, h2 = h2n;
     const uint32_t g = g_cur;
     float2 reach = make_float2(0.0f, 0.0f);
     if (i < count) {
@@ -192,8 +180,14 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     const uint32_t nb = s_keep[0] + s_keep[1] + s_keep[2] + s_keep[3];
     if (keep) {
       s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
-      s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
-      s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), reach.x, reach.y);
+      if (kD) {
+        s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w));
+        s_c[slot] = make_float4(hdsr_u2f(h2.x), hdsr_u2f(h2.y), reach.x, reach.y);
+        s_idx[slot] = i;
+      } else {
+        s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
+        s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), reach.x, reach.y);
+      }
       if (kG) {
         const uint4 c0 = b.cold[(size_t)g * 2];
         const uint4 c1 = b.cold[(size_t)g * 2 + 1];
@@ -216,8 +210,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     {
       const uint32_t i1 = base + kBatch + tid, i2 = base + 2 * kBatch + tid;
       g_cur = g_next;
-      h0n = i1 < count ? b.hot[(size_t)g_cur * 2] : z4;
-      h1n = i1 < count ? b.hot[(size_t)g_cur * 2 + 1] : z4;
+      h0n = i1 < count ? rec[(size_t)g_cur * kRec] : z4;
+      h1n = i1 < count ? rec[(size_t)g_cur * kRec + 1] : z4;
+      if (kD) h2n = i1 < count ? rec[(size_t)g_cur * kRec + 2] : z4;
       g_next = i2 < count ? b.entries[start + i2] : 0u;
     }
     loaded += min(kBatch, count - base);
@@ -240,7 +235,8 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
           gb = s_gb[j];
           gc = s_gc[j];
         }
-        blend<kG>(p, q, pw, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
+        const bool v = blend<kG>(p, q, pw, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
+        if (kD && v) last = s_idx[j] + 1u;
       }
     }
     const int wave_done = __all(p.T < 0.001f);
@@ -249,6 +245,11 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) break;
   }
   store_px<kG>(F, X, Y, p, q);
+  if (kD && X < F.width && Y < F.height) {
+    const size_t o = (size_t)Y * F.width + X;
+    if (F.t_final) F.t_final[o] = p.T;
+    if (F.last_idx) F.last_idx[o] = last;
+  }
   if (tid == 0 && b.stats) {
     atomicAdd(&b.stats->composite_refs, loaded);
     atomicMax(&b.stats->composite_max_refs, loaded);
@@ -259,10 +260,12 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
 
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer) {
   if (!b.total_tiles) return;
-  if (gbuffer)
-    hipLaunchKernelGGL(k_composite<true>, dim3(b.total_tiles), dim3(256), 0, stream, b);
+  if (b.drec)
+    hipLaunchKernelGGL((k_composite<false, true>), dim3(b.total_tiles), dim3(256), 0, stream, b);
+  else if (gbuffer)
+    hipLaunchKernelGGL((k_composite<true, false>), dim3(b.total_tiles), dim3(256), 0, stream, b);
   else
-    hipLaunchKernelGGL(k_composite<false>, dim3(b.total_tiles), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL((k_composite<false, false>), dim3(b.total_tiles), dim3(256), 0, stream, b);
 }
 
 }  // namespace hdsr

@@ -14,37 +14,10 @@
 // Arithmetic is IEEE fp32 with no contraction, in the operation order of the WGSL, so the
 // record is bit-identical to the CPU restatement.
 #include "hdsr_internal.h"
+#include "proj_common.h"
 
 namespace hdsr {
 namespace {
-
-struct F3 {
-  float x, y, z;
-};
-__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
-__device__ __forceinline__ F3 add(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ F3 sub(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ F3 mul(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ F3 smul(float s, F3 a) { return f3(s * a.x, s * a.y, s * a.z); }
-__device__ __forceinline__ F3 vdiv(F3 a, float s) { return f3(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ F3 cross(F3 a, F3 b) {
-  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float vlen(F3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ F3 normalize(F3 a) { return vdiv(a, vlen(a)); }
-
-__device__ __forceinline__ void mat4_point(const float* m, F3 p, float out[4]) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i) out[i] = ((m[i] * p.x + m[4 + i] * p.y) + m[8 + i] * p.z) + m[12 + i] * 1.0f;
-}
-
-// rotateAxis (gpu_project_splats.wgsl:75-79), q already divided by max(|q|, 1e-6)
-__device__ __forceinline__ F3 rotate_axis(const float q[4], F3 axis) {
-  const F3 u = f3(q[1], q[2], q[3]);
-  const F3 inner = add(cross(u, axis), smul(q[0], axis));
-  return add(axis, smul(2.0f, cross(u, inner)));
-}
 
 __device__ __forceinline__ void oct_encode(F3 n, float* ox, float* oy) {
   const float sum = (fabsf(n.x) + fabsf(n.y)) + fabsf(n.z);
@@ -60,20 +33,6 @@ __device__ __forceinline__ void oct_encode(F3 n, float* ox, float* oy) {
   *ox = px;
   *oy = py;
 }
-
-#define SH_C1 0.4886025119029199f
-#define SH_C2_0 1.0925484305920792f
-#define SH_C2_1 1.0925484305920792f
-#define SH_C2_2 0.31539156525252005f
-#define SH_C2_3 1.0925484305920792f
-#define SH_C2_4 0.5462742152960396f
-#define SH_C3_0 0.5900435899266435f
-#define SH_C3_1 2.890611442640554f
-#define SH_C3_2 0.4570457994644658f
-#define SH_C3_3 0.3731763325901154f
-#define SH_C3_4 0.4570457994644658f
-#define SH_C3_5 1.445305721320277f
-#define SH_C3_6 0.5900435899266435f
 
 __global__ __launch_bounds__(256) void k_project(Bufs b) {
   const uint32_t blk = blockIdx.x;
@@ -140,7 +99,9 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
       if (!(radius <= 0.0f)) {
         const float det_inv = 1.0f / fmaxf(det_blur, 0.000001f);
         const float k0 = cyy * det_inv, k1 = -cxy * det_inv, k2 = cxx * det_inv;
-        const float opac = fminf(fmaxf(F.opacity[id] * aa, 0.0f), 0.99f);
+        // differentiable mode: the opacity input is a logit (sigmoid evaluated here)
+        const float o_in = F.diff ? 1.0f / (1.0f + hdsr_expf(-F.opacity[id])) : F.opacity[id];
+        const float opac = fminf(fmaxf(o_in * aa, 0.0f), 0.99f);
 
         // tile box (gpu_project_splats.wgsl:210-218,229-232)
         const float ts = fmaxf(F.tile, 1.0f);
@@ -258,6 +219,13 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
         c0 = make_uint4(F.material[id], hdsr_f16_bits(radius) | (op16 << 16), hdsr_pack2x16(ox, oy),
                         hdsr_pack2x16(conf, 0.0f));
         c1 = make_uint4(hdsr_pack2x16(em[0], em[1]), hdsr_pack2x16(em[2], 0.0f), 0u, 0u);
+        if (F.diff && b.drec) {
+          // fp32 record for the differentiable rasterizer: no f16 rounding of colour / opacity
+          float4* d = b.drec + (size_t)g * 3;
+          d[0] = make_float4(px, py, k0, k1);
+          d[1] = make_float4(k2, opac, cr, cg);
+          d[2] = make_float4(cb, depth, 0.0f, 0.0f);
+        }
       }
     }
   }

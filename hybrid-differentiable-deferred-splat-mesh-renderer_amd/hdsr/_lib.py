@@ -25,12 +25,13 @@ HDSR_GBUFFER = 4
 HDSR_DEBUG = 8
 HDSR_OUT_RGBA16 = 16
 HDSR_PROFILE = 32
+HDSR_DIFFERENTIABLE = 64
 STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "sort", "composite"]
 
 HDSR_MEM_HOST = 0
 HDSR_MEM_DEVICE = 1
 
-OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16 = range(5)
+OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX = range(7)
 
 DBG = {
     "projcache": 0, "depth": 1, "tile_counts": 2, "tile_offsets": 3, "entries": 4, "entries_canon": 5,
@@ -70,6 +71,11 @@ class Stats(C.Structure):
         "cap_chunks_tripped", "composite_refs", "composite_max_refs")]
 
 
+class Grads(C.Structure):
+    _fields_ = [("means", C.c_void_p), ("log_scales", C.c_void_p), ("rotations", C.c_void_p),
+                ("opacity_logits", C.c_void_p), ("sh_data", C.c_void_p)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_splats_per_batch", C.c_uint64),
                 ("max_views_per_batch", C.c_uint32), ("slots", C.c_uint32), ("use_graphs", C.c_uint32),
@@ -105,11 +111,13 @@ SIGNATURES = [
     ("hdsr_composite_only", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_sync", C.c_int, [C.c_void_p]),
     ("hdsr_get_outputs", C.c_int, [C.c_void_p, C.POINTER(Outputs)]),
+    ("hdsr_bind_outputs", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("hdsr_readback", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
     ("hdsr_debug_size", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     ("hdsr_debug_readback", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
     ("hdsr_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("hdsr_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
+    ("hdsr_backward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Grads), C.c_void_p]),
     ("hdsr_last_error", C.c_char_p, [C.c_void_p]),
     ("hdsr_version", C.c_char_p, []),
     ("hdsr_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),

@@ -62,6 +62,10 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
 #define HDSR_DEBUG 8u           /* keep the pre-sort entry list and the 20-bit keys for readback */
 #define HDSR_OUT_RGBA16 16u     /* also write binary16 RGBA (the reference's rgba16float texture) */
 #define HDSR_PROFILE 32u        /* record a HIP event between pipeline stages of every render */
+#define HDSR_DIFFERENTIABLE 64u /* differentiable mode (see hdsr_backward): opacities are logits,
+                                   the rasterizer reads fp32 colour / opacity (no f16 cache
+                                   rounding) and keeps per-pixel state for the backward; not
+                                   combinable with HDSR_GBUFFER */
 
 typedef struct hdsr_ctx hdsr_ctx;
 
@@ -129,11 +133,19 @@ typedef struct hdsr_outputs {
 } hdsr_outputs;
 int hdsr_get_outputs(hdsr_ctx* ctx, hdsr_outputs* out);
 
+/* The outputTexture / gbufferTextures.depth arguments of createTileSplatBindGroups
+ * (gpuTileSplatCompositor.ts:805-806): later renders write RGBA (H*W*4 f32) and depth (H*W f32)
+ * into these caller-owned device buffers instead of the context's own.  NULL for either restores
+ * the context's buffer.  The buffers must outlive their use by renders. */
+int hdsr_bind_outputs(hdsr_ctx* ctx, float* rgba, float* depth);
+
 #define HDSR_OUT_RGBA 0
 #define HDSR_OUT_DEPTH 1
 #define HDSR_OUT_GB_NORMAL 2
 #define HDSR_OUT_GB_MATERIAL 3
 #define HDSR_OUT_RGBA16_BUF 4
+#define HDSR_OUT_T_FINAL 5        /* H*W f32 final transmittance (HDSR_DIFFERENTIABLE) */
+#define HDSR_OUT_LAST_INDEX 6     /* H*W u32 one past the last blended entry of the pixel's tile list */
 /* Copies an output to host memory (after hdsr_sync). */
 int hdsr_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
 
@@ -189,6 +201,27 @@ int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 #define HDSR_STAGE_COMPOSITE 8    /* composite */
 #define HDSR_STAGE_COUNT 9
 int hdsr_stage_times(hdsr_ctx* ctx, float* ms, uint32_t* renders);
+
+/* ---- Differentiable mode ------------------------------------------------------------------
+ * No counterpart in the reference (its renderer has no backward; the Python preprocessing
+ * package, preprocessing/splat_oracle, is the host for the autograd wrapper).  With
+ * HDSR_DIFFERENTIABLE the forward is hdsr_render as usual, with opacities read as logits
+ * (opacity = sigmoid(logit)).  hdsr_backward then takes dL/d(rgba) of that frame (device,
+ * H*W*4 f32, same layout as the rgba output) and ADDS dL/d(parameter) into the device arrays
+ * below, indexed by splat id like the inputs.  The per-tile lists and their order are held
+ * fixed, as are the discrete decisions of the forward (visibility, tile boxes, the power / alpha
+ * / transmittance cut-offs); the alpha clamp at 0.99 and the colour clamp to [0, 1] pass no
+ * gradient.  The depth output gets no gradient.  Accumulation uses float atomics, so the low
+ * bits of a gradient may differ between runs.  All pointers must be non-null device memory. */
+typedef struct hdsr_grads {
+  float* means;                   /* 3N */
+  float* log_scales;              /* 3N */
+  float* rotations;               /* 4N (w, x, y, z) */
+  float* opacity_logits;          /* N */
+  float* sh_data;                 /* N*(3+3K+3): dc and SH coefficients; the emissive tail stays 0 */
+} hdsr_grads;
+/* Waits for the last render (as hdsr_sync), then enqueues the backward on 'stream'. */
+int hdsr_backward(hdsr_ctx* ctx, const float* grad_rgba, const hdsr_grads* grads, void* stream);
 
 const char* hdsr_last_error(hdsr_ctx* ctx);
 const char* hdsr_version(void);

@@ -186,7 +186,11 @@ struct Splats {
   const float* shdata;
   uint32_t n;
   uint32_t sh_degree;
+  bool logits;  // differentiable mode: opacity inputs are logits
 };
+
+inline float sigmoidT(float x) { return 1.0f / (1.0f + hdsr_expf(-x)); }
+inline double sigmoidT(double x) { return 1.0 / (1.0 + std::exp(-x)); }
 
 // The projected record of one splat, in F precision, before packing.
 template <typename F>
@@ -263,7 +267,9 @@ Proj<F> project_one(const Splats& S, const Uniforms& U, uint32_t rank) {
   P.k0 = cyy * det_inv;
   P.k1 = -cxy * det_inv;
   P.k2 = cxx * det_inv;
-  P.opacity = fmn(fmx((F)S.opacity[id] * aa, (F)0), (F)0.99f);
+  // differentiable mode: the opacity input is a logit
+  const F o_in = S.logits ? sigmoidT((F)S.opacity[id]) : (F)S.opacity[id];
+  P.opacity = fmn(fmx(o_in * aa, (F)0), (F)0.99f);
 
   // tile bounds (gpu_project_splats.wgsl:210-218,229-232); integer results, computed in fp32
   {
@@ -431,6 +437,7 @@ struct Frame {
   std::vector<uint32_t> gb_normal, gb_material;  // H*W, H*W*4
   std::vector<uint32_t> stats;  // see orc_stat_*
   std::vector<double> rgba64, depth64;
+  std::vector<float> diffrec;  // differentiable mode: (opacity, r, g, b) per rank, no f16 rounding
   Splats S;
 };
 
@@ -647,13 +654,16 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
             // all four quad pixels <= -4 skips the splat for the whole quad (no state change),
             // equivalent per pixel to 'valid' being false; T is unchanged either way.
             const float power = (ca * dx * dx + cbx * dx * dy) + cc * dy * dy;
-            const float o = h2f(w[6] >> 16);
+            const bool dm = !F.diffrec.empty();
+            const float o = dm ? F.diffrec[(size_t)r * 4] : h2f(w[6] >> 16);
             const float g = hdsr_expf(power);
             const float alpha = std::fmin(0.99f, o * g);
             const float newT = T * (1.0f - alpha);
             const bool valid = power > -4.0f && alpha > (1.0f / 255.0f) && T >= 0.001f;
             const float wgt = (alpha * T) * (valid ? 1.0f : 0.0f);
-            const float r_ = h2f(w[9]), g_ = h2f(w[9] >> 16), b_ = h2f(w[10]);
+            const float r_ = dm ? F.diffrec[(size_t)r * 4 + 1] : h2f(w[9]);
+            const float g_ = dm ? F.diffrec[(size_t)r * 4 + 2] : h2f(w[9] >> 16);
+            const float b_ = dm ? F.diffrec[(size_t)r * 4 + 3] : h2f(w[10]);
             cr = cr + r_ * wgt;
             cg = cg + g_ * wgt;
             cb = cb + b_ * wgt;
@@ -709,13 +719,15 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
 }
 
 // FP64 twin: projection and compositing in double over the fp32 pipeline's per-tile order.
-// Quantisation points of the reference stay (f16 colour/opacity in the cache, via f32).
-void composite64(Frame& F) {
+// Quantisation points of the reference stay (f16 colour/opacity in the cache, via f32), unless
+// 'quantize' is false: then it is the FP64 twin of the differentiable mode, whose rasterizer
+// reads fp32 colour / opacity.
+void composite64(Frame& F, bool quantize) {
   const uint32_t W = F.width, H = F.height, n = F.n;
   std::vector<Proj<double>> P(n);
   for (uint32_t r = 0; r < n; ++r) {
     P[r] = project_one<double>(F.S, F.U, r);
-    if (P[r].visible) {
+    if (P[r].visible && quantize) {
       P[r].opacity = h2f(hdsr_f16_bits((float)P[r].opacity));
       P[r].r = h2f(hdsr_f16_bits((float)P[r].r));
       P[r].g = h2f(hdsr_f16_bits((float)P[r].g));
@@ -806,6 +818,8 @@ enum {
 
 #define ORC_FLAG_GBUFFER 1u
 #define ORC_FLAG_FP64 2u
+#define ORC_FLAG_FP64_NOQUANT 8u
+#define ORC_FLAG_DIFF 16u  // differentiable mode: logit opacities, unquantized rasterizer inputs
 #define ORC_FLAG_NO_COMPOSITE 4u
 
 static double now_s() {
@@ -822,7 +836,7 @@ void* orc_run_timed(const orc_splats* s, const void* uniforms256, uint32_t width
   Frame* F = new Frame();
   F->U = parse_uniforms(uniforms256);
   F->S = Splats{s->pos, s->logscale, s->rot, s->opacity, s->sorted, s->material, s->normals, s->normals_len,
-                s->shdata, s->n, s->sh_degree};
+                s->shdata, s->n, s->sh_degree, false};
   F->n = s->n;
   F->width = width;
   F->height = height;
@@ -855,22 +869,31 @@ void* orc_run(const orc_splats* s, const void* uniforms256, uint32_t width, uint
   Frame* F = new Frame();
   F->U = parse_uniforms(uniforms256);
   F->S = Splats{s->pos, s->logscale, s->rot, s->opacity, s->sorted, s->material, s->normals, s->normals_len,
-                s->shdata, s->n, s->sh_degree};
+                s->shdata, s->n, s->sh_degree, false};
   F->n = s->n;
   F->width = width;
   F->height = height;
   F->tiles_x = F->U.gx;
   F->tiles_y = F->U.gy;
   F->morton_count = (F->tiles_x && F->tiles_y) ? hdsr_morton2(F->tiles_x - 1, F->tiles_y - 1) + 1 : 0;
+  F->S.logits = (flags & ORC_FLAG_DIFF) != 0;
   F->cache.assign((size_t)F->n * 14, 0);
   F->depth.assign(F->n, 0);
+  if (flags & ORC_FLAG_DIFF) F->diffrec.assign((size_t)F->n * 4, 0.0f);
   for (uint32_t r = 0; r < F->n; ++r) {
     const Proj<float> P = project_one<float>(F->S, F->U, r);
     pack_record(P, &F->cache[(size_t)r * 14], &F->depth[r]);
+    if (!F->diffrec.empty() && P.visible) {
+      float* d = &F->diffrec[(size_t)r * 4];
+      d[0] = P.opacity;
+      d[1] = P.r;
+      d[2] = P.g;
+      d[3] = P.b;
+    }
   }
   bin_and_sort(*F);
   if (!(flags & ORC_FLAG_NO_COMPOSITE)) composite32(*F, (flags & ORC_FLAG_GBUFFER) != 0);
-  if (flags & ORC_FLAG_FP64) composite64(*F);
+  if (flags & (ORC_FLAG_FP64 | ORC_FLAG_FP64_NOQUANT)) composite64(*F, (flags & ORC_FLAG_FP64_NOQUANT) == 0);
   return F;
 }
 

@@ -22,7 +22,7 @@ OUT = {
 }
 STATS = ["refs", "small_tiles", "large_tiles", "chunks", "large_splats", "visible", "cap_refs_tripped",
          "cap_large_splats_tripped", "cap_chunks_tripped", "ref_cap"]
-FLAG_GBUFFER, FLAG_FP64, FLAG_NO_COMPOSITE = 1, 2, 4
+FLAG_GBUFFER, FLAG_FP64, FLAG_NO_COMPOSITE, FLAG_FP64_NOQUANT, FLAG_DIFF = 1, 2, 4, 8, 16
 
 
 class OrcSplats(C.Structure):
