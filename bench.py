@@ -29,6 +29,14 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Stages that are one kernel each: the roofline is taken on the slowest of them.
+SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_bucket", "sort": "k_segsort",
+                        "composite": "k_composite"}
+# HBM bytes per launch of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a 1 GiB copy (FETCH x2.0, WRITE
+# x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  Valid for the kernels as profiled.
+PMC_TRAFFIC_BYTES = {"k_project": 1427.8e6, "k_emit": 340.0e6, "k_bucket": 1257.4e6, "k_segsort": 666.7e6,
+                     "k_composite": 203.5e6}
 
 
 def parse():
@@ -41,6 +49,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
+    ap.add_argument("--diff-only", action="store_true", help="run only the differentiation workload")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0, help="target CPU seconds for the sample")
     return ap.parse_args()
 
@@ -56,6 +68,8 @@ def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composi
         return 4 * n + 8 * R
     if stage == "group":
         return 2 * 16 * R  # two 8-bit passes: read + write an 8-byte pair each
+    if stage == "bucket":
+        return 44 * st.get("large_tile_refs", 0)
     if stage == "sort":
         return 12 * R
     if stage == "composite":
@@ -72,6 +86,19 @@ def main() -> int:
     import torch
 
     from hdsr import compositor, scenes
+
+    if a.calibrate_copy:
+        n = a.calibrate_copy << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n, "copies": 5}))
+        return 0
+    if a.diff_only:
+        print(json.dumps({"diff": diff_bench()}))
+        return 0
 
     t_setup = time.time()
     scene = scenes.scene_config1(a.n)
@@ -137,9 +164,11 @@ def main() -> int:
         byts = algorithmic_bytes(name, scene.count, k, mean_st, pixels, comp_refs)
         gbs = (byts / (ms * 1e-3) / 1e9) if ms > 0 and byts else None
         per_stage[name] = {"ms": round(ms, 4), "GBps": None if gbs is None else round(gbs, 1)}
-    dom = max(stage_ms, key=lambda s: stage_ms[s])
+    dom = max(SINGLE_KERNEL_STAGES, key=lambda s: stage_ms[s])
+    dom_kernel = SINGLE_KERNEL_STAGES[dom]
     dom_bytes = algorithmic_bytes(dom, scene.count, k, mean_st, pixels, comp_refs)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
+    full_size = a.n == 1_000_000 and len(cams) == 8
 
     res.destroy()
     serving = None if a.no_serving else serving_bench()
@@ -166,9 +195,10 @@ def main() -> int:
                    "frames_per_step": len(cams)},
         "ms_per_frame": round(wall * 1e3 / frames, 4),
         "gpu_ms_per_frame": round(gpu_ms / frames, 4),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": dom_bytes},
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PMC_TRAFFIC_BYTES.get(dom_kernel) if full_size else None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "ms_per_launch": round(stage_ms[dom], 4)},
         "stages_ms_per_frame": per_stage,
         "frame_stats": {k2: round(v2, 1) for k2, v2 in mean_st.items()},
         "cpu_baseline": cpu,

@@ -367,6 +367,7 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->cap_chunks_tripped = s.chunks > 1024u;
   o->composite_refs = s.composite_refs;
   o->composite_max_refs = s.composite_max_refs;
+  o->large_tile_refs = s.scratch_used;
   return HDSR_OK;
 }
 

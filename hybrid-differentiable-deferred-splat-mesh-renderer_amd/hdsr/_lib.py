@@ -68,7 +68,7 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "refs", "small_pairs", "large_pairs", "large_splats", "visible_splats", "small_tiles", "large_tiles",
         "chunks", "segments", "pair_capacity", "overflow_reruns", "cap_refs_tripped", "cap_large_splats_tripped",
-        "cap_chunks_tripped", "composite_refs", "composite_max_refs")]
+        "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs")]
 
 
 class Grads(C.Structure):

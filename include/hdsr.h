@@ -182,6 +182,7 @@ typedef struct hdsr_stats {
   uint32_t cap_chunks_tripped;    /* > 1024 chunks (gpu_tile_bucket_sort.wgsl:126) */
   uint32_t composite_refs;        /* entries the rasterizer loaded (up to each tile's saturation) */
   uint32_t composite_max_refs;    /* the largest per-tile count of those */
+  uint32_t large_tile_refs;       /* references in tiles with > 4096 entries (bucket cascade input) */
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 
