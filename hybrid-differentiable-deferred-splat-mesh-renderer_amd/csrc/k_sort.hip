@@ -12,7 +12,8 @@
 // the reference's bitonic network orders them exactly as a stable sort on q20 does; that sort is
 // done here as an LSD radix sort in LDS (digits of 8, 8 and 4 bits, a pass skipped when all keys
 // share its digit).  Equal digits are ranked inside a wave with eight ballots and a wave-private
-// running count, so a pass needs three workgroup barriers instead of the bitonic network's 78.
+// running count, so a pass needs five barriers of a four-wave workgroup instead of the bitonic
+// network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
 // Large tiles: one 1024-thread workgroup finds the depth range and writes the tile's entries in
@@ -167,9 +168,6 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   }
 }
 
-constexpr uint32_t kSortThreads = 1024;
-constexpr uint32_t kSortWaves = kSortThreads / 64;
-
 __device__ __forceinline__ uint32_t wave_and(uint32_t v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d, 64);
@@ -181,10 +179,18 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
+// Threads per segment-sort workgroup (16 keys per thread).  Four waves per workgroup keep the
+// passes' barriers cheap, and the register budget is held to four workgroups per CU (the LDS
+// allows four as well), so the global loads of one segment overlap the passes of others.
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortMinWaves = 4;
+constexpr uint32_t kSortWaves = kSortThreads / 64;
+constexpr uint32_t kSortPer = HDSR_MAX_TILE_ENTRIES / kSortThreads;
+
 // One LSD pass over cur[0..n): stable by the digit (key >> shift) & (2^bits - 1), into nxt.
-// Wave w owns positions [w*S, min((w+1)*S, n)) (S <= 256) and walks them in rounds of 64;
-// equal digits inside a round are ranked by ballots, across rounds by a wave-private count.
-// Every thread of the workgroup must call it (it contains barriers).
+// Wave w owns positions [w*S, min((w+1)*S, n)) (S a multiple of 64, at most 64 * kSortPer) and
+// walks them in rounds of 64; equal digits inside a round are ranked by ballots, across rounds by
+// a wave-private count.  Every thread of the workgroup must call it (it contains barriers).
 __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
                                                uint32_t shift, uint32_t bits, uint32_t (*wc)[256],
                                                uint32_t* wsum) {
@@ -194,9 +200,9 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   for (uint32_t q = t; q < kSortWaves * 256; q += kSortThreads) wc[q >> 8][q & 255] = 0;
   __syncthreads();
   const uint32_t lo = w * S, hi = min(lo + S, n);
-  uint32_t keys[4], dig[4], pos[4];
+  uint32_t keys[kSortPer], dig[kSortPer], pos[kSortPer];
 #pragma unroll
-  for (uint32_t r = 0; r < 4; ++r) {
+  for (uint32_t r = 0; r < kSortPer; ++r) {
     const uint32_t p = lo + r * 64 + lane;
     const bool valid = p < hi;
     const uint32_t key = valid ? cur[p] : 0u;
@@ -244,15 +250,18 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   }
   __syncthreads();
 #pragma unroll
-  for (uint32_t r = 0; r < 4; ++r)
+  for (uint32_t r = 0; r < kSortPer; ++r)
     if (pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
   __syncthreads();
 }
 
-__global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
+// Small tiles and chunks: one workgroup per segment (grid-stride).  Keys are built from the
+// streamed (depth, ln depth) pairs; the entries wait in registers during the passes and are then
+// staged in the free ping-pong buffer, where each sorted key's low 12 bits index them (small
+// tiles are sorted in place, so the source range cannot be read after the first write).
+__global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b) {
   __shared__ uint32_t ka[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t kb[HDSR_MAX_TILE_ENTRIES];
-  __shared__ uint32_t vals[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t wc[kSortWaves][256];
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t red[4][kSortWaves];
@@ -263,15 +272,17 @@ __global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint2* esrc = (S.large == kNone ? b.ekey : b.ekey_s) + S.src;
     const uint32_t count = S.count;
-    uint2 dl[4];
+    uint2 dl[kSortPer];
+    uint32_t vv[kSortPer];
     uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
 #pragma unroll
-    for (uint32_t r = 0; r < 4; ++r) {
+    for (uint32_t r = 0; r < kSortPer; ++r) {
       const uint32_t i = r * kSortThreads + t;
       dl[r] = make_uint2(0, 0);
+      vv[r] = 0;
       if (i < count) {
-        vals[i] = src[i];
         dl[r] = esrc[i];
+        vv[r] = src[i];
         dmn = min(dmn, dl[r].x);
         dmx = max(dmx, dl[r].x);
       }
@@ -299,7 +310,7 @@ __global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
     hdsr_log_frame(dmn, dmx, &lmin, &scale);
     uint32_t kand = 0xFFFFFFFFu, kor = 0;
 #pragma unroll
-    for (uint32_t r = 0; r < 4; ++r) {
+    for (uint32_t r = 0; r < kSortPer; ++r) {
       const uint32_t i = r * kSortThreads + t;
       if (i < count) {
         const uint32_t key = (hdsr_q20_of(log_pos(dl[r].y, lmin, scale)) << 12) | i;
@@ -337,10 +348,16 @@ __global__ __launch_bounds__(1024) void k_segsort(Bufs b) {
         nxt = tmp;
       }
     }
+#pragma unroll
+    for (uint32_t r = 0; r < kSortPer; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      if (i < count) nxt[i] = vv[r];
+    }
+    __syncthreads();
     uint32_t* dst = b.entries + S.dst;
     for (uint32_t i = t; i < count; i += kSortThreads) {
       const uint32_t key = cur[i];
-      dst[i] = vals[key & 0xFFFu];
+      dst[i] = nxt[key & 0xFFFu];
       if (b.keys20) b.keys20[S.dst + i] = key;
     }
     __syncthreads();
@@ -357,7 +374,8 @@ void launch_bucket(const Bufs& b, hipStream_t stream) {
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   if (!b.seg_cap) return;
-  const uint32_t grid = b.seg_cap < 2048u ? b.seg_cap : 2048u;
+  const uint32_t cap = 2048u * 1024u / kSortThreads;
+  const uint32_t grid = b.seg_cap < cap ? b.seg_cap : cap;
   hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
 }
 
