@@ -21,9 +21,11 @@ struct Layout {
   uint32_t splat_blocks = 0;
   uint32_t total_tiles = 0;
   uint32_t total_morton = 0;
-  uint32_t key_bits = 0;
+  uint32_t total_cells = 0;
+  uint32_t cell_key_bits = 0;
   std::vector<uint32_t> block_start;  // [V+1]
   std::vector<uint32_t> tile_start;   // [V+1]
+  std::vector<uint32_t> cell_start;   // [V+1]
 };
 
 class Engine {
@@ -74,11 +76,11 @@ class Engine {
   void enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, bool debug, hipEvent_t* ev);
   void drop_graphs();
   struct GraphEntry {
-    uint64_t sig[6];
+    uint64_t sig[7];
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs_;
-  std::vector<std::array<uint64_t, 6>> seen_sigs_;
+  std::vector<std::array<uint64_t, 7>> seen_sigs_;
   uint64_t epoch_ = 0;  // bumped by every (re)allocation: graphs bake buffer addresses
   void* gstage_ = nullptr;
   size_t gstage_bytes_ = 0;
@@ -95,6 +97,7 @@ class Engine {
   // allocation sizes (elements)
   size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;
   size_t n_morton_ = 0, n_tiles_ = 0, n_seg_ = 0, n_frames_ = 0, n_debug_ = 0;
+  size_t n_cellpairs_ = 0, n_cells_ = 0, n_chunkcnt_ = 0, n_radix_tmp_need_ = 0;
   FrameDesc* d_frames_ = nullptr;
   uint32_t* d_starts_ = nullptr;
   // pinned staging ring for the frame table

@@ -40,27 +40,35 @@ Layout Engine::layout(FrameDesc* frames, uint32_t V) {
   L.num_frames = V;
   L.block_start.resize(V + 1);
   L.tile_start.resize(V + 1);
-  uint32_t sb = 0, mb = 0, tb = 0;
+  L.cell_start.resize(V + 1);
+  uint32_t sb = 0, mb = 0, tb = 0, cb = 0;
   for (uint32_t v = 0; v < V; ++v) {
     FrameDesc& F = frames[v];
     F.splat_base = sb;
     F.morton_base = mb;
     F.tile_base = tb;
     F.morton_count = (F.tiles_x && F.tiles_y) ? hdsr_morton2(F.tiles_x - 1, F.tiles_y - 1) + 1 : 0;
+    F.cells_x = (F.tiles_x + kCellTiles - 1) / kCellTiles;
+    F.cells_y = (F.tiles_y + kCellTiles - 1) / kCellTiles;
+    F.cell_base = cb;
     L.block_start[v] = sb / kSplatBlock;
     L.tile_start[v] = tb;
+    L.cell_start[v] = cb;
     sb += align_up(std::max(F.n, 1u), kSplatBlock);
     mb += F.morton_count;
     tb += F.tiles_x * F.tiles_y;
+    cb += F.cells_x * F.cells_y;
   }
   L.block_start[V] = sb / kSplatBlock;
   L.tile_start[V] = tb;
+  L.cell_start[V] = cb;
   L.splat_blocks = sb / kSplatBlock;
   L.total_tiles = tb;
   L.total_morton = mb;
+  L.total_cells = cb;
   uint32_t bits = 0;
-  while (bits < 32 && (1ull << bits) < (uint64_t)mb) ++bits;
-  L.key_bits = bits;
+  while (bits < 32 && (1ull << bits) < (uint64_t)cb) ++bits;
+  L.cell_key_bits = bits;
   return L;
 }
 
@@ -71,8 +79,10 @@ void Engine::release() {
   if (gstage_) (void)hipHostFree(gstage_);
   gstage_ = nullptr;
   gstage_bytes_ = 0;
-  void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre, b_.scan_tmp, b_.large_list,
-                  b_.keys_a, b_.vals_a, b_.keys_b, b_.vals_b, b_.entries, b_.tkeys, b_.keys20, b_.entries_canon, b_.scratch, b_.ekey, b_.ekey_s,
+  void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre,
+                  b_.block_cc, b_.block_ccpre, b_.scan_tmp, b_.large_list, b_.ckeys_a, b_.cvals_a, b_.ckeys_b,
+                  b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
+                  b_.scratch, b_.ekey, b_.ekey_s,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
   for (void* p : ptrs)
@@ -95,6 +105,7 @@ void Engine::release() {
   prof_head_ = prof_count_ = 0;
   n_splat_ = n_blocks_ = n_scan4_ = n_cap_ = n_hist_ = n_hist_tmp_ = 0;
   n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = n_diff_ = 0;
+  n_cellpairs_ = n_cells_ = n_chunkcnt_ = 0;
 }
 
 template <typename T>
@@ -146,26 +157,45 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.block_cnt, &have, L.splat_blocks))) return rc;
     have = n_blocks_;
     if ((rc = grow(&b_.block_pre, &have, L.splat_blocks))) return rc;
+    have = n_blocks_;
+    if ((rc = grow(&b_.block_cc, &have, L.splat_blocks))) return rc;
+    have = n_blocks_;
+    if ((rc = grow(&b_.block_ccpre, &have, L.splat_blocks))) return rc;
     n_blocks_ = L.splat_blocks;
   }
+  // cell pairs: at most kMaxCellsPerSplat per splat slot
+  const size_t ncp = ns * kMaxCellsPerSplat;
+  if (ncp > n_cellpairs_ || !b_.ckeys_a) {
+    have = n_cellpairs_;
+    if ((rc = grow(&b_.ckeys_a, &have, ncp))) return rc;
+    have = n_cellpairs_;
+    if ((rc = grow(&b_.cvals_a, &have, ncp))) return rc;
+    have = n_cellpairs_;
+    if ((rc = grow(&b_.ckeys_b, &have, ncp))) return rc;
+    have = n_cellpairs_;
+    if ((rc = grow(&b_.cvals_b, &have, ncp))) return rc;
+    n_cellpairs_ = ncp;
+  }
+  b_.cell_cap = (uint32_t)std::min<size_t>(n_cellpairs_, 0xFFFFFFF0u);
+  if (L.total_cells > n_cells_ || !b_.cell_offsets) {
+    have = n_cells_ * 4;
+    if ((rc = grow(&b_.cell_offsets, &have, (size_t)L.total_cells * 4))) return rc;
+    n_cells_ = L.total_cells;
+  }
+  b_.cell_counts = b_.cell_offsets + n_cells_;
+  b_.cell_nchunk = b_.cell_offsets + 2 * n_cells_;
+  b_.cell_chunk0 = b_.cell_offsets + 3 * n_cells_;
+  const size_t max_chunks = (n_cellpairs_ + kCellChunk - 1) / kCellChunk + L.total_cells;
+  if ((rc = grow(&b_.chunk_cnt, &n_chunkcnt_, max_chunks * 64))) return rc;
+  b_.max_chunks = (uint32_t)(n_chunkcnt_ / 64);
   const size_t scan4 = scan_u4_tmp_words(L.splat_blocks) + 2 * (size_t)L.total_tiles + scan_u4_tmp_words(L.total_tiles);
   if ((rc = grow(&b_.scan_tmp, &n_scan4_, scan4))) return rc;
   // pair capacity
   uint64_t cap = std::max<uint64_t>({min_cap_, want_cap_, 1});
   cap = std::min<uint64_t>(cap, 0xFFFFFFF0ull);
-  if (cap > n_cap_ || !b_.keys_a) {
-    have = n_cap_;
-    if ((rc = grow(&b_.keys_a, &have, cap))) return rc;
-    have = n_cap_;
-    if ((rc = grow(&b_.vals_a, &have, cap))) return rc;
-    have = n_cap_;
-    if ((rc = grow(&b_.keys_b, &have, cap))) return rc;
-    have = n_cap_;
-    if ((rc = grow(&b_.vals_b, &have, cap))) return rc;
+  if (cap > n_cap_ || !b_.entries) {
     have = n_cap_;
     if ((rc = grow(&b_.entries, &have, cap))) return rc;
-    have = n_cap_;
-    if ((rc = grow(&b_.tkeys, &have, cap))) return rc;
     have = n_cap_;
     if ((rc = grow(&b_.scratch, &have, cap))) return rc;
     have = n_cap_;
@@ -185,9 +215,11 @@ int Engine::ensure(const Layout& L, bool debug) {
       n_debug_ = n_cap_;
     }
   }
-  const size_t rblocks = (n_cap_ + kRadixBlockItems - 1) / kRadixBlockItems;
+  const size_t rblocks = (n_cellpairs_ + kRadixBlockItems - 1) / kRadixBlockItems;
   if ((rc = grow(&b_.radix_hist, &n_hist_, rblocks * kRadixBuckets))) return rc;
-  if ((rc = grow(&b_.radix_tmp, &n_hist_tmp_, scan_u32_tmp_words((uint32_t)(rblocks * kRadixBuckets))))) return rc;
+  const size_t tmpw = std::max({scan_u32_tmp_words((uint32_t)(rblocks * kRadixBuckets)),
+                                scan_u32_tmp_words(L.total_cells), scan_u32_tmp_words(L.total_morton)});
+  if ((rc = grow(&b_.radix_tmp, &n_hist_tmp_, tmpw))) return rc;
   if (L.total_morton > n_morton_ || !b_.tile_offsets) {
     have = n_morton_;
     if ((rc = grow(&b_.tile_offsets, &have, L.total_morton))) return rc;
@@ -206,7 +238,8 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.large_tiles, &have, L.total_tiles))) return rc;
     n_tiles_ = L.total_tiles;
   }
-  const size_t segs = (size_t)L.total_tiles + rblocks + 1;
+  // segments: every small tile, plus at most one 4096-entry chunk per 4096 pairs and per large tile
+  const size_t segs = 2 * (size_t)L.total_tiles + (n_cap_ + HDSR_MAX_TILE_ENTRIES - 1) / HDSR_MAX_TILE_ENTRIES + 1;
   if ((rc = grow(&b_.segs, &n_seg_, segs))) return rc;
   b_.seg_cap = (uint32_t)n_seg_;
   if (!b_.stats) {
@@ -217,7 +250,7 @@ int Engine::ensure(const Layout& L, bool debug) {
     have = n_frames_;
     if ((rc = grow(&d_frames_, &have, L.num_frames))) return rc;
     have = n_frames_;
-    if ((rc = grow(&d_starts_, &have, 2 * (L.num_frames + 1)))) return rc;
+    if ((rc = grow(&d_starts_, &have, 3 * (L.num_frames + 1)))) return rc;
     n_frames_ = L.num_frames;
   }
   if (!hstats_) {
@@ -232,7 +265,7 @@ int Engine::ensure(const Layout& L, bool debug) {
     err = "hipEventCreate failed";
     return HDSR_E_HIP;
   }
-  const size_t sb = sizeof(FrameDesc) * L.num_frames + 8 * (L.num_frames + 1);
+  const size_t sb = sizeof(FrameDesc) * L.num_frames + 12 * (L.num_frames + 1);
   if (sb > stage_bytes_) {
     for (int i = 0; i < kRing; ++i) {
       if (stage_ev_[i]) (void)hipEventSynchronize(stage_ev_[i]);
@@ -253,11 +286,13 @@ int Engine::ensure(const Layout& L, bool debug) {
   b_.frames = d_frames_;
   b_.block_frame_start = d_starts_;
   b_.tile_frame_start = d_starts_ + (L.num_frames + 1);
+  b_.cell_frame_start = d_starts_ + 2 * (L.num_frames + 1);
   b_.num_frames = L.num_frames;
   b_.splat_blocks = L.splat_blocks;
   b_.total_tiles = L.total_tiles;
   b_.total_morton = L.total_morton;
-  b_.key_bits = L.key_bits;
+  b_.total_cells = L.total_cells;
+  b_.cell_key_bits = L.cell_key_bits;
   if (!debug) {
     // keep the buffers, but the kernels only write them when asked
   }
@@ -278,9 +313,10 @@ int Engine::upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t 
   for (uint32_t v = 0; v <= L.num_frames; ++v) {
     st[v] = L.block_start[v];
     st[L.num_frames + 1 + v] = L.tile_start[v];
+    st[2 * (L.num_frames + 1) + v] = L.cell_start[v];
   }
   if (hipMemcpyAsync(d_frames_, s, fb, hipMemcpyHostToDevice, stream) != hipSuccess ||
-      hipMemcpyAsync(d_starts_, st, 8 * (L.num_frames + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
+      hipMemcpyAsync(d_starts_, st, 12 * (L.num_frames + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
       hipEventRecord(stage_ev_[slot], stream) != hipSuccess) {
     err = "frame table upload failed";
     return HDSR_E_HIP;
@@ -330,7 +366,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
   int rc = ensure(L, false);
   if (rc) return rc;
   lay_ = L;
-  const size_t fb = sizeof(FrameDesc) * V, sb = 8 * (size_t)(V + 1);
+  const size_t fb = sizeof(FrameDesc) * V, sb = 12 * (size_t)(V + 1);
   if (fb + sb > gstage_bytes_) {
     if (gstage_) (void)hipHostFree(gstage_);
     gstage_ = nullptr;
@@ -355,8 +391,10 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
   for (uint32_t v = 0; v <= V; ++v) {
     tab[v] = L.block_start[v];
     tab[V + 1 + v] = L.tile_start[v];
+    tab[2 * (V + 1) + v] = L.cell_start[v];
   }
-  const uint64_t sig[6] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.key_bits, (uint64_t)gbuffer};
+  const uint64_t sig[7] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.total_cells, L.cell_key_bits,
+                           (uint64_t)gbuffer};
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& g : graphs_)
     if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
@@ -365,7 +403,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     bool seen = false;
     for (const auto& s : seen_sigs_) seen |= memcmp(s.data(), sig, sizeof sig) == 0;
     if (!seen) {
-      std::array<uint64_t, 6> a;
+      std::array<uint64_t, 7> a;
       memcpy(a.data(), sig, sizeof sig);
       if (seen_sigs_.size() >= 1024) seen_sigs_.erase(seen_sigs_.begin());
       seen_sigs_.push_back(a);

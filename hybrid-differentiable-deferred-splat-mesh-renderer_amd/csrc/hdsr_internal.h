@@ -6,8 +6,12 @@
 //   splats  g in [splat_base, splat_base + n)          (splat_base a multiple of 256)
 //   Morton  m in [morton_base, morton_base + morton_count)
 //   tiles   t in [tile_base, tile_base + tiles_x * tiles_y)   (row-major inside the frame)
-// A tile pair's key is morton_base + morton(tx, ty), so one stable radix sort over the batch
-// groups every frame's tiles at once and keeps each frame's canonical per-tile order.
+//   cells   c in [cell_base, cell_base + cells_x * cells_y) (8 x 8-tile cells, row-major)
+// Tile lists are built in two levels: every visible splat emits one (cell, rank) pair per cell
+// its tile box touches, in canonical order (small splats by rank, then large splats by rank); one
+// stable radix sort on the global cell id groups them; each cell's list is then expanded into
+// its 64 tiles' lists with ballots, which keeps the canonical order inside every tile.  The
+// entry list is laid out by global Morton slot (morton_base + morton(tx, ty)).
 //
 // Projected splat records (the reference's 14-word cache, gpu_project_splats.wgsl:2-16, split
 // by who reads it):
@@ -29,6 +33,9 @@ constexpr uint32_t kRadixBuckets = 1u << kRadixBits;
 constexpr uint32_t kRadixBlockItems = 4096;  // pairs per radix workgroup (256 threads x 16)
 constexpr uint32_t kScanChunk = 4096;        // elements per workgroup of the generic scan
 constexpr uint32_t kMaxFrames = 64;
+constexpr uint32_t kCellTiles = 8;            // a cell is 8 x 8 tiles
+constexpr uint32_t kCellChunk = 1024;         // cell-list entries per expansion workgroup
+constexpr uint32_t kMaxCellsPerSplat = 9;     // a tile box spans at most 17 tiles per axis
 
 struct FrameDesc {
   // scene (device pointers)
@@ -53,6 +60,7 @@ struct FrameDesc {
   uint32_t tiles_x, tiles_y;
   // batch layout
   uint32_t splat_base, morton_base, morton_count, tile_base;
+  uint32_t cells_x, cells_y, cell_base;
   uint32_t width, height;
   // outputs (device pointers)
   float* rgba;           // width*height*4
@@ -82,7 +90,9 @@ struct DevStats {
   uint32_t scratch_used;
   uint32_t composite_refs;      // entries the rasterizer loaded (up to each tile's saturation)
   uint32_t composite_max_refs;  // largest per-tile count of those
-  uint32_t pad[3];
+  uint32_t cell_pairs;          // (cell, rank) pairs
+  uint32_t cell_chunks;         // expansion workgroups needed (sum over cells of ceil(count / kCellChunk))
+  uint32_t pad;
 };
 
 struct Seg {
@@ -105,11 +115,13 @@ struct Bufs {
   const FrameDesc* frames;   // [V]
   const uint32_t* block_frame_start;  // [V+1] first splat block of each frame (host-built)
   const uint32_t* tile_frame_start;   // [V+1] first tile of each frame
+  const uint32_t* cell_frame_start;   // [V+1] first cell of each frame
   uint32_t num_frames;
   uint32_t splat_blocks;     // sum of padded splats / 256
   uint32_t total_tiles;      // sum of tiles
   uint32_t total_morton;     // sum of Morton spans
-  uint32_t key_bits;         // bits needed for a global Morton key
+  uint32_t total_cells;      // sum of cells
+  uint32_t cell_key_bits;    // bits needed for a global cell id
   uint4* hot;                // [splat_blocks*256*2]
   uint4* cold;               // [splat_blocks*256*2]
   float4* drec;              // [splat_blocks*256*3] fp32 record of the differentiable mode, or null:
@@ -120,22 +132,30 @@ struct Bufs {
   uint2* dlog;               // [splat_blocks*256] (depth bits, ln(max(depth, 1e-6)) bits) for the sorts
   uint4* block_cnt;          // [splat_blocks] per block (small pairs, large pairs, large splats, visible)
   uint4* block_pre;          // [splat_blocks] exclusive prefix of block_cnt
+  uint4* block_cc;           // [splat_blocks] per block (small-splat cell pairs, large-splat cell pairs, 0, 0)
+  uint4* block_ccpre;        // [splat_blocks] exclusive prefix of block_cc
   uint4* scan_tmp;           // scratch of the uint4 scan
   uint32_t* large_list;      // [splat_blocks*256] sort ranks (global g) of large splats
-  uint32_t* keys_a;          // [cap] pair keys (global Morton key), emission order
-  uint32_t* vals_a;          // [cap] pair values (global splat index g)
-  uint32_t* keys_b;          // [cap] radix ping-pong
-  uint32_t* vals_b;          // [cap]
+  uint32_t* ckeys_a;         // [cell_cap] cell pair keys (global cell id), emission order
+  uint32_t* cvals_a;         // [cell_cap] cell pair values (global splat index g)
+  uint32_t* ckeys_b;         // [cell_cap] radix ping-pong
+  uint32_t* cvals_b;         // [cell_cap]
+  uint32_t cell_cap;         // kMaxCellsPerSplat per splat slot
+  uint32_t* cell_offsets;    // [total_cells] first entry of each cell in the grouped cell list
+  uint32_t* cell_counts;     // [total_cells]
+  uint32_t* cell_nchunk;     // [total_cells] expansion chunks of each cell
+  uint32_t* cell_chunk0;     // [total_cells] first expansion chunk of each cell (scan of cell_nchunk)
+  uint32_t* chunk_cnt;       // [max_chunks * 64] per chunk and tile of its cell: count, then prefix
+  uint32_t max_chunks;
   uint32_t* entries;         // [cap] final per-tile order (global g)
-  uint32_t* tkeys;           // [cap] Morton key of each entry after grouping
   uint32_t* keys20;          // [cap] (q20 << 12) | local index, or null
   uint32_t* entries_canon;   // [cap] entries before the per-tile sort, or null
   uint32_t* scratch;         // [cap] bucket-ordered copies of large tiles
   uint2* ekey;               // [cap] dlog[] of each grouped entry (same order as entries)
   uint2* ekey_s;             // [cap] dlog[] of each scratch entry
   uint32_t cap;              // pair capacity
-  uint32_t* radix_hist;      // [256 * radix blocks] digit-major
-  uint32_t* radix_tmp;       // scratch of the u32 scan
+  uint32_t* radix_hist;      // [256 * radix blocks] digit-major (radix sort of the cell pairs)
+  uint32_t* radix_tmp;       // scratch of the u32 scans (radix histogram, cell chunks, tile counts)
   uint32_t* tile_offsets;    // [total_morton]
   uint32_t* tile_counts;     // [total_morton]
   uint32_t* small_list;      // [total_tiles] global Morton slots
@@ -151,9 +171,9 @@ struct Bufs {
 void launch_clear_stats(const Bufs& b, hipStream_t stream);
 void launch_project(const Bufs& b, hipStream_t stream);
 void launch_scan_blocks(const Bufs& b, hipStream_t stream);
-void launch_emit_pairs(const Bufs& b, hipStream_t stream);
-void launch_group_pairs(const Bufs& b, hipStream_t stream);   // radix passes on the Morton key
-void launch_tile_ranges(const Bufs& b, hipStream_t stream);
+void launch_emit_pairs(const Bufs& b, hipStream_t stream);    // (cell, rank) pairs
+void launch_group_pairs(const Bufs& b, hipStream_t stream);   // radix on the cell id, tile counts
+void launch_tile_ranges(const Bufs& b, hipStream_t stream);   // tile offsets, expansion into tile lists
 void launch_classify(const Bufs& b, hipStream_t stream);
 void launch_bucket(const Bufs& b, hipStream_t stream);
 void launch_segment_sort(const Bufs& b, hipStream_t stream);

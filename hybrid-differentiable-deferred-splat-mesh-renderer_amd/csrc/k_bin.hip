@@ -1,4 +1,5 @@
-// k_bin.hip - count -> hierarchical prefix scan -> scatter -> radix grouping -> tile ranges -> classify.
+// k_bin.hip - count -> hierarchical prefix scan -> pair emission -> grouping into tile lists ->
+// classify.
 //
 // Reference: count_tile_refs / scatter_tile_refs (src/shaders/gpu_tile_splat_composite.wgsl:64-119),
 // large scatter (gpu_large_splat_scatter.wgsl:52-89), scan / scan_block_sums / add_block_sums
@@ -6,14 +7,17 @@
 // src/gpuRadixSort.ts:313-354) and classify_tiles (gpu_tile_classify.wgsl:44-86).
 //
 // The reference resolves slots with atomics, so its per-tile lists come out in arbitrary order.
-// Here the order is canonical by construction: every visible splat's pairs (Morton key, rank)
-// are written at an offset from an exclusive scan over the per-splat tile counts (small splats
-// in ascending rank, then large splats in ascending rank), and a stable LSD radix sort on the
-// Morton key groups them by tile while keeping that order.  The key width decides the passes
-// (at most 8 bits each: a 1080p frame's 14-bit Morton range takes two 7-bit passes, as the
-// reference's 4 x 4-bit passes over 16 bits give the same stable order); each pass is a
-// per-workgroup digit histogram, a hierarchical scan of the digit-major histogram, and a stable
-// scatter that ranks equal digits inside each wave with ballots.  All of it is deterministic.
+// Here the order is canonical by construction: every tile's list holds the splats whose boxes
+// cover it, small splats in ascending rank, then large splats in ascending rank (the order a
+// stable sort of the reference's pairs on the Morton key gives).  The lists are built in two
+// levels.  Each visible splat emits one (cell, rank) pair per 8 x 8-tile cell its box touches
+// (about 3 per splat instead of about 40 tile pairs), at offsets from an exclusive scan of the
+// per-workgroup counts; a stable LSD radix sort on the global cell id groups them (one 8-bit pass
+// for a 1080p frame's 135 cells); then every cell's list is expanded into its 64 tiles' lists
+// with ballots (counting pass, per-cell prefix over chunks, scan of the tile counts over Morton
+// slots, writing pass).  All of it is deterministic.
+#include <algorithm>
+
 #include "hdsr_internal.h"
 
 namespace hdsr {
@@ -159,6 +163,7 @@ __global__ void k_clear_stats(DevStats* s) {
 __global__ void k_totals(Bufs b) {
   const uint32_t last = b.splat_blocks - 1;
   const uint4 t = b.block_pre[last] + b.block_cnt[last];
+  const uint4 c = b.block_ccpre[last] + b.block_cc[last];
   DevStats* s = b.stats;
   s->small_pairs = t.x;
   s->large_pairs = t.y;
@@ -167,29 +172,47 @@ __global__ void k_totals(Bufs b) {
   const uint64_t refs = (uint64_t)t.x + t.y;
   s->refs = refs > b.cap ? b.cap : (uint32_t)refs;
   s->overflow = refs > b.cap ? 1u : 0u;
+  const uint64_t cells = (uint64_t)c.x + c.y;  // <= kMaxCellsPerSplat per splat by construction
+  s->cell_pairs = cells > b.cell_cap ? b.cell_cap : (uint32_t)cells;
 }
 
+// The cell box of a tile box: first cell, width and height in cells (0 when invisible / empty).
+__device__ __forceinline__ uint32_t cell_box(uint32_t bd, uint32_t* cx0, uint32_t* cy0, uint32_t* cw) {
+  if (bd == HDSR_SENTINEL) return 0u;
+  uint32_t x0, y0, x1, y1;
+  box_of(bd, &x0, &y0, &x1, &y1);
+  if (x1 < x0 || y1 < y0) return 0u;
+  *cx0 = x0 >> 3;
+  *cy0 = y0 >> 3;
+  *cw = (x1 >> 3) - *cx0 + 1u;
+  return *cw * ((y1 >> 3) - *cy0 + 1u);
+}
+
+// First level of the tile lists: one (global cell id, g) pair per cell a visible splat's box
+// touches, small splats' pairs first (ascending rank), then large splats' (ascending rank).  A
+// wave's pairs of one class are one contiguous output range, written with consecutive lanes:
+// pair p belongs to the lane whose running start is the last one <= p.  Also writes the list of
+// large splats (classify_large_splats, gpu_classify_large_splats.wgsl:33-56).
 __global__ __launch_bounds__(256) void k_emit(Bufs b) {
   const uint32_t blk = blockIdx.x;
   const uint32_t f = frame_of(b.block_frame_start, b.num_frames, blk);
-  const uint32_t mbase = b.frames[f].morton_base;
+  const FrameDesc& F = b.frames[f];
   const uint32_t g = blk * kSplatBlock + threadIdx.x;
   const uint32_t bd = b.bounds[g];
   const uint4 c = counts_of(bd);
+  uint32_t cx0 = 0, cy0 = 0, cw = 1;
+  const uint32_t nc = cell_box(bd, &cx0, &cy0, &cw);
+  const uint4 v = make_uint4(c.z ? 0u : nc, c.z ? nc : 0u, c.z, 0u);
   uint4 tot;
-  const uint4 pre = block_excl_256<uint4>(c, &tot) + b.block_pre[blk];
-  const uint32_t small_total = b.stats->small_pairs;
+  const uint4 bp = b.block_ccpre[blk];
+  const uint4 pre = block_excl_256<uint4>(v, &tot) + make_uint4(bp.x, bp.y, b.block_pre[blk].z, 0u);
   if (c.z) b.large_list[pre.z] = g;
-  // A wave's small-splat pairs occupy one contiguous range of the output and its large-splat
-  // pairs another (both classes are in ascending rank), so the wave writes each range with
-  // consecutive lanes: pair p of the range belongs to the lane whose running start is the last
-  // one <= p, and is that splat's (p - start)-th tile in row-major order of its box.
+  const uint32_t small_total = b.block_ccpre[b.splat_blocks - 1].x + b.block_cc[b.splat_blocks - 1].x;
   const uint32_t lane = __lane_id();
-  const uint32_t cap = b.cap;
   const uint32_t g0 = g - lane;
 #pragma unroll
   for (uint32_t cls = 0; cls < 2; ++cls) {
-    const uint32_t n = cls ? c.y : c.x;
+    const uint32_t n = cls ? v.y : v.x;
     uint32_t inc = n;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -210,38 +233,34 @@ __global__ __launch_bounds__(256) void k_emit(Bufs b) {
       }
       const uint32_t o_loc = __shfl(loc, lo, 64);
       const uint32_t o_off = __shfl(off, lo, 64);
-      const uint32_t o_bd = __shfl(bd, lo, 64);
+      const uint32_t o_x = __shfl(cx0, lo, 64), o_y = __shfl(cy0, lo, 64), o_w = __shfl(cw, lo, 64);
       if (p < total) {
         const uint32_t j = p - o_loc;
-        uint32_t x0, y0, x1, y1;
-        box_of(o_bd, &x0, &y0, &x1, &y1);
-        const uint32_t sx = x1 - x0 + 1u;
-        const uint32_t ty = y0 + j / sx, tx = x0 + j % sx;
         const uint32_t dst = o_off + j;
-        if (dst < cap) {
-          b.keys_a[dst] = mbase + hdsr_morton2(tx, ty);
-          b.vals_a[dst] = g0 + lo;
+        if (dst < b.cell_cap) {
+          b.ckeys_a[dst] = F.cell_base + (o_y + j / o_w) * F.cells_x + (o_x + j % o_w);
+          b.cvals_a[dst] = g0 + lo;
         }
       }
     }
   }
 }
 
-// ---- stable LSD radix passes on the Morton key ---------------------------------------------
+// ---- stable LSD radix passes on the cell id -------------------------------------------------
 //
 // A pass over blocks of kRadixBlockItems pairs: k_radix_hist counts the digit of each key per
 // block (digit-major table, then scanned), k_radix_scatter ranks each pair inside its block and
 // writes it to its global slot.  In the scatter each wave owns 1024 consecutive pairs of the
 // block and ranks equal digits with ballots plus a wave-private running count (no workgroup
 // barrier per round); the block's pairs are then reordered in LDS so that the global writes of
-// each digit run are contiguous.
+// each digit run are contiguous.  *count is the number of pairs (device).
 
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t* hist, const DevStats* st,
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t* hist, const uint32_t* count,
                                                     uint32_t nblocks, uint32_t shift, uint32_t mask) {
   __shared__ uint32_t cnt[kRadixBuckets];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t n = st->refs;
+  const uint32_t n = *count;
   const uint32_t base = blockIdx.x * kRadixBlockItems;
   if (base < n) {
 #pragma unroll 4
@@ -254,19 +273,16 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
   hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// With 'dlog' set (the last pass), also writes each moved pair's (depth bits, log-depth) to
-// ekey_out at the same position, so the per-tile sorts read them as a stream.
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                                                       uint32_t* vout, const uint32_t* hist, const DevStats* st,
-                                                       uint32_t nblocks, uint32_t shift, uint32_t bits,
-                                                       const uint2* dlog, uint2* ekey_out) {
+                                                       uint32_t* vout, const uint32_t* hist, const uint32_t* count,
+                                                       uint32_t nblocks, uint32_t shift, uint32_t bits) {
   __shared__ uint32_t sk[kRadixBlockItems];
   __shared__ uint32_t sv[kRadixBlockItems];
   __shared__ uint32_t wc[4][kRadixBuckets];
   __shared__ uint32_t dbase[kRadixBuckets];
   __shared__ uint32_t gbase[kRadixBuckets];
   __shared__ uint32_t wsum[4];
-  const uint32_t n = st->refs;
+  const uint32_t n = *count;
   const uint32_t base = blockIdx.x * kRadixBlockItems;
   if (base >= n) return;  // uniform per workgroup
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
@@ -337,33 +353,199 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, cons
     const uint32_t kk = sk[q];
     const uint32_t d = (kk >> shift) & mask;
     const uint32_t dst = gbase[d] + (q - dbase[d]);
-    const uint32_t vv = sv[q];
     kout[dst] = kk;
-    vout[dst] = vv;
-    if (dlog) ekey_out[dst] = dlog[vv];
+    vout[dst] = sv[q];
   }
 }
 
-// ---- tile ranges: offsets / counts per global Morton slot ---------------------------------
+// ---- cells -> tile lists ----------------------------------------------------------------------
+//
+// Each cell's grouped list (canonical order) is cut into chunks of kCellChunk entries; one
+// 256-thread workgroup per chunk, each wave walking 256 consecutive entries in rounds of 64.
+// A lane holds one entry's 64-bit mask of the cell's tiles its box covers; for tile t, a ballot
+// of bit t gives the lanes covering t, and their ranks among those lanes.  Counting pass: lane t
+// accumulates tile t's count.  After a per-cell prefix over chunks and a scan of the tile counts
+// over Morton slots, the writing pass repeats the walk and stores each (tile, entry) at the
+// tile's offset + the chunk's prefix + the waves before + the rank: the canonical order of the
+// cell list is kept inside every tile.
 
-__global__ void k_tile_ranges(const uint32_t* tkeys, const DevStats* st, uint32_t total_morton,
-                              uint32_t* offsets, uint32_t* counts) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= total_morton) return;
-  const uint32_t n = st->refs;
+__global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= b.total_cells) return;
+  const uint32_t n = b.stats->cell_pairs;
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (tkeys[mid] < m) lo = mid + 1; else hi = mid;
+    if (keys[mid] < c) lo = mid + 1; else hi = mid;
   }
-  const uint32_t start = lo;
+  const uint32_t s = lo;
   hi = n;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (tkeys[mid] <= m) lo = mid + 1; else hi = mid;
+    if (keys[mid] <= c) lo = mid + 1; else hi = mid;
   }
-  offsets[m] = start;
-  counts[m] = lo - start;
+  b.cell_offsets[c] = s;
+  b.cell_counts[c] = lo - s;
+  b.cell_nchunk[c] = (lo - s + kCellChunk - 1) / kCellChunk;
+}
+
+__global__ void k_chunk_total(Bufs b) {
+  const uint32_t last = b.total_cells - 1;
+  b.stats->cell_chunks = b.cell_chunk0[last] + b.cell_nchunk[last];
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+struct ChunkInfo {
+  uint32_t cell, first, count, cx, cy;
+  const FrameDesc* F;
+};
+
+// The cell and entry range of expansion chunk q (uniform per workgroup).
+__device__ __forceinline__ ChunkInfo chunk_info(const Bufs& b, uint32_t q) {
+  uint32_t lo = 0, hi = b.total_cells;  // last cell with chunk0 <= q
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (b.cell_chunk0[mid] <= q) lo = mid; else hi = mid;
+  }
+  ChunkInfo ci;
+  ci.cell = lo;
+  const uint32_t k = q - b.cell_chunk0[lo];
+  ci.first = b.cell_offsets[lo] + k * kCellChunk;
+  ci.count = min(kCellChunk, b.cell_counts[lo] - k * kCellChunk);
+  const uint32_t f = frame_of(b.cell_frame_start, b.num_frames, lo);
+  ci.F = &b.frames[f];
+  const uint32_t lc = lo - ci.F->cell_base;
+  ci.cx = lc % ci.F->cells_x;
+  ci.cy = lc / ci.F->cells_x;
+  return ci;
+}
+
+// Bit (ly * 8 + lx) set for each tile (lx, ly) of cell (cx, cy) inside the tile box bd.
+__device__ __forceinline__ unsigned long long cover_mask(uint32_t bd, uint32_t cx, uint32_t cy) {
+  if (bd == HDSR_SENTINEL) return 0ull;
+  uint32_t x0, y0, x1, y1;
+  box_of(bd, &x0, &y0, &x1, &y1);
+  const int bx = (int)(cx * kCellTiles), by = (int)(cy * kCellTiles);
+  const int lx0 = max((int)x0 - bx, 0), lx1 = min((int)x1 - bx, 7);
+  const int ly0 = max((int)y0 - by, 0), ly1 = min((int)y1 - by, 7);
+  if (lx0 > lx1 || ly0 > ly1) return 0ull;
+  const unsigned long long row = (unsigned long long)(((2u << lx1) - (1u << lx0)) & 0xFFu);
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    if (r >= ly0 && r <= ly1) m |= row << (8 * r);
+  return m;
+}
+
+// Per-wave counts of the cell's 64 tiles over this wave's entries of the chunk: lane t returns
+// tile t's count.
+__device__ __forceinline__ uint32_t wave_tile_counts(const Bufs& b, const uint32_t* cv, const ChunkInfo& ci,
+                                                     uint32_t w, uint32_t lane) {
+  uint32_t my = 0;
+#pragma unroll 1
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t i = w * 256 + r * 64 + lane;
+    const unsigned long long mk = i < ci.count ? cover_mask(b.bounds[cv[ci.first + i]], ci.cx, ci.cy) : 0ull;
+#pragma unroll
+    for (uint32_t t = 0; t < 64; ++t) {
+      const uint32_t pc = (uint32_t)__popcll(__ballot((mk >> t) & 1ull));
+      my += lane == t ? pc : 0u;
+    }
+  }
+  return my;
+}
+
+__global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) {
+  __shared__ uint32_t wcnt[4][64];
+  const uint32_t q = blockIdx.x;
+  if (q >= b.stats->cell_chunks) return;  // uniform
+  const ChunkInfo ci = chunk_info(b, q);
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
+  wcnt[w][lane] = wave_tile_counts(b, cv, ci, w, lane);
+  __syncthreads();
+  if (t < 64) b.chunk_cnt[(size_t)q * 64 + t] = (wcnt[0][t] + wcnt[1][t]) + (wcnt[2][t] + wcnt[3][t]);
+}
+
+// One 64-thread workgroup per cell: exclusive prefix of each tile's counts over the cell's chunks
+// (in place), and the tile's total into tile_counts.
+__global__ __launch_bounds__(64) void k_cell_prefix(Bufs b) {
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t f = frame_of(b.cell_frame_start, b.num_frames, c);
+  const FrameDesc& F = b.frames[f];
+  const uint32_t lc = c - F.cell_base;
+  const uint32_t cx = lc % F.cells_x, cy = lc / F.cells_x;
+  const uint32_t q0 = b.cell_chunk0[c], nq = b.cell_nchunk[c];
+  uint32_t run = 0;
+  for (uint32_t q = q0; q < q0 + nq; ++q) {
+    const uint32_t v = b.chunk_cnt[(size_t)q * 64 + t];
+    b.chunk_cnt[(size_t)q * 64 + t] = run;
+    run += v;
+  }
+  const uint32_t tx = cx * kCellTiles + (t & 7u), ty = cy * kCellTiles + (t >> 3);
+  if (tx < F.tiles_x && ty < F.tiles_y) b.tile_counts[F.morton_base + hdsr_morton2(tx, ty)] = run;
+}
+
+// Keeps every tile's range inside the pair capacity.  Only a frame whose references exceed the
+// capacity is affected (its lists are cut; the host grows the capacity and renders it again),
+// but every later stage then reads and writes inside the entry buffers.
+__global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, uint32_t cap) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t s = offsets[i], e = s + counts[i];
+  const uint32_t cs = (uint32_t)(s < cap ? s : cap), ce = (uint32_t)(e < cap ? e : cap);
+  offsets[i] = cs;
+  counts[i] = ce - cs;
+}
+
+__global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) {
+  __shared__ uint32_t wcnt[4][64];
+  const uint32_t q = blockIdx.x;
+  if (q >= b.stats->cell_chunks) return;  // uniform
+  const ChunkInfo ci = chunk_info(b, q);
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  wcnt[w][lane] = wave_tile_counts(b, cv, ci, w, lane);
+  __syncthreads();
+  // lane t: where this wave's entries of tile t start
+  const FrameDesc& F = *ci.F;
+  const uint32_t tx = ci.cx * kCellTiles + (lane & 7u), ty = ci.cy * kCellTiles + (lane >> 3);
+  uint32_t base = 0;
+  if (tx < F.tiles_x && ty < F.tiles_y) {
+    base = b.tile_offsets[F.morton_base + hdsr_morton2(tx, ty)] + b.chunk_cnt[(size_t)q * 64 + lane];
+    for (uint32_t v = 0; v < w; ++v) base += wcnt[v][lane];
+  }
+  const uint32_t cap = b.cap;
+#pragma unroll 1
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t i = w * 256 + r * 64 + lane;
+    uint32_t g = 0;
+    uint2 ek = make_uint2(0, 0);
+    unsigned long long mk = 0ull;
+    if (i < ci.count) {
+      g = cv[ci.first + i];
+      mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
+      ek = b.dlog[g];
+    }
+#pragma unroll
+    for (uint32_t tt = 0; tt < 64; ++tt) {
+      const bool in = (mk >> tt) & 1ull;
+      const unsigned long long m = __ballot(in);
+      if (m) {  // uniform
+        const uint32_t bt = __shfl(base, tt, 64);
+        if (in) {
+          const uint32_t dst = bt + (uint32_t)__popcll(m & below);
+          if (dst < cap) {
+            b.entries[dst] = g;
+            b.ekey[dst] = ek;
+          }
+        }
+        base += lane == tt ? (uint32_t)__popcll(m) : 0u;
+      }
+    }
+  }
 }
 
 // ---- classify: small / large tile lists and the sort segments, in row-major tile order -----
@@ -441,6 +623,7 @@ void launch_clear_stats(const Bufs& b, hipStream_t stream) {
 void launch_scan_blocks(const Bufs& b, hipStream_t stream) {
   if (!b.splat_blocks) return;
   scan_u4_exclusive(b.block_cnt, b.block_pre, b.splat_blocks, b.scan_tmp, stream);
+  scan_u4_exclusive(b.block_cc, b.block_ccpre, b.splat_blocks, b.scan_tmp, stream);
   hipLaunchKernelGGL(k_totals, dim3(1), dim3(1), 0, stream, b);
 }
 
@@ -449,35 +632,51 @@ void launch_emit_pairs(const Bufs& b, hipStream_t stream) {
   hipLaunchKernelGGL(k_emit, dim3(b.splat_blocks), dim3(kSplatBlock), 0, stream, b);
 }
 
+// Passes and digit width of the cell-id radix sort; the sorted pairs end in the b buffers after an
+// odd number of passes, in the a buffers after an even one.
+static uint32_t cell_passes(const Bufs& b) {
+  return b.cell_key_bits == 0 ? 1u : (b.cell_key_bits + kRadixBits - 1) / kRadixBits;
+}
+static const uint32_t* sorted_cell_keys(const Bufs& b) { return cell_passes(b) & 1u ? b.ckeys_b : b.ckeys_a; }
+static const uint32_t* sorted_cell_vals(const Bufs& b) { return cell_passes(b) & 1u ? b.cvals_b : b.cvals_a; }
+
 void launch_group_pairs(const Bufs& b, hipStream_t stream) {
-  const uint32_t nblocks = (b.cap + kRadixBlockItems - 1) / kRadixBlockItems;
-  if (!nblocks) return;
-  const uint32_t passes = b.key_bits == 0 ? 1u : (b.key_bits + kRadixBits - 1) / kRadixBits;
-  const uint32_t bits = b.key_bits == 0 ? 1u : (b.key_bits + passes - 1) / passes;
+  const uint32_t nblocks = (b.cell_cap + kRadixBlockItems - 1) / kRadixBlockItems;
+  if (!nblocks || !b.total_cells) return;
+  const uint32_t passes = cell_passes(b);
+  const uint32_t bits = b.cell_key_bits == 0 ? 1u : (b.cell_key_bits + passes - 1) / passes;
   const uint32_t mask = (1u << bits) - 1u;
-  const uint32_t* ks = b.keys_a;
-  const uint32_t* vs = b.vals_a;
+  const uint32_t* count = &b.stats->cell_pairs;
+  uint32_t* ks = b.ckeys_a;
+  uint32_t* vs = b.cvals_a;
   for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t shift = p * bits;
-    const bool last = p + 1 == passes;
-    // the last pass writes the grouped entries and their keys; earlier ones ping-pong
-    uint32_t* kd = last ? b.tkeys : (ks == b.keys_a ? b.keys_b : b.keys_a);
-    uint32_t* vd = last ? b.entries : (vs == b.vals_a ? b.vals_b : b.vals_a);
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, ks, b.radix_hist, b.stats, nblocks, shift,
-                       mask);
+    uint32_t* kd = ks == b.ckeys_a ? b.ckeys_b : b.ckeys_a;
+    uint32_t* vd = vs == b.cvals_a ? b.cvals_b : b.cvals_a;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, ks, b.radix_hist, count, nblocks,
+                       p * bits, mask);
     scan_u32_exclusive(b.radix_hist, b.radix_hist, nblocks * kRadixBuckets, b.radix_tmp, stream);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, stream, ks, vs, kd, vd, b.radix_hist,
-                       b.stats, nblocks, shift, bits, last ? (const uint2*)b.dlog : (const uint2*)nullptr,
-                       last ? b.ekey : (uint2*)nullptr);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, stream, ks, vs, kd, vd, b.radix_hist, count,
+                       nblocks, p * bits, bits);
     ks = kd;
     vs = vd;
   }
+  // cell ranges and expansion chunks, per-chunk tile counts, per-cell prefixes -> tile counts
+  const uint32_t cg = (b.total_cells + 255) / 256;
+  hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, sorted_cell_keys(b));
+  scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
+  hipLaunchKernelGGL(k_chunk_total, dim3(1), dim3(1), 0, stream, b);
+  hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((b.total_morton + 255) / 256, 1024u)), dim3(256), 0, stream,
+                     b.tile_counts, b.total_morton, 0u);
+  hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
 }
 
 void launch_tile_ranges(const Bufs& b, hipStream_t stream) {
   if (!b.total_morton) return;
-  hipLaunchKernelGGL(k_tile_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tkeys, b.stats,
-                     b.total_morton, b.tile_offsets, b.tile_counts);
+  scan_u32_exclusive(b.tile_counts, b.tile_offsets, b.total_morton, b.radix_tmp, stream);
+  hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
+                     b.tile_counts, b.total_morton, b.cap);
+  if (b.total_cells) hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
 }
 
 void launch_classify(const Bufs& b, hipStream_t stream) {

@@ -8,8 +8,9 @@
 // The splat is gathered through sortedIndices (the only uncoalesced reads); the projected record
 // is written as two coalesced 32-byte records plus the tile box and the depth bits.  Instead of
 // atomically counting tiles, each workgroup reduces (small-splat pairs, large-splat pairs, large
-// splats, visible splats) into one uint4: the scan of those block sums gives every splat's pair
-// offset in canonical order with no atomics.
+// splats, visible splats) into one uint4, and the (small, large) counts of 8 x 8-tile cells the
+// boxes touch into another: the scans of those block sums give every splat its offsets in
+// canonical order with no atomics.
 //
 // Arithmetic is IEEE fp32 with no contraction, in the operation order of the WGSL, so the
 // record is bit-identical to the CPU restatement.
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   const uint32_t g = blk * kSplatBlock + threadIdx.x;
   const uint32_t r = g - F.splat_base;
 
-  uint32_t n_small = 0, n_large = 0, is_large = 0, vis = 0;
+  uint32_t n_small = 0, n_large = 0, is_large = 0, vis = 0, c_small = 0, c_large = 0;
   uint32_t bounds = HDSR_SENTINEL, depth_bits = HDSR_SENTINEL;
   uint4 h0 = make_uint4(0, 0, 0, 0), h1 = make_uint4(0, 0, 0, 0);
   uint4 c0 = make_uint4(0, 0, 0, 0), c1 = make_uint4(0, 0, 0, 0);
@@ -119,11 +120,16 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
         const uint32_t span = (bx1 - bx0 + 1u) * (by1 - by0 + 1u);  // u32 wrap as the WGSL
         const uint32_t ntiles = (bx1 >= bx0 && by1 >= by0) ? (bx1 - bx0 + 1u) * (by1 - by0 + 1u) : 0u;
         vis = 1;
+        // 8 x 8-tile cells the box touches (the first level of the tile-list build)
+        const uint32_t ncells =
+            ntiles ? ((bx1 >> 3) - (bx0 >> 3) + 1u) * ((by1 >> 3) - (by0 >> 3) + 1u) : 0u;
         if (span > HDSR_LARGE_SPLAT_TILES) {
           is_large = 1;
           n_large = ntiles;
+          c_large = ncells;
         } else {
           n_small = ntiles;
+          c_small = ncells;
         }
         depth_bits = hdsr_f2u(depth);
 
@@ -239,27 +245,33 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   // logMin from it (hdsr_log_pos evaluates the same expression)
   b.dlog[g] = make_uint2(depth_bits, hdsr_f2u(hdsr_logf(fmaxf(hdsr_u2f(depth_bits), 1e-6f))));
 
-  // workgroup reduction of the four counts
-  __shared__ uint32_t red[4][4];
+  // workgroup reduction of the six counts
+  __shared__ uint32_t red[4][6];
   const uint32_t s0 = wave_sum(n_small), s1 = wave_sum(n_large), s2 = wave_sum(is_large), s3 = wave_sum(vis);
+  const uint32_t s4 = wave_sum(c_small), s5 = wave_sum(c_large);
   const uint32_t w = threadIdx.x >> 6;
   if (__lane_id() == 0) {
     red[w][0] = s0;
     red[w][1] = s1;
     red[w][2] = s2;
     red[w][3] = s3;
+    red[w][4] = s4;
+    red[w][5] = s5;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint4 t = make_uint4(0, 0, 0, 0);
+    uint4 t = make_uint4(0, 0, 0, 0), c = make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       t.x += red[i][0];
       t.y += red[i][1];
       t.z += red[i][2];
       t.w += red[i][3];
+      c.x += red[i][4];
+      c.y += red[i][5];
     }
     b.block_cnt[blk] = t;
+    b.block_cc[blk] = c;
   }
 }
 
