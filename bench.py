@@ -170,6 +170,29 @@ def main() -> int:
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
     full_size = a.n == 1_000_000 and len(cams) == 8
 
+    # the same frames with the back-to-front order produced on the GPU each frame (the reference's
+    # encodeSort before the compositor) instead of resident host-made orders
+    sort_only = with_sort = None
+    if len(cams) > 0:
+        for v in range(len(cams)):  # warm
+            res.set_frame(uniforms[v])
+            compositor.encodeViewDepthSort(handle, res)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            for v in range(len(cams)):
+                res.set_frame(uniforms[v])
+                compositor.encodeViewDepthSort(handle, res)
+        torch.cuda.synchronize()
+        sort_only = (time.perf_counter() - t0) / frames * 1e3
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            for v in range(len(cams)):
+                res.set_frame(uniforms[v])
+                compositor.encodeViewDepthSort(handle, res)
+                compositor.encodeFullComputeCompositorPipeline(handle, res)
+        torch.cuda.synchronize()
+        with_sort = frames / (time.perf_counter() - t0)
     res.destroy()
     serving = None if a.no_serving else serving_bench()
     diff = None if a.no_diff else diff_bench()
@@ -205,6 +228,9 @@ def main() -> int:
         "vs_cpu": None if not cpu else round(fps / cpu["value"], 1),
         "setup_s": round(setup_s, 1),
         "profiled_renders": renders,
+        "gpu_depth_sort": None if sort_only is None else {
+            "ms_per_view": round(sort_only, 4), "frames_per_s_with_sort": round(with_sort, 2),
+            "note": "sortedIndices produced on the GPU every frame (bitonic network of the reference's encodeSort)"},
         "serving": serving,
         "diff": diff,
     }

@@ -35,6 +35,7 @@ struct hdsr_ctx {
   uint16_t* rgba16 = nullptr;
   float* t_final = nullptr;       // differentiable mode
   uint32_t* last_idx = nullptr;
+  uint2* ds_work = nullptr;       // depth-sort scratch (hdsr_sort_back_to_front), on first use
   hipStream_t last_stream = nullptr;
   bool rendered = false;
   uint32_t reruns = 0;
@@ -168,7 +169,7 @@ void hdsr_destroy(hdsr_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   free_scene(c);
-  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx};
+  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx, c->ds_work};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete c;
@@ -249,6 +250,23 @@ int hdsr_set_frame(hdsr_ctx* c, const void* u) {
   if (w[25] > 3) return fail(c, HDSR_E_INVALID_ARG, "uniform shDegree must be 0..3");
   c->frame = f;
   c->have_frame = true;
+  return HDSR_OK;
+}
+
+int hdsr_sort_back_to_front(hdsr_ctx* c, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  if (!c->have_splats || !c->have_frame) return fail(c, HDSR_E_STATE, "bind splats and set a frame first");
+  const uint32_t n = c->n;
+  if (!n) return HDSR_OK;
+  if (!c->ds_work) HIPX(c, hipMalloc((void**)&c->ds_work, (size_t)depth_sort_padded(n) * sizeof(uint2)));
+  if (!c->own_sorted) {  // a borrowed buffer is not written: switch to an owned one
+    uint32_t* own = nullptr;
+    HIPX(c, hipMalloc((void**)&own, (size_t)n * 4));
+    c->sorted = own;
+    c->own_sorted = true;
+  }
+  launch_depth_sort(c->pos, n, c->frame.view, c->sorted, c->ds_work, static_cast<hipStream_t>(stream));
+  HIPX(c, hipGetLastError());
   return HDSR_OK;
 }
 
@@ -381,6 +399,11 @@ int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {
   if (!c || !bytes) return HDSR_E_INVALID_ARG;
   int rc = hdsr_sync(c);
   if (rc) return rc;
+  if (which == HDSR_DBG_SORTED) {
+    if (!c->have_splats) return fail(c, HDSR_E_STATE, "bind splats first");
+    *bytes = (uint64_t)c->n * 4;
+    return HDSR_OK;
+  }
   if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
   const DevStats& s = c->eng.host_stats();
   const uint64_t M = c->plan.morton_tile_count;
@@ -412,6 +435,11 @@ int hdsr_debug_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
   if (!host && need) return HDSR_E_INVALID_ARG;
   if (bytes < need) return fail(c, HDSR_E_INVALID_ARG, "host buffer too small");
   if (need == 0) return HDSR_OK;
+  if (which == HDSR_DBG_SORTED) {
+    HIPX(c, hipDeviceSynchronize());
+    HIPX(c, hipMemcpy(host, c->sorted, need, hipMemcpyDeviceToHost));
+    return HDSR_OK;
+  }
   const Bufs& b = c->eng.bufs();
   const DevStats& s = c->eng.host_stats();
   switch (which) {

@@ -192,6 +192,19 @@ void launch_backward(const Bufs& b, uint32_t n, const float* grad_rgba, const Di
 // Rebuilds the reference's 14-word projection cache of frame f into dst (debug readback).
 void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream);
 
+// Stable LSD radix sort of (key, value) pairs in place of keys / vals (ping-pong through the tmp
+// buffers), *count pairs (device) of at most cap, keys below 2^key_bits, digits of at most 8 bits.
+// hist needs radix_hist_words(cap) words, hist_tmp scan_u32_tmp_words of that.  Returns true when
+// the sorted pairs end in the tmp buffers (an odd number of passes).
+size_t radix_hist_words(uint32_t cap);
+bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, const uint32_t* count,
+                      uint32_t cap, uint32_t key_bits, uint32_t* hist, uint32_t* hist_tmp, hipStream_t stream);
+// Back-to-front order of n splats for one view (see k_depthsort.hip): out[rank] = splat id.
+// work holds depth_sort_padded(n) (key, index) pairs.
+uint32_t depth_sort_padded(uint32_t n);
+void launch_depth_sort(const float* positions, uint32_t n, const float* view16, uint32_t* out, uint2* work,
+                       hipStream_t stream);
+
 // Generic exclusive scans used by several stages (hierarchical: reduce, scan of partials, downsweep).
 size_t scan_u32_tmp_words(uint32_t n);
 void scan_u32_exclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, hipStream_t stream);

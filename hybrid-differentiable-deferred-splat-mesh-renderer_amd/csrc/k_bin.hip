@@ -641,25 +641,9 @@ static const uint32_t* sorted_cell_keys(const Bufs& b) { return cell_passes(b) &
 static const uint32_t* sorted_cell_vals(const Bufs& b) { return cell_passes(b) & 1u ? b.cvals_b : b.cvals_a; }
 
 void launch_group_pairs(const Bufs& b, hipStream_t stream) {
-  const uint32_t nblocks = (b.cell_cap + kRadixBlockItems - 1) / kRadixBlockItems;
-  if (!nblocks || !b.total_cells) return;
-  const uint32_t passes = cell_passes(b);
-  const uint32_t bits = b.cell_key_bits == 0 ? 1u : (b.cell_key_bits + passes - 1) / passes;
-  const uint32_t mask = (1u << bits) - 1u;
-  const uint32_t* count = &b.stats->cell_pairs;
-  uint32_t* ks = b.ckeys_a;
-  uint32_t* vs = b.cvals_a;
-  for (uint32_t p = 0; p < passes; ++p) {
-    uint32_t* kd = ks == b.ckeys_a ? b.ckeys_b : b.ckeys_a;
-    uint32_t* vd = vs == b.cvals_a ? b.cvals_b : b.cvals_a;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, ks, b.radix_hist, count, nblocks,
-                       p * bits, mask);
-    scan_u32_exclusive(b.radix_hist, b.radix_hist, nblocks * kRadixBuckets, b.radix_tmp, stream);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, stream, ks, vs, kd, vd, b.radix_hist, count,
-                       nblocks, p * bits, bits);
-    ks = kd;
-    vs = vd;
-  }
+  if (!b.cell_cap || !b.total_cells) return;
+  (void)radix_sort_pairs(b.ckeys_a, b.cvals_a, b.ckeys_b, b.cvals_b, &b.stats->cell_pairs, b.cell_cap,
+                         b.cell_key_bits, b.radix_hist, b.radix_tmp, stream);
   // cell ranges and expansion chunks, per-chunk tile counts, per-cell prefixes -> tile counts
   const uint32_t cg = (b.total_cells + 255) / 256;
   hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, sorted_cell_keys(b));
@@ -677,6 +661,29 @@ void launch_tile_ranges(const Bufs& b, hipStream_t stream) {
   hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
                      b.tile_counts, b.total_morton, b.cap);
   if (b.total_cells) hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
+}
+
+size_t radix_hist_words(uint32_t cap) {
+  return (size_t)((cap + kRadixBlockItems - 1) / kRadixBlockItems) * kRadixBuckets;
+}
+
+bool radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, const uint32_t* count,
+                      uint32_t cap, uint32_t key_bits, uint32_t* hist, uint32_t* hist_tmp, hipStream_t stream) {
+  const uint32_t nblocks = (cap + kRadixBlockItems - 1) / kRadixBlockItems;
+  if (!nblocks) return false;
+  const uint32_t passes = key_bits == 0 ? 1u : (key_bits + kRadixBits - 1) / kRadixBits;
+  const uint32_t bits = key_bits == 0 ? 1u : (key_bits + passes - 1) / passes;
+  const uint32_t mask = (1u << bits) - 1u;
+  uint32_t *ks = keys, *vs = vals, *kd = keys_tmp, *vd = vals_tmp;
+  for (uint32_t p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblocks), dim3(256), 0, stream, ks, hist, count, nblocks, p * bits, mask);
+    scan_u32_exclusive(hist, hist, nblocks * kRadixBuckets, hist_tmp, stream);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblocks), dim3(256), 0, stream, ks, vs, kd, vd, hist, count, nblocks,
+                       p * bits, bits);
+    std::swap(ks, kd);
+    std::swap(vs, vd);
+  }
+  return (passes & 1u) != 0;
 }
 
 void launch_classify(const Bufs& b, hipStream_t stream) {

@@ -35,7 +35,7 @@ OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OU
 
 DBG = {
     "projcache": 0, "depth": 1, "tile_counts": 2, "tile_offsets": 3, "entries": 4, "entries_canon": 5,
-    "small_list": 6, "large_list": 7, "large_bases": 8, "chunks": 9, "keys20": 10, "large_splats": 11,
+    "small_list": 6, "large_list": 7, "large_bases": 8, "chunks": 9, "keys20": 10, "large_splats": 11, "sorted": 12,
 }
 
 
@@ -108,6 +108,7 @@ SIGNATURES = [
     ("hdsr_update_sorted_indices", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("hdsr_set_frame", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_render", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hdsr_sort_back_to_front", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_composite_only", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_sync", C.c_int, [C.c_void_p]),
     ("hdsr_get_outputs", C.c_int, [C.c_void_p, C.POINTER(Outputs)]),

@@ -222,6 +222,13 @@ def encodeFullComputeCompositorPipeline(encoder, resources: Resources, bindGroup
     check(resources._lib.hdsr_render(resources.handle, C.c_void_p(h) if h else None), resources.handle)
 
 
+def encodeViewDepthSort(encoder, resources: Resources) -> None:
+    """The GPU sortedIndices producer (createViewDepthSortInput + encodeGpuSortPrototype,
+    src/gpuSortPrototype.ts:120-140, 275-287) for the current frame's view matrix."""
+    h = _stream_handle(encoder)
+    check(resources._lib.hdsr_sort_back_to_front(resources.handle, C.c_void_p(h) if h else None), resources.handle)
+
+
 def encodeCompositeOnly(encoder, resources: Resources, bindGroups: BindGroups | None = None) -> None:
     h = _stream_handle(encoder)
     check(resources._lib.hdsr_composite_only(resources.handle, C.c_void_p(h) if h else None), resources.handle)

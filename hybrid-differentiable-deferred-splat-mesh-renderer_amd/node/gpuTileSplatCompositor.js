@@ -28,7 +28,7 @@ var FLAGS = { f16: 1, refCaps: 2, gbuffer: 4, debug: 8, rgba16: 16, profile: 32 
 var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4 };
 var DBG = {
   projCache: 0, depth: 1, tileCounts: 2, tileOffsets: 3, entries: 4, entriesCanon: 5, smallTileList: 6,
-  largeTileList: 7, largeTileOverflowBases: 8, chunkRanges: 9, keys20: 10, largeSplatList: 11,
+  largeTileList: 7, largeTileOverflowBases: 8, chunkRanges: 9, keys20: 10, largeSplatList: 11, sortedIndices: 12,
 };
 
 function planTileSplatCompositor(input) {
@@ -128,6 +128,12 @@ function encodeFullComputeCompositorPipeline(encoder, resources, bindGroups) {
   addon.render(resources.handle);
 }
 
+// The GPU sortedIndices producer (createViewDepthSortInput + encodeGpuSortPrototype,
+// src/gpuSortPrototype.ts:120-140, 275-287) for the current frame uniforms' view matrix.
+function encodeViewDepthSort(encoder, resources) {
+  addon.sortBackToFront(resources.handle);
+}
+
 function encodeCompositeOnly(encoder, resources, bindGroups) {
   addon.compositeOnly(resources.handle);
 }
@@ -163,6 +169,7 @@ module.exports = {
   createTileSplatBindGroups: createTileSplatBindGroups,
   encodeFullComputeCompositorPipeline: encodeFullComputeCompositorPipeline,
   encodeCompositeOnly: encodeCompositeOnly,
+  encodeViewDepthSort: encodeViewDepthSort,
   createDevice: createDevice,
   version: addon.version,
 };

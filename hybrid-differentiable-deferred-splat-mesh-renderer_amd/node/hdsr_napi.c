@@ -301,6 +301,17 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+static napi_value js_sort_back_to_front(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  const int rc = hdsr_sort_back_to_front(ctx, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_sort_back_to_front");
+  return NULL;
+}
+
 static napi_value js_composite_only(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
@@ -430,6 +441,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_default, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},
       {"compositeOnly", NULL, js_composite_only, NULL, NULL, NULL, napi_default, NULL},
+      {"sortBackToFront", NULL, js_sort_back_to_front, NULL, NULL, NULL, napi_default, NULL},
       {"sync", NULL, js_sync, NULL, NULL, NULL, napi_default, NULL},
       {"readback", NULL, js_readback, NULL, NULL, NULL, napi_default, NULL},
       {"debugReadback", NULL, js_debug, NULL, NULL, NULL, napi_default, NULL},

@@ -109,6 +109,13 @@ int hdsr_update_sorted_indices(hdsr_ctx* ctx, const uint32_t* sorted_indices, ui
  * (gpuTileSplatCompositor.ts:730-774; src/splatRenderer.ts:1030): the 256-byte block verbatim. */
 int hdsr_set_frame(hdsr_ctx* ctx, const void* uniforms256);
 
+/* The sortedIndices producer on the GPU: createViewDepthSortInput + encodeGpuSortPrototype
+ * (src/gpuSortPrototype.ts:120-140, 275-287).  Sorts the bound splats back to front for the
+ * current frame's view matrix (key f32(-view z), view z evaluated in double; descending, ties to the
+ * lower id) and binds the result as the context's own sortedIndices.  Enqueued on 'stream'; a
+ * following hdsr_render on the same stream uses it.  Needs hdsr_bind_splats and hdsr_set_frame. */
+int hdsr_sort_back_to_front(hdsr_ctx* ctx, void* stream);
+
 /* encodeFullComputeCompositorPipeline (gpuTileSplatCompositor.ts:1023-1179): enqueues the whole
  * path on 'stream' and returns.  Outputs are complete once the stream reaches this point, unless
  * the pair capacity overflowed; hdsr_sync() detects that, grows the capacity and re-renders. */
@@ -162,6 +169,7 @@ int hdsr_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
 #define HDSR_DBG_CHUNKS 9         /* (start, count) u32 pairs */
 #define HDSR_DBG_KEYS20 10        /* R u32 (q20 << 12) | local index (HDSR_DEBUG) */
 #define HDSR_DBG_LARGE_SPLATS 11  /* sort ranks of splats covering > 16 tiles */
+#define HDSR_DBG_SORTED 12        /* N u32 the bound sortedIndices (e.g. after hdsr_sort_back_to_front) */
 int hdsr_debug_size(hdsr_ctx* ctx, int which, uint64_t* bytes);
 int hdsr_debug_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
 

@@ -150,3 +150,80 @@ def time_frame(scene, sorted_indices, uniforms, width, height, row_step: int) ->
     proj, binsort, comp = t[0], t[1], t[2]
     return {"project_s": proj, "bin_sort_s": binsort, "composite_sample_s": comp, "row_step": row_step,
             "frame_s_est": proj + binsort + comp * row_step, "measured_s": proj + binsort + comp}
+
+
+# ---- the GPU sortedIndices producer (f32 keys) ----------------------------------------------
+
+def view_depth_keys_f32(positions, view16) -> np.ndarray:
+    """createViewDepthSortInput (src/gpuSortPrototype.ts:120-140, 308-320): key of splat i is
+    -(view-space z), z = ((V[2] x + V[6] y) + V[10] z) + V[14] in float64, stored as float32."""
+    p = np.asarray(positions, dtype=np.float32).reshape(-1, 3).astype(np.float64)
+    v = np.asarray(view16, dtype=np.float32).astype(np.float64)
+    z = ((v[2] * p[:, 0] + v[6] * p[:, 1]) + v[10] * p[:, 2]) + v[14]
+    return (-z).astype(np.float32)
+
+
+def gpu_sort_order(keys) -> np.ndarray:
+    """The order of the bitonic network of gpuSortPrototype.ts:4-55 in descending mode (the
+    renderer's sortedIndices), every stage vectorised over its pairs: keys padded to a power of
+    two with (-inf, 0xFFFFFFFF); stage (k, j) compare-exchanges (i, i + j) for i with bit j clear,
+    ascending when i & k != 0, with should_swap's tie rule (smaller index first in both directions,
+    so exact key ties come out in a network-dependent order).  Returns the first n indices."""
+    keys = np.asarray(keys, dtype=np.float32)
+    n = len(keys)
+    P = 1
+    while P < max(1, n):
+        P *= 2
+    k = np.full(P, -np.inf, np.float32)
+    idx = np.full(P, 0xFFFFFFFF, np.uint64)
+    k[:n] = keys
+    idx[:n] = np.arange(n)
+    t = np.arange(P // 2, dtype=np.int64)
+    kk = 2
+    while kk <= P:
+        j = kk // 2
+        while j >= 1:
+            i = (t // j) * 2 * j + t % j
+            q = i + j
+            lk, rk, li, ri = k[i], k[q], idx[i], idx[q]
+            asc = (i & kk) != 0
+            tie = (lk == rk) & (li > ri)
+            swap = np.where(asc, (lk > rk) | tie, (lk < rk) | tie)
+            k[i], k[q] = np.where(swap, rk, lk), np.where(swap, lk, rk)
+            idx[i], idx[q] = np.where(swap, ri, li), np.where(swap, li, ri)
+            j //= 2
+        kk *= 2
+    return idx[:n].astype(np.uint32)
+
+
+def bitonic_sort_restated(keys) -> np.ndarray:
+    """simulateGpuSortPrototype (src/gpuSortPrototype.ts:142-163) pair by pair in the loop order of
+    the reference, for small inputs (pins the vectorised gpu_sort_order)."""
+    n = len(keys)
+    padded = 1
+    while padded < max(1, n):
+        padded *= 2
+    k = np.full(padded, -np.inf, np.float32)
+    idx = np.full(padded, 0xFFFFFFFF, np.uint64)
+    k[:n] = keys
+    idx[:n] = np.arange(n)
+    kk = 2
+    while kk <= padded:
+        j = kk // 2
+        while j >= 1:
+            for i in range(padded):
+                partner = i ^ j
+                if partner <= i or partner >= padded:
+                    continue
+                ascending = not ((i & kk) == 0)
+                lk, rk, li, ri = k[i], k[partner], idx[i], idx[partner]
+                if ascending:
+                    swap = lk > rk or (lk == rk and li > ri)
+                else:
+                    swap = lk < rk or (lk == rk and li > ri)
+                if swap:
+                    k[i], k[partner] = rk, lk
+                    idx[i], idx[partner] = ri, li
+            j //= 2
+        kk *= 2
+    return idx[idx != 0xFFFFFFFF].astype(np.uint32)

@@ -132,3 +132,30 @@ def test_ply_f_rest_to_coefficient_major_kat():
     f_rest = np.array([[10, 11, 12, 20, 21, 22, 30, 31, 32]], np.float32)
     sh = scenes.sh_from_ply_f_rest(f_rest)
     np.testing.assert_array_equal(sh, np.array([[[10, 20, 30], [11, 21, 31], [12, 22, 32]]], np.float32))
+
+
+def test_gpu_sort_order_restatement():
+    # the vectorised network equals the reference's loop (restated pair by pair) on inputs with
+    # ties, signed zeros, infinities and non-power-of-two sizes
+    from oracle import oracle as orc
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 3, 7, 16, 33, 100):
+        for keys in (rng.choice(np.array([-2.5, -1, -0.0, 0.0, 0.5, 3, np.inf, -np.inf], np.float32), n),
+                     rng.normal(0, 1, n).astype(np.float32)):
+            assert orc.gpu_sort_order(keys).tolist() == orc.bitonic_sort_restated(keys).tolist()
+    # distinct keys: a plain descending sort; the cpuDepthSort known answers hold
+    keys = rng.permutation(1000).astype(np.float32)
+    assert orc.gpu_sort_order(keys).tolist() == np.argsort(-keys, kind="stable").tolist()
+    for pos, want in [(np.array([[0, 0, -2], [0, 0, -9], [0, 0, -5], [0, 0, -0.5]], np.float32), [1, 2, 0, 3]),
+                      (np.array([[0, 0, -4], [0, 0, -6], [1, 0, -4], [2, 0, -4]], np.float32), [1, 0, 2, 3])]:
+        assert orc.gpu_sort_order(orc.view_depth_keys_f32(pos, IDENT)).tolist() == want
+    # the network's tie order is not "ties to the lower index": equal keys can come out of order
+    keys = np.array([np.inf, 3, -np.inf, -2.5, -2.5, np.inf, 0, 3, 0, -np.inf, -2.5, 3, -2.5, -1, np.inf, -0.0],
+                    np.float32)
+    assert orc.gpu_sort_order(keys).tolist()[6:9] == [6, 15, 8]
+    # against the float64 CPU sort: identical except around f32 ties
+    sc = scenes.scene_config0(20000, seed=4)
+    cam = scenes.cameras_config0()[0]
+    a = orc.gpu_sort_order(orc.view_depth_keys_f32(sc.positions, cam.view))
+    b = scenes.sort_back_to_front(sc.positions, cam.view)
+    assert np.mean(a != b) < 0.01
