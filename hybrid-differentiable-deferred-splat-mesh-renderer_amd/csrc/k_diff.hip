@@ -41,10 +41,49 @@ __global__ void k_fill_zero(float* p, size_t n) {
     p[i] = 0.0f;
 }
 
-__device__ __forceinline__ float wave_sum_f(float v) {
+// Sums each of the 9 slots over the wave's 64 lanes with 13 cross-lane adds instead of 9 x 6: at
+// each butterfly step a lane keeps half of its remaining slots and sends the other half to its
+// partner (9 -> 5 -> 3 -> 2 -> 1 slots over the lane bits 5, 4, 3, 2; bits 1, 0 finish the last
+// one).  Returns the wave total of slot_of_lane(lane) (see below) on every lane.
+__device__ __forceinline__ float wave_sum9(const float (&v)[kSlots], uint32_t lane) {
+  const bool b5 = lane & 32u, b4 = lane & 16u, b3 = lane & 8u, b2 = lane & 4u;
+  float a[5];
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
+  for (int k = 0; k < 5; ++k) {  // keep 0..4 (bit 5 clear) or 5..8 (set)
+    const float send = b5 ? v[k] : (k < 4 ? v[5 + k] : 0.0f);
+    const float recv = __shfl_xor(send, 32, 64);
+    a[k] = b5 ? (k < 4 ? v[5 + k] + recv : 0.0f) : v[k] + recv;
+  }
+  float c[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {  // keep a0..a2 or a3, a4
+    const float send = b4 ? a[k] : (k < 2 ? a[3 + k] : 0.0f);
+    const float recv = __shfl_xor(send, 16, 64);
+    c[k] = b4 ? (k < 2 ? a[3 + k] + recv : 0.0f) : a[k] + recv;
+  }
+  float d[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {  // keep c0, c1 or c2
+    const float send = b3 ? c[k] : (k < 1 ? c[2 + k] : 0.0f);
+    const float recv = __shfl_xor(send, 8, 64);
+    d[k] = b3 ? (k < 1 ? c[2 + k] + recv : 0.0f) : c[k] + recv;
+  }
+  float e = (b2 ? d[1] : d[0]) + __shfl_xor(b2 ? d[0] : d[1], 4, 64);
+  e += __shfl_xor(e, 2, 64);
+  e += __shfl_xor(e, 1, 64);
+  return e;
+}
+
+// The slot whose total wave_sum9 leaves on 'lane', or -1 (a padding position).
+__device__ __forceinline__ int slot_of_lane(uint32_t lane) {
+  const int d = (lane & 4u) ? 1 : 0;
+  int c = d;
+  if (lane & 8u) c = d ? -1 : 2;
+  int a = c;
+  if (lane & 16u) a = (c < 0 || c > 1) ? -1 : 3 + c;
+  if (a < 0) return -1;
+  if (lane & 32u) return a > 3 ? -1 : 5 + a;
+  return a;
 }
 
 __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restrict__ grad) {
@@ -71,8 +110,9 @@ __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restr
   const float Tf = inside ? F.t_final[o] : 1.0f;
   const float4 gin = inside ? reinterpret_cast<const float4*>(grad)[o] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 
-  const uint32_t wm = wave_max(my_last);
+  const uint32_t wm = wave_max(my_last);  // no entry at or past this index blends in this wave
   if (lane == 0) s_top[wave] = wm;
+  const int my_slot = (lane & 3u) == 0u ? slot_of_lane(lane) : -1;
   __syncthreads();
   const uint32_t top = max(max(s_top[0], s_top[1]), max(s_top[2], s_top[3]));
 
@@ -118,8 +158,9 @@ __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restr
       float gv[kSlots];
 #pragma unroll
       for (int k = 0; k < kSlots; ++k) gv[k] = 0.0f;
+      float tot = 0.0f;
       const float4 C = s_c[j];
-      if (!(C.z > band_y1 || C.w < band_y0)) {  // uniform per wave
+      if (s_idx[j] < wm && !(C.z > band_y1 || C.w < band_y0)) {  // uniform per wave
         const float4 A = s_a[j];
         const float4 B = s_b[j];
         float dx = p00x - A.x, dy = p00y - A.y;
@@ -158,14 +199,10 @@ __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restr
               gv[5] = dA * gauss;
             }
           }
-#pragma unroll
-          for (int k = 0; k < kSlots; ++k) gv[k] = wave_sum_f(gv[k]);
+          tot = wave_sum9(gv, lane);
         }
       }
-      if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < kSlots; ++k) s_g[wave][k][j] = gv[k];
-      }
+      if (my_slot >= 0) s_g[wave][my_slot][j] = tot;
     }
     __syncthreads();
     if (tid < nb) {
