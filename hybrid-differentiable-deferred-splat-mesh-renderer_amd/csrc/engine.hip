@@ -82,7 +82,7 @@ void Engine::release() {
   void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre,
                   b_.block_cc, b_.block_ccpre, b_.scan_tmp, b_.large_list, b_.ckeys_a, b_.cvals_a, b_.ckeys_b,
                   b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
-                  b_.scratch, b_.ekey, b_.ekey_s,
+                  b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
   for (void* p : ptrs)
@@ -199,9 +199,9 @@ int Engine::ensure(const Layout& L, bool debug) {
     have = n_cap_;
     if ((rc = grow(&b_.scratch, &have, cap))) return rc;
     have = n_cap_;
-    if ((rc = grow(&b_.ekey, &have, cap))) return rc;
+    if ((rc = grow(&b_.qkey, &have, cap))) return rc;
     have = n_cap_;
-    if ((rc = grow(&b_.ekey_s, &have, cap))) return rc;
+    if ((rc = grow(&b_.qkey_s, &have, cap))) return rc;
     n_cap_ = cap;
     n_debug_ = 0;  // debug arrays follow the capacity
   }
@@ -225,8 +225,11 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.tile_offsets, &have, L.total_morton))) return rc;
     have = n_morton_;
     if ((rc = grow(&b_.tile_counts, &have, L.total_morton))) return rc;
+    have = n_morton_ * 2;
+    if ((rc = grow(&b_.tile_dmin, &have, (size_t)L.total_morton * 2))) return rc;
     n_morton_ = L.total_morton;
   }
+  b_.tile_dmax = b_.tile_dmin + n_morton_;
   if (L.total_tiles > n_tiles_ || !b_.small_list) {
     have = n_tiles_;
     if ((rc = grow(&b_.small_list, &have, L.total_tiles))) return rc;

@@ -107,7 +107,7 @@ struct LargeTile {
   uint32_t start;    // first entry in entries[]
   uint32_t count;
   uint32_t scratch;  // first slot in scratch[]
-  uint32_t dmin, dmax, pad0, pad1;
+  uint32_t pad0, pad1, pad2, pad3;
 };
 
 // Device buffers of one batch (all owned by the host context).
@@ -151,8 +151,10 @@ struct Bufs {
   uint32_t* keys20;          // [cap] (q20 << 12) | local index, or null
   uint32_t* entries_canon;   // [cap] entries before the per-tile sort, or null
   uint32_t* scratch;         // [cap] bucket-ordered copies of large tiles
-  uint2* ekey;               // [cap] dlog[] of each grouped entry (same order as entries)
-  uint2* ekey_s;             // [cap] dlog[] of each scratch entry
+  uint32_t* qkey;            // [cap] q20 key of each entry in its tile's log-depth frame (same order as entries)
+  uint32_t* qkey_s;          // [cap] q20 key of each scratch entry
+  uint32_t* tile_dmin;       // [total_morton] min / max depth bits over each tile's entries (u32 order,
+  uint32_t* tile_dmax;       //   as the reference's atomicMin / atomicMax)
   uint32_t cap;              // pair capacity
   uint32_t* radix_hist;      // [256 * radix blocks] digit-major (radix sort of the cell pairs)
   uint32_t* radix_tmp;       // scratch of the u32 scans (radix histogram, cell chunks, tile counts)

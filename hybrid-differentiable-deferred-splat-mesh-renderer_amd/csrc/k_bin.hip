@@ -458,15 +458,59 @@ __device__ __forceinline__ uint32_t wave_tile_counts(const Bufs& b, const uint32
   return my;
 }
 
+// Counting walk; also folds each entry's depth bits into its covered tiles' min / max (LDS, then
+// one global atomic pair per tile and chunk): the depth range that sets every tile's log frame.
 __global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) {
   __shared__ uint32_t wcnt[4][64];
+  __shared__ uint32_t s_dmin[64], s_dmax[64];
   const uint32_t q = blockIdx.x;
   if (q >= b.stats->cell_chunks) return;  // uniform
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
-  wcnt[w][lane] = wave_tile_counts(b, cv, ci, w, lane);
+  if (t < 64) {
+    s_dmin[t] = 0xFFFFFFFFu;
+    s_dmax[t] = 0u;
+  }
   __syncthreads();
-  if (t < 64) b.chunk_cnt[(size_t)q * 64 + t] = (wcnt[0][t] + wcnt[1][t]) + (wcnt[2][t] + wcnt[3][t]);
+  uint32_t my = 0;
+#pragma unroll 1
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t i = w * 256 + r * 64 + lane;
+    unsigned long long mk = 0ull;
+    uint32_t d = 0;
+    if (i < ci.count) {
+      const uint32_t g = cv[ci.first + i];
+      mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
+      d = b.depth[g];
+    }
+#pragma unroll
+    for (uint32_t tt = 0; tt < 64; ++tt) {
+      const uint32_t pc = (uint32_t)__popcll(__ballot((mk >> tt) & 1ull));
+      my += lane == tt ? pc : 0u;
+    }
+    // each lane walks its tiles from a lane-dependent rotation, so that lanes with similar boxes
+    // hit different LDS words in the same step
+    const uint32_t rot = (lane * 11u) & 63u;
+    const unsigned long long rm = rot ? (mk >> rot) | (mk << (64u - rot)) : mk;
+    for (unsigned long long mm = rm; mm; mm &= mm - 1ull) {
+      const uint32_t tt = ((uint32_t)__ffsll((long long)mm) - 1u + rot) & 63u;
+      atomicMin(&s_dmin[tt], d);
+      atomicMax(&s_dmax[tt], d);
+    }
+  }
+  wcnt[w][lane] = my;
+  __syncthreads();
+  if (t < 64) {
+    const uint32_t c = (wcnt[0][t] + wcnt[1][t]) + (wcnt[2][t] + wcnt[3][t]);
+    b.chunk_cnt[(size_t)q * 64 + t] = c;
+    const FrameDesc& F = *ci.F;
+    const uint32_t tx = ci.cx * kCellTiles + (t & 7u), ty = ci.cy * kCellTiles + (t >> 3);
+    if (c && tx < F.tiles_x && ty < F.tiles_y) {
+      const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
+      atomicMin(&b.tile_dmin[m], s_dmin[t]);
+      atomicMax(&b.tile_dmax[m], s_dmax[t]);
+    }
+  }
 }
 
 // One 64-thread workgroup per cell: exclusive prefix of each tile's counts over the cell's chunks
@@ -509,25 +553,28 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   const unsigned long long below = (1ull << lane) - 1ull;
   wcnt[w][lane] = wave_tile_counts(b, cv, ci, w, lane);
   __syncthreads();
-  // lane t: where this wave's entries of tile t start
+  // lane t: where this wave's entries of tile t start, and tile t's log-depth frame
   const FrameDesc& F = *ci.F;
   const uint32_t tx = ci.cx * kCellTiles + (lane & 7u), ty = ci.cy * kCellTiles + (lane >> 3);
   uint32_t base = 0;
+  float lmin = 0.0f, scale = 0.0f;
   if (tx < F.tiles_x && ty < F.tiles_y) {
-    base = b.tile_offsets[F.morton_base + hdsr_morton2(tx, ty)] + b.chunk_cnt[(size_t)q * 64 + lane];
+    const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
+    base = b.tile_offsets[m] + b.chunk_cnt[(size_t)q * 64 + lane];
     for (uint32_t v = 0; v < w; ++v) base += wcnt[v][lane];
+    hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lmin, &scale);
   }
   const uint32_t cap = b.cap;
 #pragma unroll 1
   for (uint32_t r = 0; r < 4; ++r) {
     const uint32_t i = w * 256 + r * 64 + lane;
     uint32_t g = 0;
-    uint2 ek = make_uint2(0, 0);
+    float ln = 0.0f;  // ln(max(depth, 1e-6)) of the entry (computed once per splat by the projection)
     unsigned long long mk = 0ull;
     if (i < ci.count) {
       g = cv[ci.first + i];
       mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
-      ek = b.dlog[g];
+      ln = hdsr_u2f(b.dlog[g].y);
     }
 #pragma unroll
     for (uint32_t tt = 0; tt < 64; ++tt) {
@@ -535,11 +582,12 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
       const unsigned long long m = __ballot(in);
       if (m) {  // uniform
         const uint32_t bt = __shfl(base, tt, 64);
+        const float lm = __shfl(lmin, tt, 64), sc = __shfl(scale, tt, 64);
         if (in) {
           const uint32_t dst = bt + (uint32_t)__popcll(m & below);
           if (dst < cap) {
             b.entries[dst] = g;
-            b.ekey[dst] = ek;
+            b.qkey[dst] = hdsr_q20_of((ln - lm) * sc);  // the entry's key in tile tt's frame
           }
         }
         base += lane == tt ? (uint32_t)__popcll(m) : 0u;
@@ -649,9 +697,11 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
   hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, sorted_cell_keys(b));
   scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
   hipLaunchKernelGGL(k_chunk_total, dim3(1), dim3(1), 0, stream, b);
+  const dim3 fg(std::min<uint32_t>((b.total_morton + 255) / 256, 1024u));
+  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_counts, b.total_morton, 0u);
+  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_dmin, b.total_morton, 0xFFFFFFFFu);
+  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_dmax, b.total_morton, 0u);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>((b.total_morton + 255) / 256, 1024u)), dim3(256), 0, stream,
-                     b.tile_counts, b.total_morton, 0u);
   hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
 }
 

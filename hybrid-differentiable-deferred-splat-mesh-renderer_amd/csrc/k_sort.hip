@@ -16,15 +16,16 @@
 // network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
-// Large tiles: one 1024-thread workgroup finds the depth range and writes the tile's entries in
-// stable bucket order to scratch (ranks by ballots, per-bucket prefix across waves).  The
-// bucket-ordered list is then cut into consecutive 4096-entry chunks (the reference's greedy
-// packing splits buckets exactly there) and each chunk is sorted like a small tile, reading
-// scratch and writing the tile's range of the entry list.
+// Large tiles: one 1024-thread workgroup writes the tile's entries in stable bucket order to
+// scratch (ranks by ballots, per-bucket prefix across waves).  The bucket-ordered list is then cut
+// into consecutive 4096-entry chunks (the reference's greedy packing splits buckets exactly there)
+// and each chunk is sorted like a small tile, reading scratch and writing the tile's range of the
+// entry list.
 //
-// ln(max(depth, 1e-6)) is computed once per splat by the projection (dlog[], bit-identical to
-// evaluating it here) and gathered once, in grouped order, by the last radix pass (ekey[]), so
-// these kernels read depths as coalesced streams.
+// Every entry's q20 arrives with it (qkey[]): the tile-list expansion evaluates it from the tile's
+// depth range (min / max folded during the counting walk) and ln(max(depth, 1e-6)) (computed once
+// per splat by the projection, dlog[]), so these kernels read one 4-byte key stream and do no log
+// arithmetic.
 #include "hdsr_internal.h"
 
 namespace hdsr {
@@ -32,18 +33,13 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-__device__ __forceinline__ float log_pos(uint32_t ldepth_bits, float log_min, float scale) {
-  return (hdsr_u2f(ldepth_bits) - log_min) * scale;
-}
-
 // Large tiles.  Wave w of the 1024-thread workgroup owns the contiguous slice [w*S, (w+1)*S)
-// of the tile's entries.  Pass 1: depth min/max.  Pass 2: per-wave bucket counts (ranks inside a
-// round from seven ballots, a wave-private running count across rounds).  Then one scan over
-// (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 3 walks the slice in
-// the same order and writes each entry at slot + its running rank: a stable bucket order with
-// three workgroup barriers per tile.
+// of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the entry's key;
+// ranks inside a round from seven ballots, a wave-private running count across rounds).  Then one
+// scan over (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 2 walks the
+// slice in the same order and writes each entry (and its key) at slot + its running rank: a
+// stable bucket order with two workgroup barriers per tile.
 __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
-  __shared__ uint32_t red_min[16], red_max[16];
   __shared__ uint32_t wc[16][HDSR_NUM_BUCKETS];
   __shared__ uint32_t wsum[2];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
@@ -52,40 +48,19 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   for (uint32_t l = blockIdx.x; l < nl; l += gridDim.x) {
     const LargeTile L = b.large_tiles[l];
     const uint32_t* src = b.entries + L.start;
-    const uint2* esrc = b.ekey + L.start;
+    const uint32_t* qsrc = b.qkey + L.start;
     const uint32_t S = ((L.count + 1023u) / 1024u) * 64u;
     const uint32_t lo = w * S, hi = min(lo + S, L.count);
-    uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
-    for (uint32_t i = lo + lane; i < hi; i += 64) {
-      const uint32_t d = esrc[i].x;
-      dmn = min(dmn, d);
-      dmx = max(dmx, d);
-    }
-    dmn = wave_min(dmn);
-    dmx = wave_max(dmx);
-    if (lane == 0) {
-      red_min[w] = dmn;
-      red_max[w] = dmx;
-    }
     wc[t >> 7][t & 127] = 0;
     wc[8 + (t >> 7)][t & 127] = 0;
     __syncthreads();
-    dmn = red_min[0];
-    dmx = red_max[0];
-#pragma unroll
-    for (uint32_t q = 1; q < 16; ++q) {
-      dmn = min(dmn, red_min[q]);
-      dmx = max(dmx, red_max[q]);
-    }
-    float lmin, scale;
-    hdsr_log_frame(dmn, dmx, &lmin, &scale);
-    // pass 2: wave-private bucket counts, four rounds of loads in flight
+    // pass 1: wave-private bucket counts, four rounds of loads in flight
     for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
       uint32_t kk[4];
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         const uint32_t i = r0 + u * 64 + lane;
-        kk[u] = i < hi ? hdsr_bucket_of(log_pos(esrc[i].y, lmin, scale)) : 0u;
+        kk[u] = i < hi ? qsrc[i] >> 13 : 0u;
       }
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
@@ -127,20 +102,19 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
       }
     }
     __syncthreads();
-    // pass 3: stable scatter in bucket order (entries and their depth pairs)
+    // pass 2: stable scatter in bucket order (entries and their keys)
     for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
-      uint32_t gg[4];
-      uint2 ee[4];
+      uint32_t gg[4], qq[4];
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         const uint32_t i = r0 + u * 64 + lane;
         gg[u] = i < hi ? src[i] : 0u;
-        ee[u] = i < hi ? esrc[i] : make_uint2(0, 0);
+        qq[u] = i < hi ? qsrc[i] : 0u;
       }
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {
         const bool valid = r0 + u * 64 + lane < hi;
-        const uint32_t k = valid ? hdsr_bucket_of(log_pos(ee[u].y, lmin, scale)) : 0u;
+        const uint32_t k = qq[u] >> 13;
         unsigned long long m = __ballot(valid);
 #pragma unroll
         for (uint32_t bit = 0; bit < 7; ++bit) {
@@ -156,13 +130,9 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
         __builtin_amdgcn_wave_barrier();
         if (valid) {
           b.scratch[L.scratch + old + rank] = gg[u];
-          b.ekey_s[L.scratch + old + rank] = ee[u];
+          b.qkey_s[L.scratch + old + rank] = qq[u];
         }
       }
-    }
-    if (t == 0) {
-      b.large_tiles[l].dmin = dmn;
-      b.large_tiles[l].dmax = dmx;
     }
     __syncthreads();
   }
@@ -264,56 +234,23 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
   __shared__ uint32_t kb[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t wc[kSortWaves][256];
   __shared__ uint32_t wsum[4];
-  __shared__ uint32_t red[4][kSortWaves];
+  __shared__ uint32_t red[2][kSortWaves];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const uint32_t nseg = b.stats->segments;
   for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
     const Seg S = b.segs[s];
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
-    const uint2* esrc = (S.large == kNone ? b.ekey : b.ekey_s) + S.src;
+    const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
     const uint32_t count = S.count;
-    uint2 dl[kSortPer];
     uint32_t vv[kSortPer];
-    uint32_t dmn = 0xFFFFFFFFu, dmx = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      dl[r] = make_uint2(0, 0);
-      vv[r] = 0;
-      if (i < count) {
-        dl[r] = esrc[i];
-        vv[r] = src[i];
-        dmn = min(dmn, dl[r].x);
-        dmx = max(dmx, dl[r].x);
-      }
-    }
-    if (S.large == kNone) {
-      dmn = wave_min(dmn);
-      dmx = wave_max(dmx);
-      if (lane == 0) {
-        red[0][w] = dmn;
-        red[1][w] = dmx;
-      }
-      __syncthreads();
-      dmn = red[0][0];
-      dmx = red[1][0];
-#pragma unroll
-      for (uint32_t q = 1; q < kSortWaves; ++q) {
-        dmn = min(dmn, red[0][q]);
-        dmx = max(dmx, red[1][q]);
-      }
-    } else {
-      dmn = b.large_tiles[S.large].dmin;
-      dmx = b.large_tiles[S.large].dmax;
-    }
-    float lmin, scale;
-    hdsr_log_frame(dmn, dmx, &lmin, &scale);
     uint32_t kand = 0xFFFFFFFFu, kor = 0;
 #pragma unroll
     for (uint32_t r = 0; r < kSortPer; ++r) {
       const uint32_t i = r * kSortThreads + t;
+      vv[r] = 0;
       if (i < count) {
-        const uint32_t key = (hdsr_q20_of(log_pos(dl[r].y, lmin, scale)) << 12) | i;
+        vv[r] = src[i];
+        const uint32_t key = (qsrc[i] << 12) | i;
         ka[i] = key;
         kand &= key;
         kor |= key;
@@ -322,16 +259,16 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     kand = wave_and(kand);
     kor = wave_or(kor);
     if (lane == 0) {
-      red[2][w] = kand;
-      red[3][w] = kor;
+      red[0][w] = kand;
+      red[1][w] = kor;
     }
     __syncthreads();
-    kand = red[2][0];
-    kor = red[3][0];
+    kand = red[0][0];
+    kor = red[1][0];
 #pragma unroll
     for (uint32_t q = 1; q < kSortWaves; ++q) {
-      kand &= red[2][q];
-      kor |= red[3][q];
+      kand &= red[0][q];
+      kor |= red[1][q];
     }
     const uint32_t differ = kand ^ kor;  // bits that are not the same in every key
     const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
