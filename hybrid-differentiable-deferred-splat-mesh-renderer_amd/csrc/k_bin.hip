@@ -440,24 +440,6 @@ __device__ __forceinline__ unsigned long long cover_mask(uint32_t bd, uint32_t c
   return m;
 }
 
-// Per-wave counts of the cell's 64 tiles over this wave's entries of the chunk: lane t returns
-// tile t's count.
-__device__ __forceinline__ uint32_t wave_tile_counts(const Bufs& b, const uint32_t* cv, const ChunkInfo& ci,
-                                                     uint32_t w, uint32_t lane) {
-  uint32_t my = 0;
-#pragma unroll 1
-  for (uint32_t r = 0; r < 4; ++r) {
-    const uint32_t i = w * 256 + r * 64 + lane;
-    const unsigned long long mk = i < ci.count ? cover_mask(b.bounds[cv[ci.first + i]], ci.cx, ci.cy) : 0ull;
-#pragma unroll
-    for (uint32_t t = 0; t < 64; ++t) {
-      const uint32_t pc = (uint32_t)__popcll(__ballot((mk >> t) & 1ull));
-      my += lane == t ? pc : 0u;
-    }
-  }
-  return my;
-}
-
 // Counting walk; also folds each entry's depth bits into its covered tiles' min / max (LDS, then
 // one global atomic pair per tile and chunk): the depth range that sets every tile's log frame.
 __global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) {
@@ -544,54 +526,58 @@ __global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, 
   counts[i] = ce - cs;
 }
 
+// Writing walk, tile-major: the chunk's entries (splat, tile mask, ln depth) are staged in LDS;
+// wave w then takes the cell's tiles 16w .. 16w+15 one at a time and walks all entries of the
+// chunk in order, so each tile's run of this chunk is written as one contiguous range (whole
+// cache lines instead of a few entries per store).
 __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) {
-  __shared__ uint32_t wcnt[4][64];
+  __shared__ unsigned long long s_mask[kCellChunk];
+  __shared__ uint32_t s_g[kCellChunk];
+  __shared__ float s_ln[kCellChunk];
   const uint32_t q = blockIdx.x;
   if (q >= b.stats->cell_chunks) return;  // uniform
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  wcnt[w][lane] = wave_tile_counts(b, cv, ci, w, lane);
-  __syncthreads();
-  // lane t: where this wave's entries of tile t start, and tile t's log-depth frame
-  const FrameDesc& F = *ci.F;
-  const uint32_t tx = ci.cx * kCellTiles + (lane & 7u), ty = ci.cy * kCellTiles + (lane >> 3);
-  uint32_t base = 0;
-  float lmin = 0.0f, scale = 0.0f;
-  if (tx < F.tiles_x && ty < F.tiles_y) {
-    const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
-    base = b.tile_offsets[m] + b.chunk_cnt[(size_t)q * 64 + lane];
-    for (uint32_t v = 0; v < w; ++v) base += wcnt[v][lane];
-    hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lmin, &scale);
-  }
-  const uint32_t cap = b.cap;
-#pragma unroll 1
-  for (uint32_t r = 0; r < 4; ++r) {
-    const uint32_t i = w * 256 + r * 64 + lane;
-    uint32_t g = 0;
-    float ln = 0.0f;  // ln(max(depth, 1e-6)) of the entry (computed once per splat by the projection)
+#pragma unroll
+  for (uint32_t r = 0; r < kCellChunk / 256; ++r) {
+    const uint32_t i = r * 256 + t;
     unsigned long long mk = 0ull;
+    uint32_t g = 0;
+    float ln = 0.0f;
     if (i < ci.count) {
       g = cv[ci.first + i];
       mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
-      ln = hdsr_u2f(b.dlog[g].y);
+      ln = hdsr_u2f(b.dlog[g].y);  // ln(max(depth, 1e-6)), computed once per splat by the projection
     }
-#pragma unroll
-    for (uint32_t tt = 0; tt < 64; ++tt) {
-      const bool in = (mk >> tt) & 1ull;
-      const unsigned long long m = __ballot(in);
-      if (m) {  // uniform
-        const uint32_t bt = __shfl(base, tt, 64);
-        const float lm = __shfl(lmin, tt, 64), sc = __shfl(scale, tt, 64);
-        if (in) {
-          const uint32_t dst = bt + (uint32_t)__popcll(m & below);
-          if (dst < cap) {
-            b.entries[dst] = g;
-            b.qkey[dst] = hdsr_q20_of((ln - lm) * sc);  // the entry's key in tile tt's frame
-          }
+    s_mask[i] = mk;
+    s_g[i] = g;
+    s_ln[i] = ln;
+  }
+  __syncthreads();
+  const FrameDesc& F = *ci.F;
+  const uint32_t cap = b.cap;
+  const uint32_t ngroups = (ci.count + 63) / 64;
+  for (uint32_t k = 0; k < 16; ++k) {
+    const uint32_t tt = w * 16 + k;  // this wave's tile (uniform)
+    const uint32_t tx = ci.cx * kCellTiles + (tt & 7u), ty = ci.cy * kCellTiles + (tt >> 3);
+    if (tx >= F.tiles_x || ty >= F.tiles_y) continue;
+    const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
+    uint32_t base = b.tile_offsets[m] + b.chunk_cnt[(size_t)q * 64 + tt];
+    float lm, sc;
+    hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lm, &sc);
+    for (uint32_t gi = 0; gi < ngroups; ++gi) {
+      const uint32_t i = gi * 64 + lane;
+      const bool in = (s_mask[i] >> tt) & 1ull;  // entries past the count have an empty mask
+      const unsigned long long mm = __ballot(in);
+      if (in) {
+        const uint32_t dst = base + (uint32_t)__popcll(mm & below);
+        if (dst < cap) {
+          b.entries[dst] = s_g[i];
+          b.qkey[dst] = hdsr_q20_of((s_ln[i] - lm) * sc);  // the entry's key in this tile's frame
         }
-        base += lane == tt ? (uint32_t)__popcll(m) : 0u;
       }
+      base += (uint32_t)__popcll(mm);
     }
   }
 }

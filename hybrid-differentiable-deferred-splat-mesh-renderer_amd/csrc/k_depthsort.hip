@@ -12,16 +12,16 @@
 // keys in both directions.  That tie rule is not the reverse of itself between the network's
 // ascending and descending runs, so for exactly equal keys the output order depends on the
 // network (it is not "ties to the lower index"); the same network is therefore run here, stage
-// for stage: all stages with stride j < 4096 of a 4096-element block in LDS, longer strides as one
-// global pass each.  For P = 2^20 that is 9 LDS kernels and 36 global passes (the reference issues
-// 210 dispatches).
+// for stage: all stages with stride j < 4096 of a 4096-element block in LDS, longer strides two per
+// global pass.  For P = 2^20 that is 9 LDS kernels and 20 global passes (the reference issues 210
+// dispatches).
 #include "hdsr_internal.h"
 
 namespace hdsr {
 namespace {
 
 constexpr uint32_t kBitonicBlock = 4096;
-constexpr uint32_t kBitonicThreads = 256;
+constexpr uint32_t kBitonicThreads = 1024;  // 2 pairs per thread per stage of a 4096-element block
 
 // should_swap of gpuSortPrototype.ts:20-26
 __device__ __forceinline__ bool should_swap(float lk, uint32_t li, float rk, uint32_t ri, bool ascending) {
@@ -61,6 +61,25 @@ __global__ void k_bitonic_global(uint2* a, uint32_t pairs, uint32_t k, uint32_t 
   compare_exchange(l, r, i, k);
   a[i] = l;
   a[i + j] = r;
+}
+
+// Two global stages (k, j) then (k, j/2) in one pass: thread t owns the four positions
+// i + {0, j/2, j, 3j/2}, i with bits j and j/2 clear; all four share bit k (j < k), so one
+// direction serves both stages.
+__global__ void k_bitonic_global2(uint2* a, uint32_t quads, uint32_t k, uint32_t j) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= quads) return;
+  const uint32_t h = j / 2;
+  const uint32_t i = (t / h) * 2u * j + (t % h);
+  uint2 x0 = a[i], x1 = a[i + h], x2 = a[i + j], x3 = a[i + j + h];
+  compare_exchange(x0, x2, i, k);
+  compare_exchange(x1, x3, i, k);
+  compare_exchange(x0, x1, i, k);
+  compare_exchange(x2, x3, i, k);
+  a[i] = x0;
+  a[i + h] = x1;
+  a[i + j] = x2;
+  a[i + j + h] = x3;
 }
 
 // All stages (k, j) with k in [k0, k1] (powers of two) and j < block, starting at j = j0 for k0,
@@ -112,7 +131,10 @@ void launch_depth_sort(const float* positions, uint32_t n, const float* view16, 
     // every stage with k <= block is local to a block
     hipLaunchKernelGGL(k_bitonic_local, dim3(nb), dim3(kBitonicThreads), 0, stream, work, block, 2u, block, 1u);
     for (uint32_t k = block * 2; k <= P; k <<= 1) {
-      for (uint32_t j = k / 2; j >= block; j >>= 1)
+      uint32_t j = k / 2;
+      for (; j >= block * 2; j >>= 2)  // strides j and j/2 together
+        hipLaunchKernelGGL(k_bitonic_global2, dim3((P / 4 + 255) / 256), dim3(256), 0, stream, work, P / 4, k, j);
+      if (j >= block)
         hipLaunchKernelGGL(k_bitonic_global, dim3((P / 2 + 255) / 256), dim3(256), 0, stream, work, P / 2, k, j);
       hipLaunchKernelGGL(k_bitonic_local, dim3(nb), dim3(kBitonicThreads), 0, stream, work, block, k, k, block / 2);
     }
