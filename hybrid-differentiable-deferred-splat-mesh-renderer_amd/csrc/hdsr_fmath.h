@@ -48,11 +48,8 @@ HDSR_HD float hdsr_u2f(uint32_t u) {
   return f;
 }
 
-// e^x with <= 2 ulp error on [-87, 88]; 0 below -87, +inf above 88, NaN passes through.
-HDSR_HD float hdsr_expf(float x) {
-  if (x != x) return x;
-  if (x > 88.0f) return INFINITY;
-  if (x < -87.0f) return 0.0f;
+// e^x for x in [-87, 88] (no range checks): the common path of hdsr_expf, bit-identical to it there.
+HDSR_HD float hdsr_expf_core(float x) {
   const float k = rintf(x * HDSR_LOG2E);
   float r = fmaf(k, -HDSR_LN2_HI, x);
   r = fmaf(k, -HDSR_LN2_LO, r);
@@ -65,6 +62,14 @@ HDSR_HD float hdsr_expf(float x) {
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
   return ldexpf(p, (int)k);
+}
+
+// e^x with <= 2 ulp error on [-87, 88]; 0 below -87, +inf above 88, NaN passes through.
+HDSR_HD float hdsr_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.0f) return INFINITY;
+  if (x < -87.0f) return 0.0f;
+  return hdsr_expf_core(x);
 }
 
 // ln x with <= 2 ulp error for finite x > 0; -inf at 0, NaN below 0, +inf at +inf.

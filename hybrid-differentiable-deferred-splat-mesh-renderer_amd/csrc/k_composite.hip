@@ -56,10 +56,13 @@ struct Gb {
   float mr, mm, nx, ny, nz, cf, er, eg, eb;
 };
 
+// One entry into one pixel, without branches: an entry that does not pass the gate leaves every
+// accumulator unchanged (w = 0, T kept).  exp is only consumed where power > -4, so its argument is
+// clamped into the range of the check-free path (the result there is hdsr_expf's, bit for bit).
 template <bool kG>
 __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
                                       const float4& ga, const float4& gb, float geb) {
-  const float gauss = hdsr_expf(power);
+  const float gauss = hdsr_expf_core(fminf(fmaxf(power, -87.0f), 88.0f));
   const float alpha = fminf(0.99f, o * gauss);
   const float newT = p.T * (1.0f - alpha);
   const bool valid = (power > -4.0f) && (alpha > (1.0f / 255.0f)) && (p.T >= 0.001f);
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
         dx = odd_x ? dx + 1.0f : dx;
         dy = odd_y ? dy + 1.0f : dy;
         const float pw = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
-        if (pw <= -4.0f) continue;
+        if (!__any(pw > -4.0f)) continue;  // uniform: no pixel of the wave is reached
         float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
         float gc = 0.0f;
         if (kG) {
