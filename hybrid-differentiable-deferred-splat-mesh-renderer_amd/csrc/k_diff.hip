@@ -167,39 +167,38 @@ __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restr
         dx = odd_x ? dx + 1.0f : dx;
         dy = odd_y ? dy + 1.0f : dy;
         const float pw = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
-        bool valid = false;
-        float gauss = 0.0f, alpha = 0.0f;
-        if (s_idx[j] < my_last && pw > -4.0f) {
-          gauss = hdsr_expf(pw);
-          alpha = fminf(0.99f, B.y * gauss);
-          valid = alpha > (1.0f / 255.0f);
-        }
-        if (__any(valid)) {
-          if (valid) {
+        const bool act = s_idx[j] < my_last && pw > -4.0f;
+        if (__any(act)) {  // uniform
+          // the forward's gaussian and alpha, bit for bit (exp only consumed where power > -4)
+          const float gauss = hdsr_expf_core(fminf(fmaxf(pw, -87.0f), 88.0f));
+          const float alpha = fminf(0.99f, B.y * gauss);
+          const bool valid = act && alpha > (1.0f / 255.0f);
+          if (__any(valid)) {
+            // every lane computes; lanes that did not blend contribute zeros and keep T, S
             const float om = 1.0f - alpha;
             const float Tj = T / om;
             const float w = alpha * Tj;
-            gv[6] = gin.x * w;
-            gv[7] = gin.y * w;
-            gv[8] = gin.z * w;
             const float dA = ((gin.x * (B.z * Tj - Sr / om) + gin.y * (B.w * Tj - Sg / om)) +
                               gin.z * (C.x * Tj - Sb / om)) + gAT / om;
-            Sr = Sr + B.z * w;
-            Sg = Sg + B.w * w;
-            Sb = Sb + C.x * w;
-            T = Tj;
-            if (B.y * gauss < 0.99f) {
-              const float gp = dA * alpha;  // d alpha / d power = o * exp(power) below the clamp
-              const float k0 = -2.0f * A.z, k1 = -A.w, k2 = -2.0f * B.x;
-              gv[0] = gp * (k0 * dx + k1 * dy);
-              gv[1] = gp * (k1 * dx + k2 * dy);
-              gv[2] = -0.5f * gp * dx * dx;
-              gv[3] = -gp * dx * dy;
-              gv[4] = -0.5f * gp * dy * dy;
-              gv[5] = dA * gauss;
-            }
+            gv[6] = valid ? gin.x * w : 0.0f;
+            gv[7] = valid ? gin.y * w : 0.0f;
+            gv[8] = valid ? gin.z * w : 0.0f;
+            // below the 0.99 clamp: d alpha / d power = o exp(power) = alpha, d alpha / d o = exp(power)
+            const bool cm = valid && B.y * gauss < 0.99f;
+            const float gp = cm ? dA * alpha : 0.0f;
+            const float k0 = -2.0f * A.z, k1 = -A.w, k2 = -2.0f * B.x;
+            gv[0] = gp * (k0 * dx + k1 * dy);
+            gv[1] = gp * (k1 * dx + k2 * dy);
+            gv[2] = -0.5f * gp * dx * dx;
+            gv[3] = -gp * dx * dy;
+            gv[4] = -0.5f * gp * dy * dy;
+            gv[5] = cm ? dA * gauss : 0.0f;
+            Sr = valid ? Sr + B.z * w : Sr;
+            Sg = valid ? Sg + B.w * w : Sg;
+            Sb = valid ? Sb + C.x * w : Sb;
+            T = valid ? Tj : T;
+            tot = wave_sum9(gv, lane);
           }
-          tot = wave_sum9(gv, lane);
         }
       }
       if (my_slot >= 0) s_g[wave][my_slot][j] = tot;
