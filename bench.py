@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
+    ap.add_argument("--no-f16", action="store_true", help="skip the f16-composite run of the same frames")
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiation workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
@@ -194,6 +195,7 @@ def main() -> int:
         torch.cuda.synchronize()
         with_sort = frames / (time.perf_counter() - t0)
     res.destroy()
+    f16 = None if a.no_f16 else f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames)
     serving = None if a.no_serving else serving_bench()
     diff = None if a.no_diff else diff_bench()
     cpu = None
@@ -231,11 +233,48 @@ def main() -> int:
         "gpu_depth_sort": None if sort_only is None else {
             "ms_per_view": round(sort_only, 4), "frames_per_s_with_sort": round(with_sort, 2),
             "note": "sortedIndices produced on the GPU every frame (bitonic network of the reference's encodeSort)"},
+        "f16_composite": f16,
         "serving": serving,
         "diff": diff,
     }
     print(json.dumps(out))
     return 0
+
+
+def f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames) -> dict:
+    """The same frames through the f16 composite variant (options.f16: binary16 transmittance and
+    colour, which the reference's renderer selects when the device has shader-f16,
+    src/splatRenderer.ts:907)."""
+    import torch
+
+    from hdsr import compositor
+    res = compositor.createTileSplatCompositor(None, plan, {"f16": True}, profile=True)
+    compositor.bind_scene(res, scene, sorted_host[0])
+
+    def render(v):
+        res.update_sorted_indices(sorted_dev[v])
+        res.set_frame(uniforms[v])
+        compositor.encodeFullComputeCompositorPipeline(handle, res)
+
+    for w in range(max(1, a.warmup)):
+        for v in range(len(uniforms)):
+            render(v)
+        res.sync()
+    reruns0 = res.stats()["overflow_reruns"]
+    torch.cuda.synchronize()
+    res.stage_times()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        for v in range(len(uniforms)):
+            render(v)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res.sync()
+    stage_ms, _ = res.stage_times()
+    ok = res.stats()["overflow_reruns"] == reruns0
+    res.destroy()
+    return {"frames_per_s": round(frames / wall, 2), "ms_per_frame": round(wall * 1e3 / frames, 4),
+            "composite_ms_per_frame": round(stage_ms["composite"], 4), "overflow_free": ok}
 
 
 def serving_bench(rates=(100, 200, 400, 600, 800, 1200, 3000, 5000), duration_s=1.0, max_requests=1500, seed=3) -> dict:

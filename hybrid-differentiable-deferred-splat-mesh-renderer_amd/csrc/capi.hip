@@ -134,8 +134,8 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out) {
 int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   if (!plan || !out) return HDSR_E_INVALID_ARG;
   *out = nullptr;
-  if (flags & (HDSR_F16_COMPOSITE | HDSR_REF_CAPS)) return HDSR_E_UNSUPPORTED;
-  if ((flags & HDSR_DIFFERENTIABLE) && (flags & HDSR_GBUFFER)) return HDSR_E_UNSUPPORTED;
+  if (flags & HDSR_REF_CAPS) return HDSR_E_UNSUPPORTED;
+  if ((flags & HDSR_DIFFERENTIABLE) && (flags & (HDSR_GBUFFER | HDSR_F16_COMPOSITE))) return HDSR_E_UNSUPPORTED;
   if (plan->tile_size_px != HDSR_TILE_PX) return HDSR_E_INVALID_ARG;
   if (plan->tile_columns == 0 || plan->tile_rows == 0 || plan->tile_columns > 255 || plan->tile_rows > 255)
     return HDSR_E_INVALID_ARG;  // u8 tile-box packing (gpu_project_splats.wgsl:229-231)
@@ -158,6 +158,7 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
     if (hipMalloc((void**)&c->last_idx, px * 4) != hipSuccess) return HDSR_E_OOM;
     c->eng.set_diff(true);
   }
+  c->eng.set_f16((flags & HDSR_F16_COMPOSITE) != 0);
   const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
   c->eng.set_min_capacity(base_cap);
   c->eng.set_profile((flags & HDSR_PROFILE) != 0);

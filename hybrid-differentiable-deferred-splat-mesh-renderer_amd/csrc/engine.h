@@ -58,6 +58,8 @@ class Engine {
   void set_min_capacity(uint64_t cap) { min_cap_ = cap; }
   // Differentiable mode: keeps the fp32 record table and the per-splat gradient slots.
   void set_diff(bool on) { diff_ = on; }
+  // Composite with binary16 transmittance / colour (the reference's f16 shader variant).
+  void set_f16(bool on) { f16_ = on; }
   // Backward of the last run (one frame, n splats); see launch_backward.
   int backward(uint32_t n, const float* grad_rgba, const DiffGrads& g, hipStream_t stream);
 
@@ -93,6 +95,7 @@ class Engine {
   uint64_t min_cap_ = 1u << 20;
   uint64_t want_cap_ = 0;
   bool diff_ = false;
+  bool f16_ = false;
   size_t n_diff_ = 0;
   // allocation sizes (elements)
   size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;

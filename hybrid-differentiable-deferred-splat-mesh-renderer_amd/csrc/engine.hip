@@ -348,7 +348,7 @@ void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, boo
   if (ev) (void)hipEventRecord(ev[7], stream);
   launch_segment_sort(b, stream);
   if (ev) (void)hipEventRecord(ev[8], stream);
-  launch_composite(b, stream, gbuffer);
+  launch_composite(b, stream, gbuffer, f16_);
   if (ev) (void)hipEventRecord(ev[9], stream);
 }
 
@@ -515,7 +515,7 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
   Layout L = layout(frames.data(), V);
   int rc = upload_frames(frames.data(), L, stream);
   if (rc) return rc;
-  launch_composite(b_, stream, gbuffer);
+  launch_composite(b_, stream, gbuffer, f16_);
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) {
     err = std::string("kernel launch failed: ") + hipGetErrorString(le);

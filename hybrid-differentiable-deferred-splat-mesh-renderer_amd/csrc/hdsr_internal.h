@@ -179,7 +179,7 @@ void launch_tile_ranges(const Bufs& b, hipStream_t stream);   // tile offsets, e
 void launch_classify(const Bufs& b, hipStream_t stream);
 void launch_bucket(const Bufs& b, hipStream_t stream);
 void launch_segment_sort(const Bufs& b, hipStream_t stream);
-void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer);
+void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16);
 // Differentiable mode (one frame, the batch's frame 0, with n splats): backward of the
 // rasterizer into gacc, then of the projection into the parameter gradients, which are
 // accumulated (added to) and indexed by splat id.

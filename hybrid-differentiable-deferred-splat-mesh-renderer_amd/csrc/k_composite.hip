@@ -19,8 +19,20 @@
 //   gw = w > 0.01 ? w : 0;    D += gw * depth;  W += gw
 // Output: rgba = (C + T * (0.02, 0.02, 0.04), 1 - T), depth = D / max(W, 1e-4).
 // With the G-buffer enabled, also the voted material / normal / confidence / emissive words.
+//
+// f16 variant (src/shaders/gpu_tile_splat_composite_f16.wgsl:170-281): power, exp and alpha stay
+// fp32; T, C and w are binary16 and every operation on them is one native f16 operation (single
+// rounding, no contraction):
+//   a16 = f16(alpha);  newT = T * (1 - a16);  w = (a16 * T) * valid;  C = C + rgb16 * w
+//   valid = power > -4 && alpha > f32(1h/255h) && f32(T) >= f32(0.001h)
+//   gw = f32(w) > 0.01 ? f32(w) : 0 (depth / G-buffer votes in fp32), emissive += e * f32(w)
+// Output: rgb = f32(C + T * (0.02h, 0.02h, 0.04h)), alpha = 1 - f32(T).  The constants are the
+// binary16 values nearest to the reference's literals: 1h/255h = 257/65536, 0.001h = 1049/2^20,
+// 0.02h = 1311/65536, 0.04h = 1311/32768.
 #include "hdsr_internal.h"
 #include "raster_common.h"
+
+#include <type_traits>
 
 namespace hdsr {
 namespace {
@@ -52,6 +64,18 @@ __device__ __forceinline__ void oct_encode(float x, float y, float z, float* ox,
 struct Px {
   float T, r, g, b, dw, ws;
 };
+// f16 variant: transmittance and colour in binary16
+struct Ph {
+  _Float16 T, r, g, b;
+  float dw, ws;
+};
+constexpr float kAlphaTh16 = 257.0f / 65536.0f;     // f16(1/255)
+constexpr float kTCut16 = 1049.0f / 1048576.0f;     // f16(0.001)
+constexpr float kBgRG16 = 1311.0f / 65536.0f;       // f16(0.02)
+constexpr float kBgB16 = 1311.0f / 32768.0f;        // f16(0.04)
+
+__device__ __forceinline__ bool saturated(const Px& p) { return p.T < 0.001f; }
+__device__ __forceinline__ bool saturated(const Ph& p) { return p.T < (_Float16)kTCut16; }
 struct Gb {
   float mr, mm, nx, ny, nz, cf, er, eg, eb;
 };
@@ -88,11 +112,52 @@ __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float 
   return valid;
 }
 
+// The f16 variant's blend: the same gate, with T, C and w carried in binary16.
 template <bool kG>
-__device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_t Y, const Px& p, const Gb& q) {
+__device__ __forceinline__ bool blend(Ph& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
+                                      const float4& ga, const float4& gb, float geb) {
+  const float gauss = hdsr_expf_core(fminf(fmaxf(power, -87.0f), 88.0f));
+  const float alpha = fminf(0.99f, o * gauss);
+  const _Float16 a16 = (_Float16)alpha;
+  const _Float16 newT = p.T * ((_Float16)1.0f - a16);
+  const bool valid = (power > -4.0f) && (alpha > kAlphaTh16) && (p.T >= (_Float16)kTCut16);
+  const _Float16 w = valid ? a16 * p.T : (_Float16)0.0f;
+  p.r = p.r + (_Float16)cr * w;
+  p.g = p.g + (_Float16)cg * w;
+  p.b = p.b + (_Float16)cb * w;
+  p.T = valid ? newT : p.T;
+  const float w32 = (float)w;
+  const float gw = w32 > 0.01f ? w32 : 0.0f;
+  p.dw = p.dw + gw * dep;
+  p.ws = p.ws + gw;
+  if (kG) {
+    q.mr = q.mr + ga.x * gw;
+    q.mm = q.mm + ga.y * gw;
+    q.nx = q.nx + gb.x * gw;
+    q.ny = q.ny + gb.y * gw;
+    q.nz = q.nz + gb.z * gw;
+    q.cf = q.cf + ga.z * gw;
+    q.er = q.er + ga.w * w32;
+    q.eg = q.eg + gb.w * w32;
+    q.eb = q.eb + geb * w32;
+  }
+  return valid;
+}
+
+__device__ __forceinline__ float4 out_rgba(const Px& p) {
+  return make_float4(p.r + p.T * 0.02f, p.g + p.T * 0.02f, p.b + p.T * 0.04f, 1.0f - p.T);
+}
+__device__ __forceinline__ float4 out_rgba(const Ph& p) {
+  const _Float16 bg = (_Float16)kBgRG16, bgb = (_Float16)kBgB16;
+  return make_float4((float)(p.r + p.T * bg), (float)(p.g + p.T * bg), (float)(p.b + p.T * bgb),
+                     1.0f - (float)p.T);
+}
+
+template <bool kG, typename P>
+__device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_t Y, const P& p, const Gb& q) {
   if (X >= F.width || Y >= F.height) return;
   const size_t o = (size_t)Y * F.width + X;
-  const float4 c = make_float4(p.r + p.T * 0.02f, p.g + p.T * 0.02f, p.b + p.T * 0.04f, 1.0f - p.T);
+  const float4 c = out_rgba(p);
   reinterpret_cast<float4*>(F.rgba)[o] = c;
   const float safe = fmaxf(p.ws, 0.0001f);
   F.out_depth[o] = p.dw / safe;
@@ -117,7 +182,8 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
 // kD: differentiable mode.  Records come from the fp32 table (drec) instead of the f16-packed
 // one, and each pixel also keeps its final transmittance and one past the index (in its tile's
 // list) of the last entry that blended, which bound the backward walk.
-template <bool kG, bool kD>
+// kH: the f16 variant (binary16 transmittance and colour).
+template <bool kG, bool kD, bool kH>
 __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_a[kBatch];  // cx, cy, a, b
   __shared__ float4 s_b[kBatch];  // c, opacity, r, g
@@ -144,7 +210,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const float p00x = (float)(X & ~1u) + 0.5f, p00y = (float)(Y & ~1u) + 0.5f;
   const bool odd_x = (lx & 1u) != 0, odd_y = (ly & 1u) != 0;
 
-  Px p = Px{1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  using PxT = typename std::conditional<kH, Ph, Px>::type;
+  PxT p{};
+  p.T = 1.0f;
   uint32_t last = 0;
   Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t loaded = 0;
@@ -219,9 +287,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
       g_next = i2 < count ? b.entries[start + i2] : 0u;
     }
     loaded += min(kBatch, count - base);
-    if (!__all(p.T < 0.001f)) {
+    if (!__all(saturated(p))) {
       for (uint32_t j = 0; j < nb; ++j) {
-        if ((j & 31u) == 31u && __all(p.T < 0.001f)) break;  // wave saturated
+        if ((j & 31u) == 31u && __all(saturated(p))) break;  // wave saturated
         const float4 C = s_c[j];
         if (C.z > band_y1 || C.w < band_y0) continue;  // uniform: the splat misses this wave's rows
         const float4 A = s_a[j];
@@ -242,7 +310,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
         if (kD && v) last = s_idx[j] + 1u;
       }
     }
-    const int wave_done = __all(p.T < 0.001f);
+    const int wave_done = __all(saturated(p));
     if ((tid & 63) == 0) s_done[tid >> 6] = wave_done;
     __syncthreads();
     if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) break;
@@ -250,7 +318,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   store_px<kG>(F, X, Y, p, q);
   if (kD && X < F.width && Y < F.height) {
     const size_t o = (size_t)Y * F.width + X;
-    if (F.t_final) F.t_final[o] = p.T;
+    if (F.t_final) F.t_final[o] = (float)p.T;
     if (F.last_idx) F.last_idx[o] = last;
   }
   if (tid == 0 && b.stats) {
@@ -261,14 +329,19 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
 
 }  // namespace
 
-void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer) {
+void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16) {
   if (!b.total_tiles) return;
+  const dim3 grid(b.total_tiles), block(256);
   if (b.drec)
-    hipLaunchKernelGGL((k_composite<false, true>), dim3(b.total_tiles), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL((k_composite<false, true, false>), grid, block, 0, stream, b);
+  else if (f16 && gbuffer)
+    hipLaunchKernelGGL((k_composite<true, false, true>), grid, block, 0, stream, b);
+  else if (f16)
+    hipLaunchKernelGGL((k_composite<false, false, true>), grid, block, 0, stream, b);
   else if (gbuffer)
-    hipLaunchKernelGGL((k_composite<true, false>), dim3(b.total_tiles), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL((k_composite<true, false, false>), grid, block, 0, stream, b);
   else
-    hipLaunchKernelGGL((k_composite<false, false>), dim3(b.total_tiles), dim3(256), 0, stream, b);
+    hipLaunchKernelGGL((k_composite<false, false, false>), grid, block, 0, stream, b);
 }
 
 }  // namespace hdsr

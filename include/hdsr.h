@@ -56,7 +56,9 @@ typedef struct hdsr_plan {
 int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
 
 /* createTileSplatCompositor options (gpuTileSplatCompositor.ts:141-145) and output selection. */
-#define HDSR_F16_COMPOSITE 1u   /* f16 compositing variant: not provided (returns HDSR_E_UNSUPPORTED) */
+#define HDSR_F16_COMPOSITE 1u   /* options.f16: composite with binary16 transmittance / colour
+                                   (gpu_tile_splat_composite_f16.wgsl); not combinable with
+                                   HDSR_DIFFERENTIABLE */
 #define HDSR_REF_CAPS 2u        /* emulate WebGPU caps: not provided (returns HDSR_E_UNSUPPORTED) */
 #define HDSR_GBUFFER 4u         /* also write the oct-normal and material/emissive/confidence words */
 #define HDSR_DEBUG 8u           /* keep the pre-sort entry list and the 20-bit keys for readback */
@@ -65,7 +67,7 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
 #define HDSR_DIFFERENTIABLE 64u /* differentiable mode (see hdsr_backward): opacities are logits,
                                    the rasterizer reads fp32 colour / opacity (no f16 cache
                                    rounding) and keeps per-pixel state for the backward; not
-                                   combinable with HDSR_GBUFFER */
+                                   combinable with HDSR_GBUFFER or HDSR_F16_COMPOSITE */
 
 typedef struct hdsr_ctx hdsr_ctx;
 

@@ -14,6 +14,7 @@
 //   bucket cascade        src/shaders/gpu_tile_bucket_sort.wgsl:41-144
 //   chunk sort            src/shaders/gpu_tile_chunk_sort.wgsl:36-97
 //   composite (fp32)      src/shaders/gpu_tile_splat_composite.wgsl:162-370
+//   composite (f16)       src/shaders/gpu_tile_splat_composite_f16.wgsl:170-281
 // and the plan arithmetic of src/gpuTileSplatCompositor.ts:109-139, 730-774.
 //
 // Decisions where the reference is nondeterministic or under-specified:
@@ -395,6 +396,19 @@ uint32_t f2u(float f) { return hdsr_f2u(f); }
 float u2f(uint32_t u) { return hdsr_u2f(u); }
 float h2f(uint32_t h) { return hdsr_f16_to_f32(h & 0xFFFFu); }
 
+// binary16 rounding (nearest, ties to even, subnormals kept, overflow to infinity) of a value
+// given exactly as a double.  Each f16 operation of the f16 composite is restated as the exact
+// result (a product or sum of two binary16 values is exact in double) rounded once by f16r.
+double f16r(double x) {
+  if (x == 0.0 || !std::isfinite(x)) return x;
+  int e;
+  std::frexp(x, &e);                       // |x| in [2^(e-1), 2^e)
+  const int ex = std::max(e - 1, -14);     // subnormals share the quantum of 2^-14's binade
+  const double q = std::ldexp(1.0, ex - 10);
+  const double r = std::nearbyint(x / q) * q;  // x / q is exact; default mode rounds to nearest even
+  return std::fabs(r) > 65504.0 ? std::copysign(INFINITY, x) : r;
+}
+
 // The 14-word cache record (gpu_project_splats.wgsl:2-16,220-232,290-357).
 void pack_record(const Proj<float>& P, uint32_t* w, uint32_t* depth_bits) {
   if (!P.visible) {
@@ -621,7 +635,14 @@ void bin_and_sort(Frame& F) {
 // composite (gpu_tile_splat_composite.wgsl:162-370), fp32, colour and depth.  Only tile rows
 // ty % row_step == 0 are composited (row_step 1 = the whole frame; larger values give a bounded
 // timing sample).
-void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
+// f16: the variant of gpu_tile_splat_composite_f16.wgsl:170-281.  Power, exp and alpha as in
+// fp32; T, the colour sums and the weight are binary16 values, every operation on them rounded
+// once to binary16 (f16r); the gates compare against the binary16 constants f16(1/255) and
+// f16(0.001); depth / G-buffer votes take f32(weight); output rgb = f16(C + f16(T * f16(bg))).
+void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false) {
+  const float a_th = f16 ? (float)f16r(1.0 / 255.0) : 1.0f / 255.0f;
+  const float t_cut = f16 ? (float)f16r(0.001) : 0.001f;
+  const double bg_rg16 = f16r(0.02), bg_b16 = f16r(0.04);
   const uint32_t W = F.width, H = F.height;
   F.rgba.assign((size_t)W * H * 4, 0.0f);
   F.out_depth.assign((size_t)W * H, 0.0f);
@@ -643,7 +664,7 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
           float gdw = 0, gws = 0;
           float mr = 0, mm = 0, nx = 0, ny = 0, nz = 0, gconf = 0, er = 0, eg = 0, eb = 0;
           for (uint32_t i = 0; i < c; ++i) {
-            if (T < 0.001f) break;  // no later splat passes 'T >= 0.001': nothing changes
+            if (T < t_cut) break;  // no later splat passes 'T >= cutoff': nothing changes
             const uint32_t r = F.entries[s + i];
             const uint32_t* w = &F.cache[(size_t)r * 14];
             const float cx = u2f(w[0]), cy = u2f(w[1]);
@@ -658,16 +679,27 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
             const float o = dm ? F.diffrec[(size_t)r * 4] : h2f(w[6] >> 16);
             const float g = hdsr_expf(power);
             const float alpha = std::fmin(0.99f, o * g);
-            const float newT = T * (1.0f - alpha);
-            const bool valid = power > -4.0f && alpha > (1.0f / 255.0f) && T >= 0.001f;
-            const float wgt = (alpha * T) * (valid ? 1.0f : 0.0f);
+            const bool valid = power > -4.0f && alpha > a_th && T >= t_cut;
             const float r_ = dm ? F.diffrec[(size_t)r * 4 + 1] : h2f(w[9]);
             const float g_ = dm ? F.diffrec[(size_t)r * 4 + 2] : h2f(w[9] >> 16);
             const float b_ = dm ? F.diffrec[(size_t)r * 4 + 3] : h2f(w[10]);
-            cr = cr + r_ * wgt;
-            cg = cg + g_ * wgt;
-            cb = cb + b_ * wgt;
-            if (valid) T = newT;
+            float wgt;
+            if (f16) {
+              const double a16 = f16r(alpha);
+              const float newT = (float)f16r((double)T * f16r(1.0 - a16));
+              wgt = valid ? (float)f16r(a16 * (double)T) : 0.0f;
+              cr = (float)f16r((double)cr + f16r((double)r_ * wgt));
+              cg = (float)f16r((double)cg + f16r((double)g_ * wgt));
+              cb = (float)f16r((double)cb + f16r((double)b_ * wgt));
+              if (valid) T = newT;
+            } else {
+              const float newT = T * (1.0f - alpha);
+              wgt = (alpha * T) * (valid ? 1.0f : 0.0f);
+              cr = cr + r_ * wgt;
+              cg = cg + g_ * wgt;
+              cb = cb + b_ * wgt;
+              if (valid) T = newT;
+            }
             const float gw = wgt > 0.01f ? wgt : 0.0f;
             gdw = gdw + gw * u2f(F.depth[r]);
             gws = gws + gw;
@@ -697,9 +729,15 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1) {
           // 'valid' is false for every later splat.
           if (X >= W || Y >= H) continue;
           const size_t p = (size_t)Y * W + X;
-          F.rgba[p * 4 + 0] = cr + T * 0.02f;
-          F.rgba[p * 4 + 1] = cg + T * 0.02f;
-          F.rgba[p * 4 + 2] = cb + T * 0.04f;
+          if (f16) {
+            F.rgba[p * 4 + 0] = (float)f16r((double)cr + f16r((double)T * bg_rg16));
+            F.rgba[p * 4 + 1] = (float)f16r((double)cg + f16r((double)T * bg_rg16));
+            F.rgba[p * 4 + 2] = (float)f16r((double)cb + f16r((double)T * bg_b16));
+          } else {
+            F.rgba[p * 4 + 0] = cr + T * 0.02f;
+            F.rgba[p * 4 + 1] = cg + T * 0.02f;
+            F.rgba[p * 4 + 2] = cb + T * 0.04f;
+          }
           F.rgba[p * 4 + 3] = 1.0f - T;
           const float safe = std::fmax(gws, 0.0001f);
           F.out_depth[p] = gdw / safe;
@@ -821,6 +859,7 @@ enum {
 #define ORC_FLAG_FP64_NOQUANT 8u
 #define ORC_FLAG_DIFF 16u  // differentiable mode: logit opacities, unquantized rasterizer inputs
 #define ORC_FLAG_NO_COMPOSITE 4u
+#define ORC_FLAG_F16 32u   // the f16 composite variant
 
 static double now_s() {
   timespec ts;
@@ -892,7 +931,8 @@ void* orc_run(const orc_splats* s, const void* uniforms256, uint32_t width, uint
     }
   }
   bin_and_sort(*F);
-  if (!(flags & ORC_FLAG_NO_COMPOSITE)) composite32(*F, (flags & ORC_FLAG_GBUFFER) != 0);
+  if (!(flags & ORC_FLAG_NO_COMPOSITE))
+    composite32(*F, (flags & ORC_FLAG_GBUFFER) != 0, 1, (flags & ORC_FLAG_F16) != 0);
   if (flags & (ORC_FLAG_FP64 | ORC_FLAG_FP64_NOQUANT)) composite64(*F, (flags & ORC_FLAG_FP64_NOQUANT) == 0);
   return F;
 }
@@ -964,6 +1004,9 @@ float orc_expf(float x) { return hdsr_expf(x); }
 float orc_logf(float x) { return hdsr_logf(x); }
 uint32_t orc_f16_bits(float x) { return hdsr_f16_bits(x); }
 float orc_f16_to_f32(uint32_t h) { return hdsr_f16_to_f32(h); }
+void orc_f16_round(const double* x, double* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = f16r(x[i]);
+}
 uint32_t orc_u32sat(float x) { return hdsr_u32sat(x); }
 uint32_t orc_morton2(uint32_t x, uint32_t y) { return hdsr_morton2(x, y); }
 

@@ -23,6 +23,7 @@ OUT = {
 STATS = ["refs", "small_tiles", "large_tiles", "chunks", "large_splats", "visible", "cap_refs_tripped",
          "cap_large_splats_tripped", "cap_chunks_tripped", "ref_cap"]
 FLAG_GBUFFER, FLAG_FP64, FLAG_NO_COMPOSITE, FLAG_FP64_NOQUANT, FLAG_DIFF = 1, 2, 4, 8, 16
+FLAG_F16 = 32  # composite with binary16 transmittance / colour (the f16 shader variant)
 
 
 class OrcSplats(C.Structure):
@@ -66,12 +67,22 @@ def load() -> C.CDLL:
     lib.orc_f16_bits.argtypes = [C.c_float]
     lib.orc_f16_to_f32.restype = C.c_float
     lib.orc_f16_to_f32.argtypes = [C.c_uint32]
+    lib.orc_f16_round.restype = None
+    lib.orc_f16_round.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
     lib.orc_u32sat.restype = C.c_uint32
     lib.orc_u32sat.argtypes = [C.c_float]
     lib.orc_morton2.restype = C.c_uint32
     lib.orc_morton2.argtypes = [C.c_uint32, C.c_uint32]
     _lib = lib
     return lib
+
+
+def f16_round(x) -> np.ndarray:
+    """The oracle's binary16 rounding (f16r) of float64 values, as float64."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    load().orc_f16_round(x.ctypes.data, out.ctypes.data, x.size)
+    return out
 
 
 _DTYPES = {"rgba": np.float32, "out_depth": np.float32, "rgba64": np.float64, "depth64": np.float64}

@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 PIXEL_TOL = 1e-5
 
 
-def render_gpu(scene, srt, u, plan, flags_debug=True, gbuffer=False):
-    res = compositor.createTileSplatCompositor(None, plan, debug=flags_debug, gbuffer=gbuffer)
+def render_gpu(scene, srt, u, plan, flags_debug=True, gbuffer=False, f16=False):
+    res = compositor.createTileSplatCompositor(None, plan, {"f16": f16}, debug=flags_debug, gbuffer=gbuffer)
     compositor.bind_scene(res, scene, srt)
     res.set_frame(u)
     compositor.encodeFullComputeCompositorPipeline(None, res)
@@ -26,7 +26,7 @@ def render_gpu(scene, srt, u, plan, flags_debug=True, gbuffer=False):
     return res
 
 
-def compare_all(res, orc, plan, gbuffer=False):
+def compare_all(res, orc, plan, gbuffer=False, exact=False):
     st = res.stats()
     assert st["refs"] == orc.stats["refs"]
     assert st["large_splats"] == orc.stats["large_splats"]
@@ -54,6 +54,9 @@ def compare_all(res, orc, plan, gbuffer=False):
     assert err <= PIXEL_TOL, f"max pixel error {err}"
     depth = res.read_depth().reshape(-1)
     assert np.abs(depth - orc["out_depth"]).max() <= PIXEL_TOL
+    if exact:
+        np.testing.assert_array_equal(rgba, orc["rgba"])
+        np.testing.assert_array_equal(depth, orc["out_depth"])
     if gbuffer:
         H, W = plan["viewportHeight"], plan["viewportWidth"]
         from hdsr import _lib
@@ -109,4 +112,21 @@ def test_repeat_runs_bit_identical(hip_device):
         b = (res.debug("entries"), res.debug("keys20"), res.read_rgba())
         for x, y in zip(a, b):
             np.testing.assert_array_equal(x, y)
+    res.destroy()
+
+
+@pytest.mark.parametrize("gbuffer", [False, True])
+def test_f16_composite_bit_exact(hip_device, gbuffer):
+    """options.f16 (gpu_tile_splat_composite_f16.wgsl): native binary16 transmittance / colour on
+    the GPU against the oracle's exactly-rounded restatement, bit for bit (lists, image, depth,
+    G-buffer words)."""
+    s = dense_scene(n=20_000, seed=12, sh_degree=3, core_fraction=0.5)
+    cam = camera.Camera(camera.orbit_eye(2.5, 0.6, 0.2), (0, 0, 0), 700.0, 700.0, 600, 420)
+    plan, srt, u = frame_inputs(s, cam)
+    flags = oracle.FLAG_F16 | oracle.FLAG_FP64 | (oracle.FLAG_GBUFFER if gbuffer else 0)
+    orc = oracle.run(s, srt, u, cam.width, cam.height, flags)
+    res = render_gpu(s, srt, u, plan, gbuffer=gbuffer, f16=True)
+    compare_all(res, orc, plan, gbuffer=gbuffer, exact=True)
+    fp64_err = np.abs(res.read_rgba().reshape(-1) - orc["rgba64"]).max()
+    print(f"f16 composite: bit-exact vs oracle, max|gpu-fp64|={fp64_err:.3g}")
     res.destroy()
