@@ -21,6 +21,7 @@ struct hdsr_ctx {
   uint64_t normals_len = 0;
   float *pos = nullptr, *scale = nullptr, *rot = nullptr, *opac = nullptr, *normals = nullptr, *sh = nullptr;
   uint32_t *material = nullptr, *sorted = nullptr;
+  float4* packed = nullptr;       // packed records of an owned scene (the projection's input)
   bool own_sorted = false;
   // frame
   bool have_frame = false;
@@ -57,10 +58,11 @@ int fail(hdsr_ctx* c, int code, const std::string& msg) {
 
 void free_scene(hdsr_ctx* c) {
   if (c->own_splats) {
-    void* p[] = {c->pos, c->scale, c->rot, c->opac, c->normals, c->sh, c->material};
+    void* p[] = {c->pos, c->scale, c->rot, c->opac, c->normals, c->sh, c->material, c->packed};
     for (void* q : p)
       if (q) (void)hipFree(q);
   }
+  c->packed = nullptr;
   if (c->own_sorted && c->sorted) (void)hipFree(c->sorted);
   c->pos = c->scale = c->rot = c->opac = c->normals = c->sh = nullptr;
   c->material = c->sorted = nullptr;
@@ -92,6 +94,8 @@ void fill_frame_scene(hdsr_ctx* c) {
   F.n = c->n;
   F.sh_degree = c->sh_degree;
   F.sh_stride = sh_stride(c->sh_degree);
+  F.packed = c->packed;
+  F.pk_stride4 = packed_stride4(F.sh_stride);
   F.width = c->plan.viewport_width;
   F.height = c->plan.viewport_height;
   F.rgba = c->user_rgba ? c->user_rgba : c->rgba;
@@ -212,6 +216,15 @@ int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
     if ((rc = dev_copy(c, &c->normals, s->normals, (size_t)s->normals_len))) return rc;
     if ((rc = dev_copy(c, &c->sh, s->sh_data, n * sh_stride(s->sh_degree)))) return rc;
     if ((rc = dev_copy(c, &c->sorted, s->sorted_indices, n))) return rc;
+    if (n) {
+      // one 128-byte-aligned record per splat for the projection's gathers
+      const uint32_t stride = sh_stride(s->sh_degree);
+      HIPX(c, hipMalloc((void**)&c->packed, n * (size_t)packed_stride4(stride) * sizeof(float4)));
+      launch_pack_scene(c->pos, c->scale, c->rot, c->opac, c->material, c->sh, (uint32_t)n, stride, c->packed,
+                        nullptr);
+      HIPX(c, hipGetLastError());
+      HIPX(c, hipDeviceSynchronize());
+    }
   } else {
     return fail(c, HDSR_E_INVALID_ARG, "memory must be HDSR_MEM_HOST or HDSR_MEM_DEVICE");
   }

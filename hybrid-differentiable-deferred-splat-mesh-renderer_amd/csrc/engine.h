@@ -23,6 +23,7 @@ struct Layout {
   uint32_t total_morton = 0;
   uint32_t total_cells = 0;
   uint32_t cell_key_bits = 0;
+  uint32_t packed = 0;                // every frame has packed splat records
   std::vector<uint32_t> block_start;  // [V+1]
   std::vector<uint32_t> tile_start;   // [V+1]
   std::vector<uint32_t> cell_start;   // [V+1]

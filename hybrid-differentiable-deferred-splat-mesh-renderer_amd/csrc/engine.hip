@@ -42,8 +42,10 @@ Layout Engine::layout(FrameDesc* frames, uint32_t V) {
   L.tile_start.resize(V + 1);
   L.cell_start.resize(V + 1);
   uint32_t sb = 0, mb = 0, tb = 0, cb = 0;
+  L.packed = V > 0;
   for (uint32_t v = 0; v < V; ++v) {
     FrameDesc& F = frames[v];
+    L.packed &= F.packed != nullptr || F.n == 0;  // an empty frame reads no splat
     F.splat_base = sb;
     F.morton_base = mb;
     F.tile_base = tb;
@@ -397,7 +399,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     tab[2 * (V + 1) + v] = L.cell_start[v];
   }
   const uint64_t sig[7] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.total_cells, L.cell_key_bits,
-                           (uint64_t)gbuffer};
+                           (uint64_t)gbuffer | ((uint64_t)L.packed << 1)};
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& g : graphs_)
     if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
@@ -417,6 +419,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     Bufs b = b_;
     b.keys20 = nullptr;
     b.entries_canon = nullptr;
+    b.packed = L.packed;
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
       err = "hipStreamBeginCapture failed";
@@ -483,6 +486,7 @@ int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool
   lay_ = L;
   if ((rc = upload_frames(frames.data(), L, stream))) return rc;
   Bufs b = b_;
+  b.packed = L.packed;
   if (!debug) {
     b.keys20 = nullptr;
     b.entries_canon = nullptr;

@@ -51,6 +51,10 @@ struct FrameDesc {
   uint32_t n;
   uint32_t sh_degree;
   uint32_t sh_stride;  // floats per splat in shdata: 3 + 3K + 3
+  // The same scene as one 128-byte-aligned record per splat (a bind-time copy of a scene the
+  // library owns), or null; see packed_stride4.
+  const float4* packed;
+  uint32_t pk_stride4;  // float4 slots per record
   // camera (parsed from the 256-byte uniform block)
   float vp[16];
   float view[16];
@@ -167,7 +171,17 @@ struct Bufs {
   Seg* segs;                 // [seg capacity]
   uint32_t seg_cap;
   DevStats* stats;
+  uint32_t packed;           // every frame reads its scene from packed records (host-set, selects the kernel)
 };
+
+// Packed splat record, in floats: [0..3] rotation wxyz, [4..6] position, [7..9] log-scale,
+// [10] opacity, [11] material bits, [12, 12 + sh_stride) the shData row, zero padding to a
+// multiple of 32 floats.  SH3 (51 floats) fills 256 bytes, SH0 128: a splat gathered through
+// sortedIndices costs two (one) 128-byte lines instead of one line per attribute array.
+inline uint32_t packed_stride4(uint32_t sh_stride) { return ((12u + sh_stride + 31u) / 32u) * 8u; }
+void launch_pack_scene(const float* pos, const float* logscale, const float* rot, const float* opacity,
+                       const uint32_t* material, const float* shdata, uint32_t n, uint32_t sh_stride, float4* out,
+                       hipStream_t stream);
 
 // Launchers (one per pipeline stage); every one enqueues on 'stream' and returns.
 void launch_clear_stats(const Bufs& b, hipStream_t stream);

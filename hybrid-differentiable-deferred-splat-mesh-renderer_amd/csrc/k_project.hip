@@ -35,6 +35,9 @@ __device__ __forceinline__ void oct_encode(F3 n, float* ox, float* oy) {
   *oy = py;
 }
 
+// kPacked: the scene is read from packed records (one or two 128-byte lines per splat) instead
+// of the caller's attribute arrays.
+template <bool kPacked>
 __global__ __launch_bounds__(256) void k_project(Bufs b) {
   const uint32_t blk = blockIdx.x;
   const uint32_t f = frame_of(b.block_frame_start, b.num_frames, blk);
@@ -49,7 +52,14 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
 
   const uint32_t id = (r < F.n) ? F.sorted[r] : 0xFFFFFFFFu;
   if (id < F.n) {
-    const F3 center = f3(F.pos[id * 3], F.pos[id * 3 + 1], F.pos[id * 3 + 2]);
+    const float4* R = kPacked ? F.packed + (size_t)id * F.pk_stride4 : nullptr;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0;
+    if (kPacked) {
+      r0 = R[0];
+      r1 = R[1];
+      r2 = R[2];
+    }
+    const F3 center = kPacked ? f3(r1.x, r1.y, r1.z) : f3(F.pos[id * 3], F.pos[id * 3 + 1], F.pos[id * 3 + 2]);
     float clip[4];
     mat4_point(F.vp, center, clip);
     if (clip[3] > 0.0f) {
@@ -59,13 +69,13 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
       const float py = (1.0f - (ndc_y * 0.5f + 0.5f)) * F.vh;
       const float depth = clip[2] / safe_w;
 
-      const float4 q4 = *reinterpret_cast<const float4*>(F.rot + (size_t)id * 4);
+      const float4 q4 = kPacked ? r0 : *reinterpret_cast<const float4*>(F.rot + (size_t)id * 4);
       const float qlen = sqrtf(((q4.x * q4.x + q4.y * q4.y) + q4.z * q4.z) + q4.w * q4.w);
       const float qd = fmaxf(qlen, 0.000001f);
       const float q[4] = {q4.x / qd, q4.y / qd, q4.z / qd, q4.w / qd};
-      const float sx = hdsr_expf(F.logscale[id * 3]) * F.splat_scale;
-      const float sy = hdsr_expf(F.logscale[id * 3 + 1]) * F.splat_scale;
-      const float sz = hdsr_expf(F.logscale[id * 3 + 2]) * F.splat_scale;
+      const float sx = hdsr_expf(kPacked ? r1.w : F.logscale[id * 3]) * F.splat_scale;
+      const float sy = hdsr_expf(kPacked ? r2.x : F.logscale[id * 3 + 1]) * F.splat_scale;
+      const float sz = hdsr_expf(kPacked ? r2.y : F.logscale[id * 3 + 2]) * F.splat_scale;
       const F3 a0 = mul(rotate_axis(q, f3(1.0f, 0.0f, 0.0f)), sx);
       const F3 a1 = mul(rotate_axis(q, f3(0.0f, 1.0f, 0.0f)), sy);
       const F3 a2 = mul(rotate_axis(q, f3(0.0f, 0.0f, 1.0f)), sz);
@@ -101,7 +111,8 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
         const float det_inv = 1.0f / fmaxf(det_blur, 0.000001f);
         const float k0 = cyy * det_inv, k1 = -cxy * det_inv, k2 = cxx * det_inv;
         // differentiable mode: the opacity input is a logit (sigmoid evaluated here)
-        const float o_in = F.diff ? 1.0f / (1.0f + hdsr_expf(-F.opacity[id])) : F.opacity[id];
+        const float o_raw = kPacked ? r2.z : F.opacity[id];
+        const float o_in = F.diff ? 1.0f / (1.0f + hdsr_expf(-o_raw)) : o_raw;
         const float opac = fminf(fmaxf(o_in * aa, 0.0f), 0.99f);
 
         // tile box (gpu_project_splats.wgsl:210-218,229-232)
@@ -176,7 +187,8 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
 
         // spherical harmonics (gpu_project_splats.wgsl:293-351)
         const uint32_t deg = F.sh_degree;
-        const float* sh = F.shdata + (size_t)id * F.sh_stride;
+        const float* sh = kPacked ? static_cast<const float*>(__builtin_assume_aligned(R + 3, 16))
+                                  : F.shdata + (size_t)id * F.sh_stride;
         float c[3] = {sh[0], sh[1], sh[2]};
         if (deg >= 1u) {
           const float x = view_dir.x, y = view_dir.y, z = view_dir.z;
@@ -222,7 +234,7 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
         const uint32_t op16 = hdsr_f16_bits(opac);
         h0 = make_uint4(hdsr_f2u(px), hdsr_f2u(py), hdsr_f2u(k0), hdsr_f2u(k1));
         h1 = make_uint4(hdsr_f2u(k2), depth_bits, hdsr_pack2x16(cr, cg), hdsr_f16_bits(cb) | (op16 << 16));
-        c0 = make_uint4(F.material[id], hdsr_f16_bits(radius) | (op16 << 16), hdsr_pack2x16(ox, oy),
+        c0 = make_uint4(kPacked ? hdsr_f2u(r2.w) : F.material[id], hdsr_f16_bits(radius) | (op16 << 16), hdsr_pack2x16(ox, oy),
                         hdsr_pack2x16(conf, 0.0f));
         c1 = make_uint4(hdsr_pack2x16(em[0], em[1]), hdsr_pack2x16(em[2], 0.0f), 0u, 0u);
         if (F.diff && b.drec) {
@@ -305,11 +317,48 @@ __global__ void k_cache14(Bufs b, uint32_t f, uint32_t n, uint32_t* dst) {
   w[13] = c0.w;
 }
 
+// One thread per float4 slot of the packed records (coalesced writes; runs once per bound scene).
+__global__ void k_pack_scene(const float* __restrict__ pos, const float* __restrict__ logscale,
+                             const float* __restrict__ rot, const float* __restrict__ opacity,
+                             const uint32_t* __restrict__ material, const float* __restrict__ shdata, uint32_t n,
+                             uint32_t sh_stride, uint32_t s4, uint64_t total, float4* __restrict__ out) {
+  const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const uint32_t i = (uint32_t)(q / s4), slot = (uint32_t)(q % s4);
+  float v[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t f = slot * 4 + k;
+    float x = 0.0f;
+    if (f < 4) x = rot[(size_t)i * 4 + f];
+    else if (f < 7) x = pos[(size_t)i * 3 + (f - 4)];
+    else if (f < 10) x = logscale[(size_t)i * 3 + (f - 7)];
+    else if (f == 10) x = opacity[i];
+    else if (f == 11) x = hdsr_u2f(material[i]);
+    else if (f - 12 < sh_stride) x = shdata[(size_t)i * sh_stride + (f - 12)];
+    v[k] = x;
+  }
+  out[q] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
 }  // namespace
 
 void launch_project(const Bufs& b, hipStream_t stream) {
   if (!b.splat_blocks) return;
-  hipLaunchKernelGGL(k_project, dim3(b.splat_blocks), dim3(kSplatBlock), 0, stream, b);
+  if (b.packed)
+    hipLaunchKernelGGL(k_project<true>, dim3(b.splat_blocks), dim3(kSplatBlock), 0, stream, b);
+  else
+    hipLaunchKernelGGL(k_project<false>, dim3(b.splat_blocks), dim3(kSplatBlock), 0, stream, b);
+}
+
+void launch_pack_scene(const float* pos, const float* logscale, const float* rot, const float* opacity,
+                       const uint32_t* material, const float* shdata, uint32_t n, uint32_t sh_stride, float4* out,
+                       hipStream_t stream) {
+  if (!n) return;
+  const uint32_t s4 = packed_stride4(sh_stride);
+  const uint64_t total = (uint64_t)n * s4;
+  hipLaunchKernelGGL(k_pack_scene, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, stream, pos, logscale, rot,
+                     opacity, material, shdata, n, sh_stride, s4, total, out);
 }
 
 void launch_cache14(const Bufs& b, uint32_t frame, uint32_t n, uint32_t* dst, hipStream_t stream) {

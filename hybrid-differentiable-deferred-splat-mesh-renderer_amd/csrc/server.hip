@@ -41,8 +41,11 @@ struct Scene {
   hdsr_splats host{};
   uint32_t n = 0, sh_degree = 0;
   bool resident = false;
-  float *pos = nullptr, *scale = nullptr, *rot = nullptr, *opac = nullptr, *normals = nullptr, *sh = nullptr;
-  uint32_t *material = nullptr, *sorted = nullptr;
+  // device copy: packed records (the projection's only splat input, built at upload), the
+  // normals and the order
+  float4* packed = nullptr;
+  float* normals = nullptr;
+  uint32_t* sorted = nullptr;
   uint64_t bytes = 0;
   uint64_t last_use = 0;
   uint32_t inflight = 0;  // requests of this scene that are batched and not yet complete
@@ -99,11 +102,12 @@ struct hdsr_server {
   }
 
   void free_scene_device(Scene& s) {
-    void* p[] = {s.pos, s.scale, s.rot, s.opac, s.normals, s.sh, s.material, s.sorted};
+    void* p[] = {s.normals, s.sorted, s.packed};
     for (void* q : p)
       if (q) (void)hipFree(q);
-    s.pos = s.scale = s.rot = s.opac = s.normals = s.sh = nullptr;
-    s.material = s.sorted = nullptr;
+    s.normals = nullptr;
+    s.sorted = nullptr;
+    s.packed = nullptr;
     if (s.resident) resident_bytes -= s.bytes;
     s.resident = false;
   }
@@ -134,10 +138,27 @@ struct hdsr_server {
       ++evictions;
     }
     const size_t n = s.n;
-    bool ok = up(&s.pos, s.host.positions, n * 3) && up(&s.scale, s.host.scales, n * 3) &&
-              up(&s.rot, s.host.rotations, n * 4) && up(&s.opac, s.host.opacities, n) &&
-              up(&s.material, s.host.materials, n) && up(&s.normals, s.host.normals, (size_t)s.host.normals_len) &&
-              up(&s.sh, s.host.sh_data, n * sh_stride_of(s.sh_degree));
+    const uint32_t stride = sh_stride_of(s.sh_degree);
+    // attribute arrays go up into temporaries and are packed on the device
+    float *pos = nullptr, *scale = nullptr, *rot = nullptr, *opac = nullptr, *sh = nullptr;
+    uint32_t* material = nullptr;
+    bool ok = up(&pos, s.host.positions, n * 3) && up(&scale, s.host.scales, n * 3) &&
+              up(&rot, s.host.rotations, n * 4) && up(&opac, s.host.opacities, n) &&
+              up(&material, s.host.materials, n) && up(&sh, s.host.sh_data, n * stride) &&
+              up(&s.normals, s.host.normals, (size_t)s.host.normals_len);
+    if (ok && n) {
+      ok = hipMalloc((void**)&s.packed, n * (size_t)packed_stride4(stride) * sizeof(float4)) == hipSuccess;
+      if (ok) {
+        launch_pack_scene(pos, scale, rot, opac, material, sh, (uint32_t)n, stride, s.packed, nullptr);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(nullptr) == hipSuccess;
+      } else {
+        s.packed = nullptr;
+        (void)hipGetLastError();
+      }
+    }
+    void* tmp[] = {pos, scale, rot, opac, sh, material};
+    for (void* q : tmp)
+      if (q) (void)hipFree(q);
     if (ok) {
       if (s.host.sorted_indices) {
         ok = up(&s.sorted, s.host.sorted_indices, n);
@@ -210,18 +231,14 @@ struct hdsr_server {
       s.last_use = ++use_clock;
       FrameDesc F{};
       parse_uniforms(r.uni, &F);
-      F.pos = s.pos;
-      F.logscale = s.scale;
-      F.rot = s.rot;
-      F.opacity = s.opac;
       F.sorted = s.sorted;
-      F.material = s.material;
       F.normals = s.normals;
       F.normals_len = (uint32_t)std::min<uint64_t>(s.host.normals_len, 0xFFFFFFFFull);
-      F.shdata = s.sh;
       F.n = s.n;
       F.sh_degree = s.sh_degree;
       F.sh_stride = sh_stride_of(s.sh_degree);
+      F.packed = s.packed;
+      F.pk_stride4 = packed_stride4(F.sh_stride);
       F.width = r.w;
       F.height = r.h;
       F.rgba = r.out.rgba;
@@ -381,8 +398,7 @@ int hdsr_server_add_scene(hdsr_server* s, uint32_t id, const hdsr_splats* sp) {
   sc.host = *sp;
   sc.n = sp->count;
   sc.sh_degree = sp->sh_degree;
-  sc.bytes = (uint64_t)sp->count * (12 + 12 + 16 + 4 + 4 + 4 + 4ull * sh_stride_of(sp->sh_degree)) +
-             sp->normals_len * 4;
+  sc.bytes = (uint64_t)sp->count * (4 + 16ull * packed_stride4(sh_stride_of(sp->sh_degree))) + sp->normals_len * 4;
   if (s->cfg.hbm_budget_bytes && sc.bytes > s->cfg.hbm_budget_bytes)
     return s->fail(HDSR_E_CAPACITY, "scene larger than the whole budget");
   s->scenes[id] = sc;
