@@ -1,8 +1,9 @@
 """Builds the HIP compositor library (lib/libhdsr.so) and the Node N-API addon for gfx950.
 
-Every .hip file under csrc/ is compiled with hipcc into an object, then linked into one shared
-library.  Floating-point contraction is off (the kernels must round exactly as the host-side
-restatement does) and packed FP32 instructions are disabled.
+Every .hip file under csrc/ is compiled with hipcc and every .cpp file (host-only code: the asset
+decoders) with g++, then the objects are linked into one shared library.  Floating-point
+contraction is off (the kernels must round exactly as the host-side restatement does) and packed
+FP32 instructions are disabled.
 
     python build.py            # library + addon
     python build.py --lib      # library only
@@ -43,16 +44,23 @@ HIP_FLAGS = [
 ]
 
 
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", f"-I{CSRC}",
+             f"-I{INCLUDE}"]
+
+
 def _sources() -> list[str]:
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
+    obj = os.path.join(OBJ, os.path.splitext(os.path.basename(src))[0] + ".o")
+    if src.endswith(".cpp"):
+        cmd = ["g++", *CXX_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *HIP_FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"compile failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 

@@ -170,9 +170,12 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   for (uint32_t q = t; q < kSortWaves * 256; q += kSortThreads) wc[q >> 8][q & 255] = 0;
   __syncthreads();
   const uint32_t lo = w * S, hi = min(lo + S, n);
+  const uint32_t rounds = S / 64;  // the same for every wave: a segment of n keys needs ceil(n/256)
   uint32_t keys[kSortPer], dig[kSortPer], pos[kSortPer];
 #pragma unroll
   for (uint32_t r = 0; r < kSortPer; ++r) {
+    pos[r] = 0xFFFFFFFFu;
+    if (r >= rounds) continue;  // uniform
     const uint32_t p = lo + r * 64 + lane;
     const bool valid = p < hi;
     const uint32_t key = valid ? cur[p] : 0u;
@@ -221,7 +224,7 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   __syncthreads();
 #pragma unroll
   for (uint32_t r = 0; r < kSortPer; ++r)
-    if (pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
+    if (r < rounds && pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
   __syncthreads();
 }
 
