@@ -10,10 +10,11 @@
 //   q20 = min(u32(y * 2^13), 2^20 - 1),  key = (q20 << 12) | i
 // with logMin / logRange from the min and max depth bits of the whole tile.  Keys are unique, so
 // the reference's bitonic network orders them exactly as a stable sort on q20 does; that sort is
-// done here as an LSD radix sort in LDS (digits of 8, 8 and 4 bits, a pass skipped when all keys
-// share its digit).  Equal digits are ranked inside a wave with eight ballots and a wave-private
-// running count, so a pass needs five barriers of a four-wave workgroup instead of the bitonic
-// network's 78 stages.
+// done here as an LSD radix sort in LDS over q20 - min(q20 of the segment), with digits of 8, 8
+// and 4 bits; a pass whose digit lies above the segment's key span is skipped (a chunk of a large
+// tile spans a fraction of its tile's range).  Equal digits are ranked inside a wave with one ballot
+// per digit bit and a wave-private running count, so a pass needs five barriers of a four-wave
+// workgroup instead of the bitonic network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
 // Large tiles: one 1024-thread workgroup writes the tile's entries in stable bucket order to
@@ -32,6 +33,17 @@ namespace hdsr {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Running count of a group of lanes with equal digits (m: the group's lanes, this lane among
+// them when valid): the group's highest lane adds the group's size to the wave-private counter
+// and the value before the add reaches every lane of the group through a permute.  Rounds do not
+// wait for one another: LDS atomics of one wave execute in issue order.
+__device__ __forceinline__ uint32_t group_count(uint32_t* counter, unsigned long long m, bool valid, uint32_t lane) {
+  const uint32_t leader = 63u - (uint32_t)__clzll(m | 1ull);
+  uint32_t old = 0;
+  if (valid && lane == leader) old = atomicAdd(counter, (uint32_t)__popcll(m));
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
+}
 
 // Large tiles.  Wave w of the 1024-thread workgroup owns the contiguous slice [w*S, (w+1)*S)
 // of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the entry's key;
@@ -123,11 +135,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
           m &= s ? bm : ~bm;
         }
         const uint32_t rank = __popcll(m & below);
-        uint32_t old = 0;
-        if (valid) old = wc[w][k];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && (m >> lane) == 1ull) wc[w][k] = old + rank + 1;
-        __builtin_amdgcn_wave_barrier();
+        const uint32_t old = group_count(&wc[w][k], m, valid, lane);
         if (valid) {
           b.scratch[L.scratch + old + rank] = gg[u];
           b.qkey_s[L.scratch + old + rank] = qq[u];
@@ -138,14 +146,14 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
   }
 }
 
-__device__ __forceinline__ uint32_t wave_and(uint32_t v) {
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v &= __shfl_xor(v, d, 64);
+  for (int d = 32; d >= 1; d >>= 1) v = min(v, (uint32_t)__shfl_xor(v, d, 64));
   return v;
 }
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v |= __shfl_xor(v, d, 64);
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
   return v;
 }
 
@@ -187,11 +195,7 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
       m &= s ? bm : ~bm;
     }
     const uint32_t rank = __popcll(m & below);
-    uint32_t old = 0;
-    if (valid) old = wc[w][d];
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (m >> lane) == 1ull) wc[w][d] = old + rank + 1;
-    __builtin_amdgcn_wave_barrier();
+    const uint32_t old = group_count(&wc[w][d], m, valid, lane);
     keys[r] = key;
     dig[r] = d;
     pos[r] = valid ? old + rank : 0xFFFFFFFFu;
@@ -245,44 +249,48 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
     const uint32_t count = S.count;
-    uint32_t vv[kSortPer];
-    uint32_t kand = 0xFFFFFFFFu, kor = 0;
+    uint32_t vv[kSortPer], qq[kSortPer];
+    uint32_t qmin = 0xFFFFFFFFu, qmax = 0;
 #pragma unroll
     for (uint32_t r = 0; r < kSortPer; ++r) {
       const uint32_t i = r * kSortThreads + t;
       vv[r] = 0;
+      qq[r] = 0;
       if (i < count) {
         vv[r] = src[i];
-        const uint32_t key = (qsrc[i] << 12) | i;
-        ka[i] = key;
-        kand &= key;
-        kor |= key;
+        qq[r] = qsrc[i];
+        qmin = min(qmin, qq[r]);
+        qmax = max(qmax, qq[r]);
       }
     }
-    kand = wave_and(kand);
-    kor = wave_or(kor);
+    qmin = wave_min(qmin);
+    qmax = wave_max(qmax);
     if (lane == 0) {
-      red[0][w] = kand;
-      red[1][w] = kor;
+      red[0][w] = qmin;
+      red[1][w] = qmax;
     }
     __syncthreads();
-    kand = red[0][0];
-    kor = red[1][0];
 #pragma unroll
-    for (uint32_t q = 1; q < kSortWaves; ++q) {
-      kand &= red[0][q];
-      kor |= red[1][q];
+    for (uint32_t q = 0; q < kSortWaves; ++q) {
+      qmin = min(qmin, red[0][q]);
+      qmax = max(qmax, red[1][q]);
     }
-    const uint32_t differ = kand ^ kor;  // bits that are not the same in every key
+    // keys relative to the segment's smallest q20: only the bits of qmax - qmin are ranked
+#pragma unroll
+    for (uint32_t r = 0; r < kSortPer; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      if (i < count) ka[i] = ((qq[r] - qmin) << 12) | i;
+    }
+    const uint32_t span = qmax - qmin;  // uniform; 0 for a segment of one entry or equal keys
+    const uint32_t qbits = span ? 32u - (uint32_t)__clz(span) : 0u;
     const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
     uint32_t* cur = ka;
     uint32_t* nxt = kb;
-    const uint32_t shifts[3] = {12, 20, 28}, widths[3] = {8, 8, 4};
+    const uint32_t lows[3] = {0, 8, 16}, widths[3] = {8, 8, 4};
 #pragma unroll
     for (uint32_t ps = 0; ps < 3; ++ps) {
-      const uint32_t dmask = ((1u << widths[ps]) - 1u) << shifts[ps];
-      if (count > 1 && (differ & dmask)) {  // uniform across the workgroup
-        lds_radix_pass(cur, nxt, count, Sw, shifts[ps], widths[ps], wc, wsum);
+      if (lows[ps] < qbits) {  // uniform: passes above the key span are skipped
+        lds_radix_pass(cur, nxt, count, Sw, 12 + lows[ps], widths[ps], wc, wsum);
         uint32_t* tmp = cur;
         cur = nxt;
         nxt = tmp;
@@ -298,7 +306,7 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     for (uint32_t i = t; i < count; i += kSortThreads) {
       const uint32_t key = cur[i];
       dst[i] = nxt[key & 0xFFFu];
-      if (b.keys20) b.keys20[S.dst + i] = key;
+      if (b.keys20) b.keys20[S.dst + i] = key + (qmin << 12);
     }
     __syncthreads();
   }
