@@ -6,7 +6,8 @@
 // gives each of 64 threads a 2x2 quad; on CDNA the tile's serial chain (every splat of a tile is
 // walked in order by the same lanes) is what bounds the kernel once all tiles are resident, so
 // spreading the 256 pixels over 256 lanes shortens the heaviest tile's chain four-fold.  Each
-// batch stages 256 entries' 32-byte records into LDS (one gather per thread); a wave whose 64
+// batch stages 256 entries' 32-byte records into LDS (one gather per thread); each wave (an 8x8
+// quadrant) walks only the entries whose conservative box reaches its quadrant, a wave whose 64
 // pixels are all saturated skips the batch, and the workgroup stops once all four are.  From
 // saturation on no splat can pass the 'T >= 0.001' gate, so the output equals the full walk.
 //
@@ -183,17 +184,23 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
 // one, and each pixel also keeps its final transmittance and one past the index (in its tile's
 // list) of the last entry that blended, which bound the backward walk.
 // kH: the f16 variant (binary16 transmittance and colour).
+//
+// Wave w owns the 8x8 quadrant (w & 1, w >> 1) of the tile.  While a batch is staged, every entry
+// tests its conservative ellipse box (reach_box) against the four quadrants and is appended, in
+// list order, to the index list of each quadrant it may reach; a wave then walks only its own
+// list, so entries that cannot touch its pixels cost it nothing.
 template <bool kG, bool kD, bool kH>
 __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_a[kBatch];  // cx, cy, a, b
   __shared__ float4 s_b[kBatch];  // c, opacity, r, g
-  __shared__ float4 s_c[kBatch];  // b, depth, reach y0, reach y1
+  __shared__ float2 s_c[kBatch];  // b, depth
   __shared__ float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
   __shared__ float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
   __shared__ float s_gc[kG ? kBatch : 1];   // emissive b
   __shared__ uint32_t s_idx[kD ? kBatch : 1];  // entry index of each compacted slot
+  __shared__ uint16_t s_list[4][kBatch];       // per quadrant: the slots that may reach it, in order
+  __shared__ uint32_t s_cnt[5][4];             // per staging wave: kept entries, then per quadrant
   __shared__ int s_done[4];
-  __shared__ uint32_t s_keep[4];
 
   const uint32_t t = blockIdx.x;
   const uint32_t f = frame_of(b.tile_frame_start, b.num_frames, t);
@@ -204,7 +211,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const uint32_t start = b.tile_offsets[m];
   const uint32_t count = b.tile_counts[m];
   const uint32_t tid = threadIdx.x;
-  const uint32_t lx = tid & 15, ly = tid >> 4;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t lx = (wave & 1u) * 8u + (lane & 7u), ly = (wave >> 1) * 8u + (lane >> 3);
   const uint32_t X = tx * 16 + lx, Y = ty * 16 + ly;
   // the reference's quad origin and the +1 offsets of the quad's odd column / row
   const float p00x = (float)(X & ~1u) + 0.5f, p00y = (float)(Y & ~1u) + 0.5f;
@@ -217,10 +226,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t loaded = 0;
 
-  const uint32_t lane = tid & 63, wave = tid >> 6;
   const float tile_x0 = (float)(tx * 16) + 0.5f, tile_y0 = (float)(ty * 16) + 0.5f;
-  // pixel-centre rows of this wave
-  const float band_y0 = tile_y0 + (float)(wave * 4), band_y1 = band_y0 + 3.0f;
   // Two-stage prefetch: while batch k is blended, the records of batch k+1 and the entry
   // indices of batch k+2 are in flight.
   const uint4 z4 = make_uint4(0, 0, 0, 0);
@@ -233,31 +239,43 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   uint32_t g_next = kBatch + tid < count ? b.entries[start + kBatch + tid] : 0u;
   for (uint32_t base = 0; base < count; base += kBatch) {
     const uint32_t i = base + tid;
-    bool keep = false;
     const uint4 h0 = h0n, h1 = h1n, h2 = h2n;
     const uint32_t g = g_cur;
-    float2 reach = make_float2(0.0f, 0.0f);
+    uint32_t qm = 0;  // quadrants the entry may reach
     if (i < count) {
-      reach = reach_y(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
-                      tile_x0 + 15.0f, tile_y0, tile_y0 + 15.0f);
-      keep = !(reach.x > reach.y);
+      const float4 bx = reach_box(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x),
+                                  tile_x0, tile_x0 + 15.0f, tile_y0, tile_y0 + 15.0f);
+      if (!(bx.x > bx.y)) {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          const float qx0 = tile_x0 + (float)((k & 1u) * 8u), qy0 = tile_y0 + (float)((k >> 1) * 8u);
+          if (!(bx.y < qx0 || bx.x > qx0 + 7.0f || bx.w < qy0 || bx.z > qy0 + 7.0f)) qm |= 1u << k;
+        }
+      }
     }
-    // compact the kept splats, in order
+    // compact the kept entries into slots, in order, and list each under its quadrants
+    const bool keep = qm != 0;
     const unsigned long long km = __ballot(keep);
-    if (lane == 0) s_keep[wave] = (uint32_t)__popcll(km);
+    unsigned long long qb[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) qb[k] = __ballot((qm >> k) & 1u);
+    if (lane == 0) {
+      s_cnt[0][wave] = (uint32_t)__popcll(km);
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) s_cnt[1 + k][wave] = (uint32_t)__popcll(qb[k]);
+    }
     __syncthreads();
-    uint32_t slot = (uint32_t)__popcll(km & ((1ull << lane) - 1ull));
-    for (uint32_t q = 0; q < wave; ++q) slot += s_keep[q];
-    const uint32_t nb = s_keep[0] + s_keep[1] + s_keep[2] + s_keep[3];
     if (keep) {
+      uint32_t slot = (uint32_t)__popcll(km & below);
+      for (uint32_t w2 = 0; w2 < wave; ++w2) slot += s_cnt[0][w2];
       s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
       if (kD) {
         s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w));
-        s_c[slot] = make_float4(hdsr_u2f(h2.x), hdsr_u2f(h2.y), reach.x, reach.y);
+        s_c[slot] = make_float2(hdsr_u2f(h2.x), hdsr_u2f(h2.y));
         s_idx[slot] = i;
       } else {
         s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
-        s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), reach.x, reach.y);
+        s_c[slot] = make_float2(h16(h1.w), hdsr_u2f(h1.y));
       }
       if (kG) {
         const uint4 c0 = b.cold[(size_t)g * 2];
@@ -276,7 +294,15 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
         s_gb[slot] = make_float4(dnx / dl, dny / dl, dnz / dl, h16(c1.x >> 16));
         s_gc[slot] = h16(c1.y);
       }
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k)
+        if ((qm >> k) & 1u) {
+          uint32_t pos = (uint32_t)__popcll(qb[k] & below);
+          for (uint32_t w2 = 0; w2 < wave; ++w2) pos += s_cnt[1 + k][w2];
+          s_list[k][pos] = (uint16_t)slot;
+        }
     }
+    const uint32_t nq = s_cnt[1 + wave][0] + s_cnt[1 + wave][1] + s_cnt[1 + wave][2] + s_cnt[1 + wave][3];
     __syncthreads();
     {
       const uint32_t i1 = base + kBatch + tid, i2 = base + 2 * kBatch + tid;
@@ -288,30 +314,30 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     }
     loaded += min(kBatch, count - base);
     if (!__all(saturated(p))) {
-      for (uint32_t j = 0; j < nb; ++j) {
+      for (uint32_t j = 0; j < nq; ++j) {
         if ((j & 31u) == 31u && __all(saturated(p))) break;  // wave saturated
-        const float4 C = s_c[j];
-        if (C.z > band_y1 || C.w < band_y0) continue;  // uniform: the splat misses this wave's rows
-        const float4 A = s_a[j];
-        const float4 B = s_b[j];
+        const uint32_t sl = s_list[wave][j];
+        const float4 A = s_a[sl];
+        const float4 B = s_b[sl];
         float dx = p00x - A.x, dy = p00y - A.y;
         dx = odd_x ? dx + 1.0f : dx;
         dy = odd_y ? dy + 1.0f : dy;
         const float pw = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
         if (!__any(pw > -4.0f)) continue;  // uniform: no pixel of the wave is reached
+        const float2 C = s_c[sl];
         float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
         float gc = 0.0f;
         if (kG) {
-          ga = s_ga[j];
-          gb = s_gb[j];
-          gc = s_gc[j];
+          ga = s_ga[sl];
+          gb = s_gb[sl];
+          gc = s_gc[sl];
         }
         const bool v = blend<kG>(p, q, pw, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
-        if (kD && v) last = s_idx[j] + 1u;
+        if (kD && v) last = s_idx[sl] + 1u;
       }
     }
     const int wave_done = __all(saturated(p));
-    if ((tid & 63) == 0) s_done[tid >> 6] = wave_done;
+    if (lane == 0) s_done[wave] = wave_done;
     __syncthreads();
     if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) break;
   }
