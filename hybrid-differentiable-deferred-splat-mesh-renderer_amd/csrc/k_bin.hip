@@ -539,6 +539,23 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
+  const FrameDesc& F = *ci.F;
+  // lanes 0..15 of wave w fetch tile 16w + lane's write base and log-depth frame up front, so
+  // the loads and the logarithms of all 16 tiles overlap the staging below
+  uint32_t my_base = 0, my_lm = 0, my_sc = 0, my_ok = 0;
+  if (lane < 16) {
+    const uint32_t tt = w * 16 + lane;
+    const uint32_t tx = ci.cx * kCellTiles + (tt & 7u), ty = ci.cy * kCellTiles + (tt >> 3);
+    if (tx < F.tiles_x && ty < F.tiles_y) {
+      const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
+      my_base = b.tile_offsets[m] + b.chunk_cnt[(size_t)q * 64 + tt];
+      float lm, sc;
+      hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lm, &sc);
+      my_lm = hdsr_f2u(lm);
+      my_sc = hdsr_f2u(sc);
+      my_ok = 1;
+    }
+  }
 #pragma unroll
   for (uint32_t r = 0; r < kCellChunk / 256; ++r) {
     const uint32_t i = r * 256 + t;
@@ -555,17 +572,14 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
     s_ln[i] = ln;
   }
   __syncthreads();
-  const FrameDesc& F = *ci.F;
   const uint32_t cap = b.cap;
   const uint32_t ngroups = (ci.count + 63) / 64;
   for (uint32_t k = 0; k < 16; ++k) {
     const uint32_t tt = w * 16 + k;  // this wave's tile (uniform)
-    const uint32_t tx = ci.cx * kCellTiles + (tt & 7u), ty = ci.cy * kCellTiles + (tt >> 3);
-    if (tx >= F.tiles_x || ty >= F.tiles_y) continue;
-    const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
-    uint32_t base = b.tile_offsets[m] + b.chunk_cnt[(size_t)q * 64 + tt];
-    float lm, sc;
-    hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lm, &sc);
+    if (!__builtin_amdgcn_readlane(my_ok, k)) continue;  // outside the tile grid
+    uint32_t base = __builtin_amdgcn_readlane(my_base, k);
+    const float lm = hdsr_u2f(__builtin_amdgcn_readlane(my_lm, k));
+    const float sc = hdsr_u2f(__builtin_amdgcn_readlane(my_sc, k));
     for (uint32_t gi = 0; gi < ngroups; ++gi) {
       const uint32_t i = gi * 64 + lane;
       const bool in = (s_mask[i] >> tt) & 1ull;  // entries past the count have an empty mask

@@ -1,4 +1,5 @@
-/* hdsr_napi.c - Node N-API addon over the C ABI of include/hdsr.h.
+/* hdsr_napi.c - Node N-API addon over the C ABI of include/hdsr.h (and the PLY decoder of
+ * include/hdsr_assets.h).
  *
  * Thin: every function converts JS values (plain objects, typed arrays, an external handle) to
  * the C structs and calls exactly one hdsr_* entry point; a nonzero return becomes a thrown JS
@@ -14,6 +15,7 @@
 #include <string.h>
 
 #include "hdsr.h"
+#include "hdsr_assets.h"
 
 #define CALL(env, call)                                        \
   do {                                                         \
@@ -424,6 +426,114 @@ static napi_value js_stage_times(napi_env env, napi_callback_info info) {
   return o;
 }
 
+/* New Float32Array / Uint32Array of n elements; *data receives its storage. */
+static napi_value new_typed(napi_env env, napi_typedarray_type type, size_t n, void** data) {
+  napi_value ab, ta;
+  if (napi_create_arraybuffer(env, n * 4, data, &ab) != napi_ok) return NULL;
+  if (napi_create_typedarray(env, type, n, ab, 0, &ta) != napi_ok) return NULL;
+  if (n) memset(*data, 0, n * 4);
+  return ta;
+}
+
+static napi_value vec3(napi_env env, const double* v) {
+  napi_value a, x;
+  napi_create_array_with_length(env, 3, &a);
+  for (uint32_t i = 0; i < 3; ++i) {
+    napi_create_double(env, v[i], &x);
+    napi_set_element(env, a, i, x);
+  }
+  return a;
+}
+
+/* plyDecode(fileName, bytes: ArrayBuffer | typed array) -> { count, shDegree, positions, ...,
+ * bounds }: hdsr_ply_inspect + hdsr_ply_decode; a failure throws the decoder's message. */
+static napi_value js_ply_decode(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  char name[1024] = {0};
+  size_t nl = 0;
+  napi_get_value_string_utf8(env, argv[0], name, sizeof name, &nl);
+  const void* bytes = NULL;
+  size_t len = 0;
+  bool is_ab = false, is_ta = false;
+  napi_is_arraybuffer(env, argv[1], &is_ab);
+  napi_is_typedarray(env, argv[1], &is_ta);
+  if (is_ab) {
+    void* d;
+    CALL(env, napi_get_arraybuffer_info(env, argv[1], &d, &len));
+    bytes = d;
+  } else if (is_ta) {
+    napi_value holder;
+    CALL(env, napi_create_object(env, &holder));
+    CALL(env, napi_set_named_property(env, holder, "a", argv[1]));
+    bytes = typed_prop(env, holder, "a", &len);
+  }
+  if (!bytes) {
+    napi_throw_type_error(env, NULL, "plyDecode: bytes must be an ArrayBuffer or a typed array");
+    return NULL;
+  }
+  char err[1024] = {0};
+  hdsr_ply_info pi;
+  int rc = hdsr_ply_inspect(name, bytes, len, &pi, err, sizeof err);
+  if (rc) {
+    napi_throw_error(env, NULL, err);
+    return NULL;
+  }
+  const size_t n = pi.count, k = pi.sh_coefficient_count;
+  hdsr_splat_attributes o;
+  memset(&o, 0, sizeof o);
+  napi_value res;
+  CALL(env, napi_create_object(env, &res));
+  struct {
+    const char* key;
+    void** dst;
+    size_t len;
+    napi_typedarray_type type;
+    int present;
+  } outs[] = {
+      {"positions", (void**)&o.positions, n * 3, napi_float32_array, 1},
+      {"colors", (void**)&o.colors, n * 3, napi_float32_array, 1},
+      {"opacities", (void**)&o.opacities, n, napi_float32_array, 1},
+      {"radii", (void**)&o.radii, n, napi_float32_array, 1},
+      {"scales", (void**)&o.scales, n * 3, napi_float32_array, 1},
+      {"rotations", (void**)&o.rotations, n * 4, napi_float32_array, 1},
+      {"shCoefficients", (void**)&o.sh_coefficients, n * k * 3, napi_float32_array, k != 0},
+      {"normals", (void**)&o.normals, n * 3, napi_float32_array, (pi.fields & HDSR_PLY_NORMALS) != 0},
+      {"roughness", (void**)&o.roughness, n, napi_float32_array, (pi.fields & HDSR_PLY_ROUGHNESS) != 0},
+      {"metalness", (void**)&o.metalness, n, napi_float32_array, (pi.fields & HDSR_PLY_METALNESS) != 0},
+      {"emissive", (void**)&o.emissive, n * 3, napi_float32_array, (pi.fields & HDSR_PLY_EMISSIVE) != 0},
+      {"detailNormals", (void**)&o.detail_normals, n * 3, napi_float32_array,
+       (pi.fields & HDSR_PLY_DETAIL_NORMALS) != 0},
+      {"originalIds", (void**)&o.original_ids, n, napi_uint32_array, 1},
+  };
+  for (size_t i = 0; i < sizeof outs / sizeof outs[0]; ++i) {
+    if (!outs[i].present) continue;
+    napi_value ta = new_typed(env, outs[i].type, outs[i].len, outs[i].dst);
+    if (!ta) {
+      napi_throw_error(env, NULL, "plyDecode: allocation failed");
+      return NULL;
+    }
+    CALL(env, napi_set_named_property(env, res, outs[i].key, ta));
+  }
+  rc = hdsr_ply_decode(name, bytes, len, &o, err, sizeof err);
+  if (rc) {
+    napi_throw_error(env, NULL, err);
+    return NULL;
+  }
+  set_u32(env, res, "count", pi.count);
+  set_u32(env, res, "shDegree", pi.sh_degree);
+  set_u32(env, res, "shCoefficientCount", pi.sh_coefficient_count);
+  napi_value bounds;
+  CALL(env, napi_create_object(env, &bounds));
+  CALL(env, napi_set_named_property(env, bounds, "min", vec3(env, o.bounds_min)));
+  CALL(env, napi_set_named_property(env, bounds, "max", vec3(env, o.bounds_max)));
+  CALL(env, napi_set_named_property(env, bounds, "center", vec3(env, o.bounds_center)));
+  set_f64(env, bounds, "radius", o.bounds_radius);
+  CALL(env, napi_set_named_property(env, res, "bounds", bounds));
+  return res;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value s;
@@ -448,6 +558,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"stats", NULL, js_stats, NULL, NULL, NULL, napi_default, NULL},
       {"stageTimes", NULL, js_stage_times, NULL, NULL, NULL, napi_default, NULL},
       {"version", NULL, js_version, NULL, NULL, NULL, napi_default, NULL},
+      {"plyDecode", NULL, js_ply_decode, NULL, NULL, NULL, napi_default, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;

@@ -6,6 +6,7 @@ GPU: one frame rendered through the facade in the reference's call order equals 
 (bit-exact entries / keys, pixels <= 1e-5).
 """
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -77,3 +78,49 @@ def test_facade_renders_like_oracle(hip_device):
         assert np.abs(rgba - orc["rgba"]).max() <= 1e-5
         depth = np.fromfile(os.path.join(d, "depth_js.bin"), np.float32)
         assert np.abs(depth - orc["out_depth"]).max() <= 1e-5
+
+
+def test_node_asset_loaders_match_python(tmp_path):
+    """The Node facade's loaders (splatAssets.js: PLY through the addon's hdsr_ply_decode, the
+    first-smoke manifest in JS) give the same arrays, bind buffers and bounds as hdsr.assets."""
+    import test_assets as ta
+    from hdsr import assets
+    rng = np.random.default_rng(5)
+    n = 300
+    props = [("float", nm) for nm in ["x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2"]]
+    props += [("float", f"f_rest_{i}") for i in range(45)]
+    props += [("float", nm) for nm in ["opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3",
+                                       "nx", "ny", "nz", "roughness", "metallic"]]
+    rows = rng.normal(0, 1, (n, len(props))).astype(np.float32).tolist()
+    (tmp_path / "sh3.ply").write_bytes(ta.ply_bytes(props, rows))
+    kat = ta.fixture([{"position": [1, 2, 3], "dc": [0, 1, -1], "opacity": 0, "scales": [0, math.log(2), math.log(0.5)],
+                       "rotation": [1, 0, 0, 0], "shRest": [0.25, 0.5, 0.75, 1, 1.25, 1.5, -0.25, -0.5, -0.75]}], sh=True)
+    (tmp_path / "kat.ply").write_bytes(kat)
+    (tmp_path / "bad.ply").write_bytes(b"not ply")
+    jobs = [{"name": "sh3", "kind": "ply", "file": str(tmp_path / "sh3.ply")},
+            {"name": "kat", "kind": "ply", "file": str(tmp_path / "kat.ply")},
+            {"name": "bad", "kind": "ply", "file": str(tmp_path / "bad.ply")},
+            {"name": "smoke", "kind": "manifest", "file": ta.SMOKE}]
+    (tmp_path / "assets.json").write_text(json.dumps(jobs))
+    out = run_node("assets_cpu.js", str(tmp_path))
+    assert out["bad"]["error"] == "Dropped file does not contain a complete PLY header"
+    for job in jobs[:2] + jobs[3:]:
+        name = job["name"]
+        a = (assets.decodeLocalPlySplatPayload(os.path.basename(job["file"]), open(job["file"], "rb").read())
+             if job["kind"] == "ply" else assets.loadFirstSmokeSplatPayload(job["file"]))
+        up = assets.uploadSplatAttributeBuffers(None, a)
+        want = {"positions": a.positions, "colors": a.colors, "opacities": a.opacities, "radii": a.radii,
+                "scales": a.scales, "rotations": a.rotations, "originalIds": a.originalIds,
+                "materials": up["materialBuffer"], "shData": up["shDataBuffer"]}
+        if a.sh:
+            want["sh"] = a.sh["coefficients"]
+        if a.normals is not None:
+            want["normals"] = a.normals
+        assert sorted(out[name]["arrays"]) == sorted(want)
+        for k, v in want.items():
+            got = np.fromfile(tmp_path / f"{name}.{k}.bin", dtype=v.dtype)
+            np.testing.assert_array_equal(got, np.asarray(v).reshape(-1), err_msg=f"{name}.{k}")
+        assert out[name]["count"] == a.count and out[name]["shDegree"] == up["shDegree"]
+        assert out[name]["bounds"] == a.bounds
+        fr = assets.framingFromBounds(a.bounds)
+        assert out[name]["framing"] == pytest.approx(fr, rel=1e-15)
