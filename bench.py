@@ -35,8 +35,8 @@ SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_b
 # HBM bytes per launch of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a 1 GiB copy (FETCH x2.0, WRITE
 # x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  Valid for the kernels as profiled.
-PMC_TRAFFIC_BYTES = {"k_project": 1427.8e6, "k_emit": 340.0e6, "k_bucket": 1257.4e6, "k_segsort": 666.7e6,
-                     "k_composite": 203.5e6}
+PMC_TRAFFIC_BYTES = {"k_project": 672.2e6, "k_emit": 30.6e6, "k_bucket": 714.1e6, "k_segsort": 493.3e6,
+                     "k_composite": 198.9e6}
 
 
 def parse():
@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation workload")
     ap.add_argument("--no-f16", action="store_true", help="skip the f16-composite run of the same frames")
+    ap.add_argument("--no-deferred", action="store_true", help="skip the deferred-normal passes")
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiation workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
@@ -58,17 +59,24 @@ def parse():
     return ap.parse_args()
 
 
-def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composite_refs: float) -> float:
-    """Bytes each stage must move at minimum (per launch, one view)."""
+def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composite_refs: float,
+                      cell_key_bits: int = 8) -> float:
+    """Bytes each stage must move at minimum (per launch, one view).  project, bucket, sort and
+    composite use the per-unit figures of the reference's path; emit, group and ranges are this
+    build's cell-level tile-list construction (P cell pairs), counted the same way."""
     R = st["refs"]
+    P = st.get("cell_pairs", 0)
     if stage == "project":
         # positions, log-scales, rotations, opacity, sortedIndex, material, shData (6+3K floats) in;
         # the 56-byte cache record + 4-byte depth out
         return n * (12 + 12 + 16 + 4 + 4 + 4 + 4 * (6 + 3 * k) + 56 + 4)
     if stage == "emit":
-        return 4 * n + 8 * R
+        return 4 * n + 8 * P + 4 * st.get("large_splats", 0)  # tile boxes in; (cell, splat) pairs out
     if stage == "group":
-        return 2 * 16 * R  # two 8-bit passes: read + write an 8-byte pair each
+        passes = max(1, -(-cell_key_bits // 8))
+        return 20 * P * passes + 12 * P  # radix passes on the cell key; the counting walk's gathers
+    if stage == "ranges":
+        return 16 * P + 8 * R  # the pairs' splat, box and log depth in; entry + key per reference out
     if stage == "bucket":
         return 44 * st.get("large_tile_refs", 0)
     if stage == "sort":
@@ -161,8 +169,10 @@ def main() -> int:
     # refs the rasterizer loaded: up to the batch in which each tile's last pixel saturated
     comp_refs = mean_st["composite_refs"]
     per_stage = {}
+    cells = -(-plan["tileColumns"] // 8) * -(-plan["tileRows"] // 8)
+    ckb = max(1, (cells - 1).bit_length())
     for name, ms in stage_ms.items():
-        byts = algorithmic_bytes(name, scene.count, k, mean_st, pixels, comp_refs)
+        byts = algorithmic_bytes(name, scene.count, k, mean_st, pixels, comp_refs, ckb)
         gbs = (byts / (ms * 1e-3) / 1e9) if ms > 0 and byts else None
         per_stage[name] = {"ms": round(ms, 4), "GBps": None if gbs is None else round(gbs, 1)}
     dom = max(SINGLE_KERNEL_STAGES, key=lambda s: stage_ms[s])
@@ -196,6 +206,7 @@ def main() -> int:
         with_sort = frames / (time.perf_counter() - t0)
     res.destroy()
     f16 = None if a.no_f16 else f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames)
+    deferred = None if a.no_deferred else deferred_bench(scene, plan, sorted_host, uniforms, handle)
     serving = None if a.no_serving else serving_bench()
     diff = None if a.no_diff else diff_bench()
     cpu = None
@@ -234,6 +245,7 @@ def main() -> int:
             "ms_per_view": round(sort_only, 4), "frames_per_s_with_sort": round(with_sort, 2),
             "note": "sortedIndices produced on the GPU every frame (bitonic network of the reference's encodeSort)"},
         "f16_composite": f16,
+        "deferred_normals": deferred,
         "serving": serving,
         "diff": diff,
     }
@@ -275,6 +287,40 @@ def f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames)
     res.destroy()
     return {"frames_per_s": round(frames / wall, 2), "ms_per_frame": round(wall * 1e3 / frames, 4),
             "composite_ms_per_frame": round(stage_ms["composite"], 4), "overflow_free": ok}
+
+
+def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
+    """The first deferred consumers of the G-buffer on view 0: screen-space normal reconstruction
+    and the bilateral normal filter (src/splatRenderer.ts:1042-1069), each timed by a HIP event pair
+    around its launch.  Algorithmic bytes per pixel: 12 (depth + normal word in, word out) each."""
+    import torch
+
+    from hdsr import compositor
+    res = compositor.createTileSplatCompositor(None, plan, gbuffer=True)
+    compositor.bind_scene(res, scene, sorted_host[0])
+    res.set_frame(uniforms[0])
+    compositor.encodeFullComputeCompositorPipeline(handle, res)
+    res.sync()
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, fn in (("screen_space_normals", res.encodeScreenSpaceNormals),
+                     ("bilateral_normal_filter", res.encodeBilateralNormalFilter)):
+        fn(handle)  # warm
+        total = 0.0
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            res.sync()
+            e0.record(stream)
+            fn(handle)
+            e1.record(stream)
+            e1.synchronize()
+            total += e0.elapsed_time(e1)
+        ms = total / reps
+        px = plan["viewportWidth"] * plan["viewportHeight"]
+        out[name] = {"ms": round(ms, 4), "GBps": round(12 * px / (ms * 1e-3) / 1e9, 1)}
+    res.destroy()
+    out["note"] = "view 0 at 1920x1080 with the G-buffer; 12 algorithmic bytes per pixel per pass"
+    return out
 
 
 def serving_bench(rates=(100, 200, 400, 600, 800, 1200, 3000, 5000), duration_s=1.0, max_requests=1500, seed=3) -> dict:

@@ -33,12 +33,14 @@ struct hdsr_ctx {
   float* user_depth = nullptr;
   uint32_t* gbn = nullptr;
   uint32_t* gbm = nullptr;
+  uint32_t* gbn_filtered = nullptr;  // screen-space normals output (allocated on first use)
   uint16_t* rgba16 = nullptr;
   float* t_final = nullptr;       // differentiable mode
   uint32_t* last_idx = nullptr;
   uint2* ds_work = nullptr;       // depth-sort scratch (hdsr_sort_back_to_front), on first use
   hipStream_t last_stream = nullptr;
   bool rendered = false;
+  bool settled = false;  // the last render's capacity check has run (hdsr_sync since the render)
   uint32_t reruns = 0;
   std::string err;
 };
@@ -174,7 +176,7 @@ void hdsr_destroy(hdsr_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   free_scene(c);
-  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx, c->ds_work};
+  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx, c->ds_work, c->gbn_filtered};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete c;
@@ -293,6 +295,7 @@ int hdsr_render(hdsr_ctx* c, void* stream) {
   if (rc) return fail(c, rc, c->eng.err);
   c->last_stream = static_cast<hipStream_t>(stream);
   c->rendered = true;
+  c->settled = false;
   return HDSR_OK;
 }
 
@@ -312,7 +315,10 @@ int hdsr_sync(hdsr_ctx* c) {
     int rc = c->eng.wait();
     if (rc) return fail(c, rc, c->eng.err);
     HIPX(c, hipStreamSynchronize(c->last_stream));
-    if (!c->eng.overflowed()) return HDSR_OK;
+    if (!c->eng.overflowed()) {
+      c->settled = true;
+      return HDSR_OK;
+    }
     c->eng.grow_for_overflow();
     c->reruns++;
     rc = c->eng.run(&c->frame, 1, c->last_stream, (c->flags & HDSR_GBUFFER) != 0, (c->flags & HDSR_DEBUG) != 0);
@@ -355,11 +361,111 @@ int hdsr_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
     case HDSR_OUT_RGBA16_BUF: src = c->rgba16; need = px * 8; break;
     case HDSR_OUT_T_FINAL: src = c->t_final; need = px * 4; break;
     case HDSR_OUT_LAST_INDEX: src = c->last_idx; need = px * 4; break;
+    case HDSR_OUT_FILTERED_NORMAL: src = c->gbn_filtered; need = px * 4; break;
     default: return fail(c, HDSR_E_INVALID_ARG, "unknown output");
   }
   if (!src) return fail(c, HDSR_E_STATE, "output not enabled by the create flags");
   if (bytes < need) return fail(c, HDSR_E_INVALID_ARG, "host buffer too small");
   HIPX(c, hipMemcpy(host, src, need, hipMemcpyDeviceToHost));
+  return HDSR_OK;
+}
+
+void hdsr_normal_params_default(hdsr_normal_params* p) {
+  if (!p) return;
+  p->recovery_mode = 0;
+  p->grazing_start = 0.45f;
+  p->grazing_end = 0.80f;
+  p->spatial_sigma = 3.0f;
+  p->depth_sigma = 0.05f;
+  p->normal_sigma = 1.5f;
+  p->kernel_radius = 3;
+}
+
+namespace {
+
+// mat4Inverse (src/splatRenderer.ts:148-168): cofactors in double from the fp32 matrix, stored to
+// fp32; false when |det| < 1e-10 (the reference then skips the deferred passes).
+bool mat4_inverse_ref(const float* m32, float* o) {
+  double m[16];
+  for (int i = 0; i < 16; ++i) m[i] = (double)m32[i];
+  const double a00 = m[0], a01 = m[1], a02 = m[2], a03 = m[3], a10 = m[4], a11 = m[5], a12 = m[6], a13 = m[7];
+  const double a20 = m[8], a21 = m[9], a22 = m[10], a23 = m[11], a30 = m[12], a31 = m[13], a32 = m[14], a33 = m[15];
+  const double b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+  const double b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+  const double b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+  const double b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+  double det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+  if (std::fabs(det) < 1e-10) return false;
+  det = 1.0 / det;
+  o[0] = (float)((a11 * b11 - a12 * b10 + a13 * b09) * det);
+  o[1] = (float)((a02 * b10 - a01 * b11 - a03 * b09) * det);
+  o[2] = (float)((a31 * b05 - a32 * b04 + a33 * b03) * det);
+  o[3] = (float)((a22 * b04 - a21 * b05 - a23 * b03) * det);
+  o[4] = (float)((a12 * b08 - a10 * b11 - a13 * b07) * det);
+  o[5] = (float)((a00 * b11 - a02 * b08 + a03 * b07) * det);
+  o[6] = (float)((a32 * b02 - a30 * b05 - a33 * b01) * det);
+  o[7] = (float)((a20 * b05 - a22 * b02 + a23 * b01) * det);
+  o[8] = (float)((a10 * b10 - a11 * b08 + a13 * b06) * det);
+  o[9] = (float)((a01 * b08 - a00 * b10 - a03 * b06) * det);
+  o[10] = (float)((a30 * b04 - a31 * b02 + a33 * b00) * det);
+  o[11] = (float)((a21 * b02 - a20 * b04 - a23 * b00) * det);
+  o[12] = (float)((a11 * b07 - a10 * b09 - a12 * b06) * det);
+  o[13] = (float)((a00 * b09 - a01 * b07 + a02 * b06) * det);
+  o[14] = (float)((a31 * b01 - a30 * b03 - a32 * b00) * det);
+  o[15] = (float)((a20 * b03 - a21 * b01 + a22 * b00) * det);
+  return true;
+}
+
+int deferred_ready(hdsr_ctx* c) {
+  if (!(c->flags & HDSR_GBUFFER)) return fail(c, HDSR_E_STATE, "create the context with HDSR_GBUFFER");
+  if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
+  // the render is final only once a capacity overflow has been re-run
+  if (!c->settled) {
+    const int rc = hdsr_sync(c);
+    if (rc) return rc;
+  }
+  if (!c->gbn_filtered) {
+    const size_t px = (size_t)c->plan.viewport_width * c->plan.viewport_height;
+    HIPX(c, hipMalloc((void**)&c->gbn_filtered, px * 4));
+  }
+  return HDSR_OK;
+}
+
+}  // namespace
+
+int hdsr_screen_space_normals(hdsr_ctx* c, const hdsr_normal_params* params, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  int rc = deferred_ready(c);
+  if (rc) return rc;
+  hdsr_normal_params p;
+  hdsr_normal_params_default(&p);
+  if (params) p = *params;
+  if (p.recovery_mode > 2) return fail(c, HDSR_E_INVALID_ARG, "recovery_mode must be 0, 1 or 2");
+  float inv[16];
+  if (!mat4_inverse_ref(c->frame.vp, inv)) return fail(c, HDSR_E_INVALID_ARG, "viewProj is singular");
+  const float* depth = c->user_depth ? c->user_depth : c->depth;
+  launch_screen_space_normals(depth, c->gbn, c->gbn_filtered, c->plan.viewport_width, c->plan.viewport_height, inv,
+                              c->frame.cam, p.recovery_mode, p.grazing_start, p.grazing_end,
+                              static_cast<hipStream_t>(stream));
+  HIPX(c, hipGetLastError());
+  c->last_stream = static_cast<hipStream_t>(stream);
+  return HDSR_OK;
+}
+
+int hdsr_bilateral_normal_filter(hdsr_ctx* c, const hdsr_normal_params* params, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  int rc = deferred_ready(c);
+  if (rc) return rc;
+  hdsr_normal_params p;
+  hdsr_normal_params_default(&p);
+  if (params) p = *params;
+  const float* depth = c->user_depth ? c->user_depth : c->depth;
+  if (launch_bilateral_normals(c->gbn_filtered, depth, c->gbn, c->plan.viewport_width, c->plan.viewport_height,
+                               p.spatial_sigma, p.depth_sigma, p.normal_sigma, (int)p.kernel_radius,
+                               static_cast<hipStream_t>(stream)) != 0)
+    return fail(c, HDSR_E_INVALID_ARG, "kernel_radius must be at most 8");
+  HIPX(c, hipGetLastError());
+  c->last_stream = static_cast<hipStream_t>(stream);
   return HDSR_OK;
 }
 
@@ -400,6 +506,7 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->composite_refs = s.composite_refs;
   o->composite_max_refs = s.composite_max_refs;
   o->large_tile_refs = s.scratch_used;
+  o->cell_pairs = s.cell_pairs;
   return HDSR_OK;
 }
 

@@ -179,6 +179,14 @@ struct Bufs {
 // multiple of 32 floats.  SH3 (51 floats) fills 256 bytes, SH0 128: a splat gathered through
 // sortedIndices costs two (one) 128-byte lines instead of one line per attribute array.
 inline uint32_t packed_stride4(uint32_t sh_stride) { return ((12u + sh_stride + 31u) / 32u) * 8u; }
+// Deferred consumers of the G-buffer (k_deferred.hip).
+void launch_screen_space_normals(const float* depth, const uint32_t* src, uint32_t* out, uint32_t w, uint32_t h,
+                                 const float* inv16, const float* cam3, uint32_t mode, float g0, float g1,
+                                 hipStream_t stream);
+// Returns -1 (nothing launched) for a radius outside [0, 8].
+int launch_bilateral_normals(const uint32_t* in, const float* depth, uint32_t* out, uint32_t w, uint32_t h,
+                             float spatial_sigma, float depth_sigma, float normal_sigma, int radius,
+                             hipStream_t stream);
 void launch_pack_scene(const float* pos, const float* logscale, const float* rot, const float* opacity,
                        const uint32_t* material, const float* shdata, uint32_t n, uint32_t sh_stride, float4* out,
                        hipStream_t stream);

@@ -31,7 +31,7 @@ STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "s
 HDSR_MEM_HOST = 0
 HDSR_MEM_DEVICE = 1
 
-OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX = range(7)
+OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX, OUT_FILTERED_NORMAL = range(8)
 
 DBG = {
     "projcache": 0, "depth": 1, "tile_counts": 2, "tile_offsets": 3, "entries": 4, "entries_canon": 5,
@@ -68,12 +68,32 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "refs", "small_pairs", "large_pairs", "large_splats", "visible_splats", "small_tiles", "large_tiles",
         "chunks", "segments", "pair_capacity", "overflow_reruns", "cap_refs_tripped", "cap_large_splats_tripped",
-        "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs")]
+        "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs", "cell_pairs")]
 
 
 class Grads(C.Structure):
     _fields_ = [("means", C.c_void_p), ("log_scales", C.c_void_p), ("rotations", C.c_void_p),
                 ("opacity_logits", C.c_void_p), ("sh_data", C.c_void_p)]
+
+
+class NormalParams(C.Structure):
+    _fields_ = [("recovery_mode", C.c_uint32), ("grazing_start", C.c_float), ("grazing_end", C.c_float),
+                ("spatial_sigma", C.c_float), ("depth_sigma", C.c_float), ("normal_sigma", C.c_float),
+                ("kernel_radius", C.c_uint32)]
+
+
+class PlyInfo(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_uint32), ("sh_coefficient_count", C.c_uint32),
+                ("fields", C.c_uint32), ("data_offset", C.c_uint64), ("row_stride", C.c_uint32),
+                ("property_count", C.c_uint32)]
+
+
+class SplatAttributesOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "opacities", "radii", "scales", "rotations",
+                                          "sh_coefficients", "normals", "roughness", "metalness", "emissive",
+                                          "detail_normals", "original_ids")] + [
+        ("bounds_min", C.c_double * 3), ("bounds_max", C.c_double * 3), ("bounds_center", C.c_double * 3),
+        ("bounds_radius", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -99,7 +119,7 @@ class ServerStats(C.Structure):
                 ("p99_ms", C.c_double), ("mean_batch_views", C.c_double), ("busy_s", C.c_double)]
 
 
-# Every symbol include/hdsr.h declares: (name, restype, argtypes).
+# Every symbol include/hdsr.h and include/hdsr_assets.h declare: (name, restype, argtypes).
 SIGNATURES = [
     ("hdsr_plan_compute", C.c_int, [C.POINTER(PlanIn), C.POINTER(Plan)]),
     ("hdsr_create", C.c_int, [C.POINTER(Plan), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -119,6 +139,12 @@ SIGNATURES = [
     ("hdsr_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("hdsr_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     ("hdsr_backward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Grads), C.c_void_p]),
+    ("hdsr_normal_params_default", None, [C.POINTER(NormalParams)]),
+    ("hdsr_screen_space_normals", C.c_int, [C.c_void_p, C.POINTER(NormalParams), C.c_void_p]),
+    ("hdsr_bilateral_normal_filter", C.c_int, [C.c_void_p, C.POINTER(NormalParams), C.c_void_p]),
+    ("hdsr_ply_inspect", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(PlyInfo), C.c_char_p, C.c_uint32]),
+    ("hdsr_ply_decode", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(SplatAttributesOut), C.c_char_p,
+                                  C.c_uint32]),
     ("hdsr_last_error", C.c_char_p, [C.c_void_p]),
     ("hdsr_version", C.c_char_p, []),
     ("hdsr_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),

@@ -68,32 +68,15 @@ class AssetError(ValueError):
 # ----------------------------------------------------------------------------------------------
 # PLY (C++ decoder in the library)
 
-class _PlyInfo(C.Structure):
-    _fields_ = [("count", C.c_uint32), ("sh_degree", C.c_uint32), ("sh_coefficient_count", C.c_uint32),
-                ("fields", C.c_uint32), ("data_offset", C.c_uint64), ("row_stride", C.c_uint32),
-                ("property_count", C.c_uint32)]
-
-
-class _PlyOut(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "opacities", "radii", "scales", "rotations",
-                                          "sh_coefficients", "normals", "roughness", "metalness", "emissive",
-                                          "detail_normals", "original_ids")] + [
-        ("bounds_min", C.c_double * 3), ("bounds_max", C.c_double * 3), ("bounds_center", C.c_double * 3),
-        ("bounds_radius", C.c_double)]
+_PlyInfo = _lib.PlyInfo
+_PlyOut = _lib.SplatAttributesOut
 
 
 PLY_NORMALS, PLY_ROUGHNESS, PLY_METALNESS, PLY_EMISSIVE, PLY_DETAIL_NORMALS = 1, 2, 4, 8, 16
 
 
 def _ply_lib():
-    lib = _lib.load()
-    if not getattr(lib, "_ply_declared", False):
-        lib.hdsr_ply_inspect.argtypes = [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(_PlyInfo), C.c_char_p, C.c_uint32]
-        lib.hdsr_ply_inspect.restype = C.c_int
-        lib.hdsr_ply_decode.argtypes = [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(_PlyOut), C.c_char_p, C.c_uint32]
-        lib.hdsr_ply_decode.restype = C.c_int
-        lib._ply_declared = True
-    return lib
+    return _lib.load()
 
 
 def decodeLocalPlySplatPayload(fileName: str, data) -> SplatAttributes:

@@ -102,6 +102,26 @@ class Resources:
         check(self._lib.hdsr_readback(self.handle, which, out.ctypes.data, out.nbytes), self.handle)
         return out
 
+    def normal_params(self, **overrides) -> "_lib.NormalParams":
+        """The reference's deferred-normal parameters (hdsr_normal_params_default), with overrides."""
+        p = _lib.NormalParams()
+        self._lib.hdsr_normal_params_default(C.byref(p))
+        for k, v in overrides.items():
+            setattr(p, k, v)
+        return p
+
+    def encodeScreenSpaceNormals(self, stream=None, params=None) -> None:
+        """createScreenSpaceNormalsPass().encode (src/splatRenderer.ts:317-376, :1048-1057):
+        G-buffer depth + normals -> the filtered-normal buffer (needs gbuffer=True)."""
+        p = params if params is not None else self.normal_params()
+        check(self._lib.hdsr_screen_space_normals(self.handle, C.byref(p), C.c_void_p(stream)), self.handle)
+
+    def encodeBilateralNormalFilter(self, stream=None, params=None) -> None:
+        """createBilateralNormalFilter().encode (src/splatRenderer.ts:382-442, :1063-1069): filtered
+        normals + depth -> the G-buffer normal words."""
+        p = params if params is not None else self.normal_params()
+        check(self._lib.hdsr_bilateral_normal_filter(self.handle, C.byref(p), C.c_void_p(stream)), self.handle)
+
     def debug(self, name: str) -> np.ndarray:
         which = _lib.DBG[name]
         n = C.c_uint64()

@@ -404,6 +404,7 @@ static napi_value js_stats(napi_env env, napi_callback_info info) {
   set_u32(env, o, "compositeRefs", s.composite_refs);
   set_u32(env, o, "compositeMaxRefs", s.composite_max_refs);
   set_u32(env, o, "largeTileRefs", s.large_tile_refs);
+  set_u32(env, o, "cellPairs", s.cell_pairs);
   return o;
 }
 

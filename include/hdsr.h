@@ -155,8 +155,36 @@ int hdsr_bind_outputs(hdsr_ctx* ctx, float* rgba, float* depth);
 #define HDSR_OUT_RGBA16_BUF 4
 #define HDSR_OUT_T_FINAL 5        /* H*W f32 final transmittance (HDSR_DIFFERENTIABLE) */
 #define HDSR_OUT_LAST_INDEX 6     /* H*W u32 one past the last blended entry of the pixel's tile list */
+#define HDSR_OUT_FILTERED_NORMAL 7 /* H*W u32 oct-normal words of hdsr_screen_space_normals (HDSR_GBUFFER) */
 /* Copies an output to host memory (after hdsr_sync). */
 int hdsr_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
+
+/* The first deferred consumers of the G-buffer (src/splatRenderer.ts:1042-1069), over the last
+ * render's fp32 depth and oct-normal words (HDSR_GBUFFER).  The inverse view-projection is the
+ * reference's mat4Inverse (src/splatRenderer.ts:148-168: double arithmetic, stored to fp32) of the
+ * frame's viewProj; the camera position is the frame's.  Asynchronous on 'stream'. */
+typedef struct hdsr_normal_params {
+  uint32_t recovery_mode;  /* 0 screen-space normals, 1 grazing recovery, 2 source pass-through;
+                              the reference passes 0 for scenes without per-splat normals, else 2
+                              (src/splatRenderer.ts:1047) */
+  float grazing_start;     /* 0.45 (src/splatRenderer.ts:359-360) */
+  float grazing_end;       /* 0.80 */
+  float spatial_sigma;     /* 3.0  bilateral filter (src/splatRenderer.ts:419-423) */
+  float depth_sigma;       /* 0.05 */
+  float normal_sigma;      /* 1.5 */
+  uint32_t kernel_radius;  /* 3, at most 8 */
+} hdsr_normal_params;
+void hdsr_normal_params_default(hdsr_normal_params* out);
+
+/* Screen-space normal reconstruction (src/shaders/gpu_screen_space_normals.wgsl): G-buffer depth
+ * and normal words -> the context's filtered-normal buffer (HDSR_OUT_FILTERED_NORMAL).  'params'
+ * may be null (the reference's values). */
+int hdsr_screen_space_normals(hdsr_ctx* ctx, const hdsr_normal_params* params, void* stream);
+
+/* Depth-guided bilateral normal filter (src/shaders/bilateral_normal_filter.wgsl): the
+ * filtered-normal buffer and depth -> the G-buffer normal words, which it overwrites, as the
+ * reference writes its result back into the G-buffer normal texture. */
+int hdsr_bilateral_normal_filter(hdsr_ctx* ctx, const hdsr_normal_params* params, void* stream);
 
 /* Intermediates of the last frame, in the reference's layouts (ranks, Morton ids). */
 #define HDSR_DBG_PROJCACHE 0      /* N*14 u32 (gpu_project_splats.wgsl:2-16) */
@@ -193,6 +221,7 @@ typedef struct hdsr_stats {
   uint32_t composite_refs;        /* entries the rasterizer loaded (up to each tile's saturation) */
   uint32_t composite_max_refs;    /* the largest per-tile count of those */
   uint32_t large_tile_refs;       /* references in tiles with > 4096 entries (bucket cascade input) */
+  uint32_t cell_pairs;            /* (8x8-tile cell, splat) pairs of the tile-list build */
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 

@@ -15,6 +15,9 @@
 //   chunk sort            src/shaders/gpu_tile_chunk_sort.wgsl:36-97
 //   composite (fp32)      src/shaders/gpu_tile_splat_composite.wgsl:162-370
 //   composite (f16)       src/shaders/gpu_tile_splat_composite_f16.wgsl:170-281
+//   deferred normals      src/shaders/gpu_screen_space_normals.wgsl:59-120,
+//                         src/shaders/bilateral_normal_filter.wgsl:39-86, mat4Inverse
+//                         (src/splatRenderer.ts:148-168)
 // and the plan arithmetic of src/gpuTileSplatCompositor.ts:109-139, 730-774.
 //
 // Decisions where the reference is nondeterministic or under-specified:
@@ -1004,6 +1007,167 @@ float orc_expf(float x) { return hdsr_expf(x); }
 float orc_logf(float x) { return hdsr_logf(x); }
 uint32_t orc_f16_bits(float x) { return hdsr_f16_bits(x); }
 float orc_f16_to_f32(uint32_t h) { return hdsr_f16_to_f32(h); }
+// ---- deferred consumers of the G-buffer ----
+// WGSL builtins as this restatement reads them: normalize(v) = v / sqrt(dot(v, v)); dot sums
+// left to right; mix(a, b, t) = a (1 - t) + b t; smoothstep as the WGSL spec's polynomial;
+// pow(x, y) = exp(y ln x) with hdsr_expf / hdsr_logf; matrix times vector sums the four column
+// terms left to right.
+
+struct D3 {
+  float x, y, z;
+};
+static D3 d3_norm(D3 a) {
+  const float l = std::sqrt((a.x * a.x + a.y * a.y) + a.z * a.z);
+  return D3{a.x / l, a.y / l, a.z / l};
+}
+static float d3_dot(D3 a, D3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+static D3 oct_dec(uint32_t w) {
+  // octDecode(unpack2x16float(w)) (gpu_screen_space_normals.wgsl:44-50)
+  const float ex = hdsr_f16_to_f32(w & 0xFFFFu), ey = hdsr_f16_to_f32(w >> 16);
+  float nx = ex, ny = ey;
+  const float nz = (1.0f - std::fabs(ex)) - std::fabs(ey);
+  if (nz < 0.0f) {
+    nx = (1.0f - std::fabs(ey)) * (ex >= 0.0f ? 1.0f : -1.0f);
+    ny = (1.0f - std::fabs(ex)) * (ey >= 0.0f ? 1.0f : -1.0f);
+  }
+  return d3_norm(D3{nx, ny, nz});
+}
+static uint32_t oct_enc(D3 n) {
+  // pack2x16float(octEncode(n)) (gpu_screen_space_normals.wgsl:36-42)
+  const float sum = (std::fabs(n.x) + std::fabs(n.y)) + std::fabs(n.z);
+  float px = n.x / sum, py = n.y / sum;
+  if (n.z < 0.0f) {
+    const float qx = (1.0f - std::fabs(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+    const float qy = (1.0f - std::fabs(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+    px = qx;
+    py = qy;
+  }
+  return hdsr_pack2x16(px, py);
+}
+static D3 face_fwd(D3 n0, D3 v0) {
+  const D3 n = d3_norm(n0), v = d3_norm(v0);
+  return d3_dot(n, v) < 0.0f ? D3{-n.x, -n.y, -n.z} : n;
+}
+
+void orc_mat4_inverse(const float* m32, float* o, int* ok) {
+  // mat4Inverse (src/splatRenderer.ts:148-168): double cofactors, fp32 result
+  double a[16];
+  for (int i = 0; i < 16; ++i) a[i] = m32[i];
+  const double b00 = a[0] * a[5] - a[1] * a[4], b01 = a[0] * a[6] - a[2] * a[4], b02 = a[0] * a[7] - a[3] * a[4];
+  const double b03 = a[1] * a[6] - a[2] * a[5], b04 = a[1] * a[7] - a[3] * a[5], b05 = a[2] * a[7] - a[3] * a[6];
+  const double b06 = a[8] * a[13] - a[9] * a[12], b07 = a[8] * a[14] - a[10] * a[12];
+  const double b08 = a[8] * a[15] - a[11] * a[12], b09 = a[9] * a[14] - a[10] * a[13];
+  const double b10 = a[9] * a[15] - a[11] * a[13], b11 = a[10] * a[15] - a[11] * a[14];
+  double det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+  *ok = !(std::fabs(det) < 1e-10);
+  if (!*ok) return;
+  det = 1 / det;
+  const double r[16] = {
+      (a[5] * b11 - a[6] * b10 + a[7] * b09) * det,   (a[2] * b10 - a[1] * b11 - a[3] * b09) * det,
+      (a[13] * b05 - a[14] * b04 + a[15] * b03) * det, (a[10] * b04 - a[9] * b05 - a[11] * b03) * det,
+      (a[6] * b08 - a[4] * b11 - a[7] * b07) * det,   (a[0] * b11 - a[2] * b08 + a[3] * b07) * det,
+      (a[14] * b02 - a[12] * b05 - a[15] * b01) * det, (a[8] * b05 - a[10] * b02 + a[11] * b01) * det,
+      (a[4] * b10 - a[5] * b08 + a[7] * b06) * det,   (a[1] * b08 - a[0] * b10 - a[3] * b06) * det,
+      (a[12] * b04 - a[13] * b02 + a[15] * b00) * det, (a[9] * b02 - a[8] * b04 - a[11] * b00) * det,
+      (a[5] * b07 - a[4] * b09 - a[6] * b06) * det,   (a[0] * b09 - a[1] * b07 + a[2] * b06) * det,
+      (a[13] * b01 - a[12] * b03 - a[14] * b00) * det, (a[8] * b03 - a[9] * b01 + a[10] * b00) * det};
+  for (int i = 0; i < 16; ++i) o[i] = (float)r[i];
+}
+
+void orc_screen_space_normals(const float* depth, const uint32_t* src, uint32_t W, uint32_t H, const float* M,
+                              const float* cam, uint32_t mode_u, float g0, float g1, uint32_t* out) {
+  const float mode = (float)mode_u, vw = (float)W, vh = (float)H;
+  auto recon = [&](float px, float py, float d) {
+    // reconstructViewPos (gpu_screen_space_normals.wgsl:26-34)
+    const float nx = px / vw * 2.0f - 1.0f, ny = 1.0f - py / vh * 2.0f;
+    const float x = ((M[0] * nx + M[4] * ny) + M[8] * d) + M[12] * 1.0f;
+    const float y = ((M[1] * nx + M[5] * ny) + M[9] * d) + M[13] * 1.0f;
+    const float z = ((M[2] * nx + M[6] * ny) + M[10] * d) + M[14] * 1.0f;
+    const float w = ((M[3] * nx + M[7] * ny) + M[11] * d) + M[15] * 1.0f;
+    return D3{x / w, y / w, z / w};
+  };
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      const size_t o = (size_t)y * W + x;
+      const float d = depth[o];
+      if (d >= 0.9999f) {
+        out[o] = mode > 0.5f ? src[o] : hdsr_pack2x16(0.0f, 0.0f);
+        continue;
+      }
+      const float cx = (float)x + 0.5f, cy = (float)y + 0.5f;
+      const D3 pc = recon(cx, cy, d);
+      const float dL = depth[(size_t)y * W + (x > 0 ? x - 1 : 0)];
+      const float dR = depth[(size_t)y * W + std::min(x + 1, W - 1)];
+      const float dU = depth[(size_t)(y > 0 ? y - 1 : 0) * W + x];
+      const float dD = depth[(size_t)std::min(y + 1, H - 1) * W + x];
+      D3 ddx, ddy;
+      if (std::fabs(dL - d) < std::fabs(dR - d)) {
+        const D3 q = recon(cx - 1.0f, cy, dL);
+        ddx = D3{pc.x - q.x, pc.y - q.y, pc.z - q.z};
+      } else {
+        const D3 q = recon(cx + 1.0f, cy, dR);
+        ddx = D3{q.x - pc.x, q.y - pc.y, q.z - pc.z};
+      }
+      if (std::fabs(dU - d) < std::fabs(dD - d)) {
+        const D3 q = recon(cx, cy - 1.0f, dU);
+        ddy = D3{pc.x - q.x, pc.y - q.y, pc.z - q.z};
+      } else {
+        const D3 q = recon(cx, cy + 1.0f, dD);
+        ddy = D3{q.x - pc.x, q.y - pc.y, q.z - pc.z};
+      }
+      const D3 vd = d3_norm(D3{cam[0] - pc.x, cam[1] - pc.y, cam[2] - pc.z});
+      const D3 cr{ddy.y * ddx.z - ddy.z * ddx.y, ddy.z * ddx.x - ddy.x * ddx.z, ddy.x * ddx.y - ddy.y * ddx.x};
+      const D3 screen = face_fwd(d3_norm(cr), vd);
+      const D3 source = face_fwd(oct_dec(src[o]), vd);
+      D3 nrm = screen;
+      if (mode > 1.5f) {
+        nrm = source;
+      } else if (mode > 0.5f) {
+        const float sdv = std::fmax(d3_dot(source, vd), 0.0f);
+        float t = (sdv - g0) / (g1 - g0);
+        t = std::fmin(std::fmax(t, 0.0f), 1.0f);
+        const float blend = 1.0f - t * t * (3.0f - 2.0f * t);
+        nrm = d3_norm(D3{source.x * (1.0f - blend) + screen.x * blend, source.y * (1.0f - blend) + screen.y * blend,
+                         source.z * (1.0f - blend) + screen.z * blend});
+      }
+      out[o] = oct_enc(nrm);
+    }
+}
+
+void orc_bilateral_normals(const uint32_t* in, const float* depth, uint32_t W, uint32_t H, float ss, float dsg,
+                           float nsg, int radius, uint32_t* out) {
+  for (int y = 0; y < (int)H; ++y)
+    for (int x = 0; x < (int)W; ++x) {
+      const size_t o = (size_t)y * W + x;
+      const D3 cn = oct_dec(in[o]);
+      const float cd = depth[o];
+      if (cd >= 0.9999f) {
+        out[o] = in[o];
+        continue;
+      }
+      D3 acc{0.0f, 0.0f, 0.0f};
+      float tw = 0.0f;
+      for (int dy = -radius; dy <= radius; ++dy)
+        for (int dx = -radius; dx <= radius; ++dx) {
+          const int sx = std::min(std::max(x + dx, 0), (int)W - 1), sy = std::min(std::max(y + dy, 0), (int)H - 1);
+          const size_t so = (size_t)sy * W + sx;
+          const D3 sn = oct_dec(in[so]);
+          const float sd = depth[so];
+          const float dist2 = (float)(dx * dx + dy * dy);
+          const float sw = hdsr_expf(-dist2 / (2.0f * ss * ss));
+          const float diff = std::fabs(cd - sd) / std::fmax(cd, 0.001f);
+          const float dw = hdsr_expf(-diff * diff / (2.0f * dsg * dsg));
+          const float nd = std::fmax(d3_dot(cn, sn), 0.0f);
+          const float nw = hdsr_expf((1.0f / std::fmax(nsg, 0.01f)) * hdsr_logf(nd));
+          const float wgt = sw * dw * nw;
+          acc = D3{acc.x + sn.x * wgt, acc.y + sn.y * wgt, acc.z + sn.z * wgt};
+          tw = tw + wgt;
+        }
+      const float m = std::fmax(tw, 0.0001f);
+      out[o] = oct_enc(d3_norm(D3{acc.x / m, acc.y / m, acc.z / m}));
+    }
+}
+
 void orc_f16_round(const double* x, double* out, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) out[i] = f16r(x[i]);
 }

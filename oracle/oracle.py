@@ -69,12 +69,81 @@ def load() -> C.CDLL:
     lib.orc_f16_to_f32.argtypes = [C.c_uint32]
     lib.orc_f16_round.restype = None
     lib.orc_f16_round.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+    lib.orc_mat4_inverse.restype = None
+    lib.orc_mat4_inverse.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+    lib.orc_screen_space_normals.restype = None
+    lib.orc_screen_space_normals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                             C.c_uint32, C.c_float, C.c_float, C.c_void_p]
+    lib.orc_bilateral_normals.restype = None
+    lib.orc_bilateral_normals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                          C.c_float, C.c_int, C.c_void_p]
     lib.orc_u32sat.restype = C.c_uint32
     lib.orc_u32sat.argtypes = [C.c_float]
     lib.orc_morton2.restype = C.c_uint32
     lib.orc_morton2.argtypes = [C.c_uint32, C.c_uint32]
     _lib = lib
     return lib
+
+
+def mat4_inverse(m16):
+    """mat4Inverse (src/splatRenderer.ts:148-168): (fp32 inverse, True), or (None, False) when
+    |det| < 1e-10."""
+    m = np.ascontiguousarray(m16, np.float32)
+    out = np.zeros(16, np.float32)
+    ok = C.c_int(0)
+    load().orc_mat4_inverse(m.ctypes.data, out.ctypes.data, C.byref(ok))
+    return (out, True) if ok.value else (None, False)
+
+
+def screen_space_normals(depth, src_words, inv16, cam3, mode=0, grazing_start=0.45, grazing_end=0.80):
+    """gpu_screen_space_normals.wgsl over an (H, W) depth image and normal words."""
+    d = np.ascontiguousarray(depth, np.float32)
+    s = np.ascontiguousarray(src_words, np.uint32)
+    H, W = d.shape
+    inv = np.ascontiguousarray(inv16, np.float32)
+    cam = np.ascontiguousarray(cam3, np.float32)
+    out = np.zeros((H, W), np.uint32)
+    load().orc_screen_space_normals(d.ctypes.data, s.ctypes.data, W, H, inv.ctypes.data, cam.ctypes.data, mode,
+                                    grazing_start, grazing_end, out.ctypes.data)
+    return out
+
+
+def bilateral_normals(words, depth, spatial_sigma=3.0, depth_sigma=0.05, normal_sigma=1.5, radius=3):
+    """bilateral_normal_filter.wgsl over (H, W) normal words and depth."""
+    w = np.ascontiguousarray(words, np.uint32)
+    d = np.ascontiguousarray(depth, np.float32)
+    H, W = d.shape
+    out = np.zeros((H, W), np.uint32)
+    load().orc_bilateral_normals(w.ctypes.data, d.ctypes.data, W, H, spatial_sigma, depth_sigma, normal_sigma, radius,
+                                 out.ctypes.data)
+    return out
+
+
+def oct_words_to_normals(words) -> np.ndarray:
+    """Unpack oct-encoded normal words to (..., 3) float64 unit vectors (for checks)."""
+    w = np.asarray(words, np.uint32)
+    e = np.stack([(w & 0xFFFF).astype(np.uint16).view(np.float16), (w >> 16).astype(np.uint16).view(np.float16)],
+                 -1).astype(np.float64)
+    n = np.concatenate([e, (1 - np.abs(e[..., :1]) - np.abs(e[..., 1:]))], -1)
+    neg = n[..., 2] < 0
+    fx = (1 - np.abs(n[..., 1])) * np.where(n[..., 0] >= 0, 1, -1)
+    fy = (1 - np.abs(n[..., 0])) * np.where(n[..., 1] >= 0, 1, -1)
+    n[..., 0] = np.where(neg, fx, n[..., 0])
+    n[..., 1] = np.where(neg, fy, n[..., 1])
+    return n / np.linalg.norm(n, axis=-1, keepdims=True)
+
+
+def normals_to_oct_words(n) -> np.ndarray:
+    """octEncode + pack2x16float (round to nearest even) of (..., 3) unit vectors."""
+    n = np.asarray(n, np.float64)
+    s = np.abs(n).sum(-1, keepdims=True)
+    p = n[..., :2] / s
+    neg = n[..., 2] < 0
+    q0 = (1 - np.abs(p[..., 1])) * np.where(p[..., 0] >= 0, 1, -1)
+    q1 = (1 - np.abs(p[..., 0])) * np.where(p[..., 1] >= 0, 1, -1)
+    p = np.stack([np.where(neg, q0, p[..., 0]), np.where(neg, q1, p[..., 1])], -1).astype(np.float32)
+    h = p.astype(np.float16).view(np.uint16).astype(np.uint32)
+    return h[..., 0] | (h[..., 1] << 16)
 
 
 def f16_round(x) -> np.ndarray:
