@@ -25,7 +25,7 @@ var addon = require(path.join(__dirname, 'hdsr_napi.node'));
 var TILE_SPLAT_FRAME_UNIFORM_BYTES = 256;
 
 var FLAGS = { f16: 1, refCaps: 2, gbuffer: 4, debug: 8, rgba16: 16, profile: 32 };
-var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4 };
+var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4, filteredNormal: 7 };
 var DBG = {
   projCache: 0, depth: 1, tileCounts: 2, tileOffsets: 3, entries: 4, entriesCanon: 5, smallTileList: 6,
   largeTileList: 7, largeTileOverflowBases: 8, chunkRanges: 9, keys20: 10, largeSplatList: 11, sortedIndices: 12,
@@ -134,6 +134,19 @@ function encodeViewDepthSort(encoder, resources) {
   addon.sortBackToFront(resources.handle);
 }
 
+// The first deferred consumers of the G-buffer (src/splatRenderer.ts:1042-1069): screen-space
+// normal reconstruction into the filtered-normal buffer, then the bilateral filter back into the
+// G-buffer normal words.  options: recoveryMode (0 screen-space, 1 grazing recovery, 2 source
+// pass-through), grazingStart / grazingEnd, spatialSigma / depthSigma / normalSigma /
+// kernelRadius; unset fields take the reference's values.
+function encodeScreenSpaceNormals(encoder, resources, options) {
+  addon.screenSpaceNormals(resources.handle, options || {});
+}
+
+function encodeBilateralNormalFilter(encoder, resources, options) {
+  addon.bilateralNormalFilter(resources.handle, options || {});
+}
+
 function encodeCompositeOnly(encoder, resources, bindGroups) {
   addon.compositeOnly(resources.handle);
 }
@@ -170,6 +183,8 @@ module.exports = {
   encodeFullComputeCompositorPipeline: encodeFullComputeCompositorPipeline,
   encodeCompositeOnly: encodeCompositeOnly,
   encodeViewDepthSort: encodeViewDepthSort,
+  encodeScreenSpaceNormals: encodeScreenSpaceNormals,
+  encodeBilateralNormalFilter: encodeBilateralNormalFilter,
   createDevice: createDevice,
   version: addon.version,
 };

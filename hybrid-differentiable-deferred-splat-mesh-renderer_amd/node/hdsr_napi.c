@@ -535,6 +535,44 @@ static napi_value js_ply_decode(napi_env env, napi_callback_info info) {
   return res;
 }
 
+/* The reference's deferred-normal parameters, overridden by the fields a JS object sets. */
+static void normal_params_from(napi_env env, napi_value obj, hdsr_normal_params* p) {
+  hdsr_normal_params_default(p);
+  p->recovery_mode = get_u32(env, obj, "recoveryMode", p->recovery_mode);
+  p->grazing_start = (float)get_f64(env, obj, "grazingStart", p->grazing_start);
+  p->grazing_end = (float)get_f64(env, obj, "grazingEnd", p->grazing_end);
+  p->spatial_sigma = (float)get_f64(env, obj, "spatialSigma", p->spatial_sigma);
+  p->depth_sigma = (float)get_f64(env, obj, "depthSigma", p->depth_sigma);
+  p->normal_sigma = (float)get_f64(env, obj, "normalSigma", p->normal_sigma);
+  p->kernel_radius = get_u32(env, obj, "kernelRadius", p->kernel_radius);
+}
+
+static napi_value js_screen_space_normals(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  hdsr_normal_params p;
+  normal_params_from(env, argv[1], &p);
+  const int rc = hdsr_screen_space_normals(ctx, &p, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_screen_space_normals");
+  return NULL;
+}
+
+static napi_value js_bilateral_normal_filter(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  hdsr_normal_params p;
+  normal_params_from(env, argv[1], &p);
+  const int rc = hdsr_bilateral_normal_filter(ctx, &p, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_bilateral_normal_filter");
+  return NULL;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
   (void)info;
   napi_value s;
@@ -560,6 +598,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"stageTimes", NULL, js_stage_times, NULL, NULL, NULL, napi_default, NULL},
       {"version", NULL, js_version, NULL, NULL, NULL, napi_default, NULL},
       {"plyDecode", NULL, js_ply_decode, NULL, NULL, NULL, napi_default, NULL},
+      {"screenSpaceNormals", NULL, js_screen_space_normals, NULL, NULL, NULL, napi_default, NULL},
+      {"bilateralNormalFilter", NULL, js_bilateral_normal_filter, NULL, NULL, NULL, napi_default, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;

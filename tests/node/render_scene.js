@@ -11,7 +11,7 @@ function f32(name) { var b = fs.readFileSync(path.join(dir, name)); return new F
 function u32(name) { var b = fs.readFileSync(path.join(dir, name)); return new Uint32Array(b.buffer, b.byteOffset, b.length / 4); }
 var device = C.createDevice();
 var plan = C.planTileSplatCompositor({ viewportWidth: meta.width, viewportHeight: meta.height, tileSizePx: 16, splatCount: meta.count });
-var resources = C.createTileSplatCompositor(device, plan, { debug: true });
+var resources = C.createTileSplatCompositor(device, plan, { debug: true, gbuffer: !!meta.gbuffer });
 var bindGroups = C.createTileSplatBindGroups(device, resources, {
   positionBuffer: f32('positions.bin'), scaleBuffer: f32('scales.bin'), rotationBuffer: f32('rotations.bin'),
   opacityBuffer: f32('opacities.bin'), materialBuffer: u32('materials.bin'), shDataBuffer: f32('shdata.bin'),
@@ -28,5 +28,15 @@ fs.writeFileSync(path.join(dir, 'rgba_js.bin'), Buffer.from(resources.readRGBA()
 fs.writeFileSync(path.join(dir, 'depth_js.bin'), Buffer.from(resources.readDepth().buffer));
 fs.writeFileSync(path.join(dir, 'entries_js.bin'), Buffer.from(resources.debug('entries').buffer));
 fs.writeFileSync(path.join(dir, 'keys_js.bin'), Buffer.from(resources.debug('keys20').buffer));
+if (meta.gbuffer) {
+  // the deferred consumers of the G-buffer, as the reference's encode* calls after the composite
+  var px = meta.width * meta.height;
+  fs.writeFileSync(path.join(dir, 'gbn_js.bin'), Buffer.from(resources.readOutput('gbufferNormal', new Uint32Array(px)).buffer));
+  var opts = { recoveryMode: meta.recoveryMode };
+  C.encodeScreenSpaceNormals(encoder, resources, opts);
+  fs.writeFileSync(path.join(dir, 'ssn_js.bin'), Buffer.from(resources.readOutput('filteredNormal', new Uint32Array(px)).buffer));
+  C.encodeBilateralNormalFilter(encoder, resources, opts);
+  fs.writeFileSync(path.join(dir, 'bil_js.bin'), Buffer.from(resources.readOutput('gbufferNormal', new Uint32Array(px)).buffer));
+}
 console.log(JSON.stringify(resources.stats()));
 resources.destroy();

@@ -64,6 +64,14 @@ def test_screen_space_normals_of_a_plane():
     out = oracle.screen_space_normals(depth, src, inv, eye, mode=2)
     got = oracle.oct_words_to_normals(out)
     assert np.degrees(np.arccos(np.clip(got @ facing, -1, 1))).max() < 0.2
+    # recovery mode: grazing bounds that force the smoothstep to 1 keep the source normal, bounds that
+    # force it to 0 give the screen-space normal (a swapped blend fails one of the two)
+    keep = oracle.screen_space_normals(depth, src, inv, eye, mode=1, grazing_start=-2.0, grazing_end=-1.0)
+    np.testing.assert_array_equal(keep, oracle.screen_space_normals(depth, src, inv, eye, mode=2))
+    rec = oracle.oct_words_to_normals(oracle.screen_space_normals(depth, src, inv, eye, mode=1,
+                                                                 grazing_start=2.0, grazing_end=3.0))
+    ang = np.degrees(np.arccos(np.clip(rec[1:-1, 1:-1] @ facing, -1, 1)))
+    assert np.median(ang) < 0.1 and ang.max() < 1.0
     # background pixels: a zero word in screen-space mode, the source word in recovery modes
     depth[0, 0] = 1.0
     assert oracle.screen_space_normals(depth, src, inv, eye, mode=0)[0, 0] == 0

@@ -51,7 +51,10 @@ def test_facade_cpu():
 
 
 @pytest.mark.gpu
-def test_facade_renders_like_oracle(hip_device):
+@pytest.mark.parametrize("gbuffer", [False, True])
+def test_facade_renders_like_oracle(hip_device, gbuffer):
+    """One frame through the JS facade in the reference's call order; with the G-buffer on, the
+    screen-space normal and bilateral passes run after it through the same facade."""
     s = scenes.scene_config0(n=12_000, seed=21)
     cam = camera.Camera(camera.orbit_eye(3.0, -0.6, 0.25), (0, 0, 0), 600.0, 600.0, 480, 320)
     plan = compositor.planTileSplatCompositor({"viewportWidth": 480, "viewportHeight": 320, "tileSizePx": 16,
@@ -65,8 +68,8 @@ def test_facade_renders_like_oracle(hip_device):
                   "view.bin": cam.view, "proj.bin": cam.proj}
         for k, v in arrays.items():
             np.ascontiguousarray(v).tofile(os.path.join(d, k))
-        json.dump({"width": 480, "height": 320, "count": s.count, "shDegree": s.sh_degree},
-                  open(os.path.join(d, "meta.json"), "w"))
+        json.dump({"width": 480, "height": 320, "count": s.count, "shDegree": s.sh_degree, "gbuffer": gbuffer,
+                   "recoveryMode": 1}, open(os.path.join(d, "meta.json"), "w"))
         st = run_node("render_scene.js", d)
         uj = np.fromfile(os.path.join(d, "uniforms_js.bin"), np.uint32)
         np.testing.assert_array_equal(uj, u.view(np.uint32))
@@ -78,6 +81,15 @@ def test_facade_renders_like_oracle(hip_device):
         assert np.abs(rgba - orc["rgba"]).max() <= 1e-5
         depth = np.fromfile(os.path.join(d, "depth_js.bin"), np.float32)
         assert np.abs(depth - orc["out_depth"]).max() <= 1e-5
+        if gbuffer:
+            gbn = np.fromfile(os.path.join(d, "gbn_js.bin"), np.uint32).reshape(320, 480)
+            inv, ok = oracle.mat4_inverse(u[:16])
+            assert ok
+            want = oracle.screen_space_normals(depth.reshape(320, 480), gbn, inv, u[28:31], mode=1)
+            ssn = np.fromfile(os.path.join(d, "ssn_js.bin"), np.uint32).reshape(320, 480)
+            np.testing.assert_array_equal(ssn, want)
+            bil = np.fromfile(os.path.join(d, "bil_js.bin"), np.uint32).reshape(320, 480)
+            np.testing.assert_array_equal(bil, oracle.bilateral_normals(want, depth.reshape(320, 480)))
 
 
 def test_node_asset_loaders_match_python(tmp_path):
