@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-f16", action="store_true", help="skip the f16-composite run of the same frames")
     ap.add_argument("--no-deferred", action="store_true", help="skip the deferred-normal passes")
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiation workload")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 6M-splat 3840x2160 stress workload")
+    ap.add_argument("--stress-only", action="store_true", help="run only the stress workload")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -107,6 +109,9 @@ def main() -> int:
         return 0
     if a.diff_only:
         print(json.dumps({"diff": diff_bench()}))
+        return 0
+    if a.stress_only:
+        print(json.dumps({"stress": stress_bench()}))
         return 0
 
     t_setup = time.time()
@@ -209,6 +214,7 @@ def main() -> int:
     deferred = None if a.no_deferred else deferred_bench(scene, plan, sorted_host, uniforms, handle)
     serving = None if a.no_serving else serving_bench()
     diff = None if a.no_diff else diff_bench()
+    stress = None if a.no_stress else stress_bench()
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(scene, sorted_host[0], uniforms[0], W, H, plan, a.cpu_budget_s)
@@ -248,9 +254,91 @@ def main() -> int:
         "deferred_normals": deferred,
         "serving": serving,
         "diff": diff,
+        "stress": stress,
     }
     print(json.dumps(out))
     return 0
+
+
+def stress_bench(views=16, steps=2, seed=2) -> dict:
+    """Stress workload: 6M SH3 splats with half of them inside a 0.2-unit ball one unit in front
+    of the first camera (tiles of more than 100k entries), 16 orbit views at 3840x2160, fx=fy=1500.
+    Each view's back-to-front order is made on the host beforehand and resident in HBM.  Reports
+    frames/s, per-stage times with algorithmic GB/s, and keys per second of the two orderings: the
+    tile-key order (tile-list build: emit + group + ranges) and the per-tile depth order (bucket
+    cascade + segment sort)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+
+    from hdsr import compositor, scenes
+    from hdsr.camera import frame_uniforms_for
+    scene = scenes.scene_config2(6_000_000, seed)
+    cams = scenes.cameras_config2()[:views]
+    W, H = cams[0].width, cams[0].height
+    plan = compositor.planTileSplatCompositor({"viewportWidth": W, "viewportHeight": H, "tileSizePx": 16,
+                                               "splatCount": scene.count})
+    with ThreadPoolExecutor(8) as ex:
+        sorted_host = list(ex.map(lambda c: scenes.sort_back_to_front(scene.positions, c.view), cams))
+    uniforms = [frame_uniforms_for(c, plan, scene.sh_degree) for c in cams]
+    dev = torch.device("cuda:0")
+    sorted_dev = [torch.from_numpy(x.astype(np.int32)).to(dev) for x in sorted_host]
+    res = compositor.createTileSplatCompositor(None, plan, profile=True)
+    compositor.bind_scene(res, scene, sorted_host[0])
+    handle = torch.cuda.current_stream().cuda_stream
+
+    def render(v):
+        res.update_sorted_indices(sorted_dev[v])
+        res.set_frame(uniforms[v])
+        compositor.encodeFullComputeCompositorPipeline(handle, res)
+
+    view_stats = []
+    for v in range(len(cams)):
+        render(v)
+        res.sync()
+        view_stats.append(res.stats())
+        print(f"stress view {v}: {view_stats[-1]['refs']} refs", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    res.stage_times()
+    reruns0 = res.stats()["overflow_reruns"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for v in range(len(cams)):
+            render(v)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res.sync()
+    if res.stats()["overflow_reruns"] != reruns0:
+        raise RuntimeError("capacity overflow inside the timed stress region")
+    stage_ms, _ = res.stage_times()
+    frames = steps * len(cams)
+    mean_st = {key: float(np.mean([x[key] for x in view_stats])) for key in view_stats[0]}
+    k = (scene.sh_degree + 1) ** 2 - 1
+    cells = -(-plan["tileColumns"] // 8) * -(-plan["tileRows"] // 8)
+    ckb = max(1, (cells - 1).bit_length())
+    per_stage = {}
+    for name, ms in stage_ms.items():
+        byts = algorithmic_bytes(name, scene.count, k, mean_st, W * H, mean_st["composite_refs"], ckb)
+        gbs = (byts / (ms * 1e-3) / 1e9) if ms > 0 and byts else None
+        per_stage[name] = {"ms": round(ms, 3), "GBps": None if gbs is None else round(gbs, 1)}
+    R = mean_st["refs"]
+    order_ms = stage_ms["emit"] + stage_ms["group"] + stage_ms["ranges"]
+    depth_ms = stage_ms["bucket"] + stage_ms["sort"]
+    res.destroy()
+    return {
+        "workload": f"6M-splat SH3 stress scene (half in a 0.2-unit ball before camera 0), {len(cams)} orbit views "
+                    f"at {W}x{H}, fx=fy=1500",
+        "frames_per_s": round(frames / wall, 2),
+        "ms_per_frame": round(wall * 1e3 / frames, 3),
+        "refs_per_frame": round(R),
+        "large_tile_refs_per_frame": round(mean_st["large_tile_refs"]),
+        "chunks_per_frame": round(mean_st["chunks"]),
+        "tile_key_order_keys_per_s": round(R / (order_ms * 1e-3)),
+        "depth_order_keys_per_s": round(R / (depth_ms * 1e-3)),
+        "scatter_GBps": per_stage["ranges"]["GBps"],
+        "peak_GBps": HBM_PEAK_GBS,
+        "stages_ms_per_frame": per_stage,
+    }
 
 
 def f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames) -> dict:
