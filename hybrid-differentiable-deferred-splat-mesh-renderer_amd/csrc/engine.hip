@@ -86,7 +86,7 @@ void Engine::release() {
                   b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
-                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_, d_starts_};
+                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b_ = Bufs{};
@@ -251,13 +251,15 @@ int Engine::ensure(const Layout& L, bool debug) {
     size_t h = 0;
     if ((rc = grow(&b_.stats, &h, 1))) return rc;
   }
-  if (L.num_frames > n_frames_ || !d_frames_) {
+  // the frame table and its three start arrays (uint32, num_frames + 1 each) share one allocation,
+  // laid out as the host stages them, so a batch's upload is one copy
+  const size_t slots = L.num_frames + (12 * ((size_t)L.num_frames + 1) + sizeof(FrameDesc) - 1) / sizeof(FrameDesc);
+  if (slots > n_frames_ || !d_frames_) {
     have = n_frames_;
-    if ((rc = grow(&d_frames_, &have, L.num_frames))) return rc;
-    have = n_frames_;
-    if ((rc = grow(&d_starts_, &have, 3 * (L.num_frames + 1)))) return rc;
-    n_frames_ = L.num_frames;
+    if ((rc = grow(&d_frames_, &have, slots))) return rc;
+    n_frames_ = slots;
   }
+  d_starts_ = reinterpret_cast<uint32_t*>(d_frames_ + L.num_frames);
   if (!hstats_) {
     if (hipHostMalloc((void**)&hstats_, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
@@ -320,8 +322,7 @@ int Engine::upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t 
     st[L.num_frames + 1 + v] = L.tile_start[v];
     st[2 * (L.num_frames + 1) + v] = L.cell_start[v];
   }
-  if (hipMemcpyAsync(d_frames_, s, fb, hipMemcpyHostToDevice, stream) != hipSuccess ||
-      hipMemcpyAsync(d_starts_, st, 12 * (L.num_frames + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
+  if (hipMemcpyAsync(d_frames_, s, fb + 12 * (L.num_frames + 1), hipMemcpyHostToDevice, stream) != hipSuccess ||
       hipEventRecord(stage_ev_[slot], stream) != hipSuccess) {
     err = "frame table upload failed";
     return HDSR_E_HIP;
@@ -425,8 +426,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
       err = "hipStreamBeginCapture failed";
       return HDSR_E_HIP;
     }
-    (void)hipMemcpyAsync(d_frames_, st, fb, hipMemcpyHostToDevice, stream);
-    (void)hipMemcpyAsync(d_starts_, tab, sb, hipMemcpyHostToDevice, stream);
+    (void)hipMemcpyAsync(d_frames_, st, fb + sb, hipMemcpyHostToDevice, stream);
     enqueue_stages(b, stream, gbuffer, false, nullptr);
     (void)hipMemcpyAsync(hstats_, b.stats, sizeof(DevStats), hipMemcpyDeviceToHost, stream);
     if (hipStreamEndCapture(stream, &graph) != hipSuccess || !graph) {

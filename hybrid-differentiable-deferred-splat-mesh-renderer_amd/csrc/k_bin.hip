@@ -34,6 +34,7 @@ struct Ops<uint32_t> {
   __device__ static uint32_t zero() { return 0u; }
   __device__ static uint32_t up(uint32_t v, int d) { return __shfl_up(v, d, 64); }
   __device__ static uint32_t add(uint32_t a, uint32_t b) { return a + b; }
+  __device__ static uint32_t sub(uint32_t a, uint32_t b) { return a - b; }
 };
 template <>
 struct Ops<uint4> {
@@ -42,6 +43,7 @@ struct Ops<uint4> {
     return make_uint4(__shfl_up(v.x, d, 64), __shfl_up(v.y, d, 64), __shfl_up(v.z, d, 64), __shfl_up(v.w, d, 64));
   }
   __device__ static uint4 add(uint4 a, uint4 b) { return a + b; }
+  __device__ static uint4 sub(uint4 a, uint4 b) { return make_uint4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
 };
 
 // Exclusive scan over a 256-thread workgroup of one value per thread; *total gets the sum.
@@ -70,6 +72,66 @@ __device__ __forceinline__ T block_excl_256(T v, T* total) {
   __syncthreads();
   *total = tot;
   return Ops<T>::add(base, exc);
+}
+
+template <typename T>
+__device__ __forceinline__ T readlane63(T v);
+template <>
+__device__ __forceinline__ uint32_t readlane63(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// One 1024-thread workgroup scans up to kOneGroupScan elements: the tile-count scan of a frame
+// then takes one launch instead of the hierarchical path's three.
+constexpr uint32_t kOneGroupScan = 16384;
+
+// Exclusive scan of in[0..n) into out (n <= kWaves * 1024) by a workgroup of kWaves waves;
+// returns the total.  Wave w owns the contiguous span [w*S, (w+1)*S), S a multiple of 64, read in
+// coalesced rounds of 64: a first read sums the span (all loads in flight together), the spans'
+// sums give every wave its base, and a second read (from L2) scans each round across the wave
+// with a running carry.
+template <typename T, uint32_t kWaves>
+__device__ __forceinline__ T one_group_scan(const T* in, T* out, uint32_t n) {
+  __shared__ T wsum[kWaves];
+  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+  const uint32_t S = ((n + kWaves * 64u - 1u) / (kWaves * 64u)) * 64u, rounds = S / 64u;
+  T part = Ops<T>::zero();
+#pragma unroll
+  for (uint32_t r = 0; r < 16; ++r) {
+    const uint32_t i = w * S + r * 64u + lane;
+    if (r < rounds && i < n) part = Ops<T>::add(part, in[i]);
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {  // inclusive scan across the wave: lane 63 holds the span's sum
+    const T t = Ops<T>::up(part, d);
+    if (lane >= (uint32_t)d) part = Ops<T>::add(part, t);
+  }
+  if (lane == 63) wsum[w] = part;
+  __syncthreads();
+  T carry = Ops<T>::zero(), tot = Ops<T>::zero();
+#pragma unroll
+  for (uint32_t q = 0; q < kWaves; ++q) {
+    const T x = wsum[q];
+    if (q < w) carry = Ops<T>::add(carry, x);
+    tot = Ops<T>::add(tot, x);
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < 16; ++r) {
+    if (r < rounds) {  // uniform
+      const uint32_t i = w * S + r * 64u + lane;
+      const T o = i < n ? in[i] : Ops<T>::zero();
+      T inc = o;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const T t = Ops<T>::up(inc, d);
+        if (lane >= (uint32_t)d) inc = Ops<T>::add(inc, t);
+      }
+      if (i < n) out[i] = Ops<T>::add(carry, Ops<T>::sub(inc, o));
+      carry = Ops<T>::add(carry, readlane63(inc));
+    }
+  }
+  __syncthreads();  // wsum may be reused by the next call
+  return tot;
 }
 
 // ---- generic hierarchical exclusive scan: 4096 elements per workgroup (256 threads x 16) ----
@@ -160,7 +222,11 @@ __global__ void k_clear_stats(DevStats* s) {
   if (threadIdx.x < sizeof(DevStats) / 4) p[threadIdx.x] = 0;
 }
 
+// One thread: clears the statistics (no earlier stage of the frame writes them) and stores the
+// totals of the block scans.
 __global__ void k_totals(Bufs b) {
+  uint32_t* st = reinterpret_cast<uint32_t*>(b.stats);
+  for (uint32_t i = 0; i < sizeof(DevStats) / 4; ++i) st[i] = 0;
   const uint32_t last = b.splat_blocks - 1;
   const uint4 t = b.block_pre[last] + b.block_cnt[last];
   const uint4 c = b.block_ccpre[last] + b.block_cc[last];
@@ -206,7 +272,7 @@ __global__ __launch_bounds__(256) void k_emit(Bufs b) {
   uint4 tot;
   const uint4 bp = b.block_ccpre[blk];
   const uint4 pre = block_excl_256<uint4>(v, &tot) + make_uint4(bp.x, bp.y, b.block_pre[blk].z, 0u);
-  if (c.z) b.large_list[pre.z] = g;
+  if (c.z && pre.z < b.splat_blocks * kSplatBlock) b.large_list[pre.z] = g;
   const uint32_t small_total = b.block_ccpre[b.splat_blocks - 1].x + b.block_cc[b.splat_blocks - 1].x;
   const uint32_t lane = __lane_id();
   const uint32_t g0 = g - lane;
@@ -369,8 +435,14 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, cons
 // tile's offset + the chunk's prefix + the waves before + the rank: the canonical order of the
 // cell list is kept inside every tile.
 
+// Also resets the per-tile counts and depth ranges that the counting walk accumulates into.
 __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t i = c; i < b.total_morton; i += gridDim.x * blockDim.x) {
+    b.tile_counts[i] = 0u;
+    b.tile_dmin[i] = 0xFFFFFFFFu;
+    b.tile_dmax[i] = 0u;
+  }
   if (c >= b.total_cells) return;
   const uint32_t n = b.stats->cell_pairs;
   uint32_t lo = 0, hi = n;
@@ -389,14 +461,12 @@ __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
   b.cell_nchunk[c] = (lo - s + kCellChunk - 1) / kCellChunk;
 }
 
-__global__ void k_chunk_total(Bufs b) {
+// Expansion workgroups of the frame batch: sum over cells of ceil(count / kCellChunk).
+__device__ __forceinline__ uint32_t cell_chunk_total(const Bufs& b) {
   const uint32_t last = b.total_cells - 1;
-  b.stats->cell_chunks = b.cell_chunk0[last] + b.cell_nchunk[last];
+  return b.cell_chunk0[last] + b.cell_nchunk[last];
 }
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
-}
 
 struct ChunkInfo {
   uint32_t cell, first, count, cx, cy;
@@ -446,7 +516,9 @@ __global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) 
   __shared__ uint32_t wcnt[4][64];
   __shared__ uint32_t s_dmin[64], s_dmax[64];
   const uint32_t q = blockIdx.x;
-  if (q >= b.stats->cell_chunks) return;  // uniform
+  const uint32_t nchunks = cell_chunk_total(b);
+  if (q >= nchunks) return;  // uniform
+  if (q == 0 && threadIdx.x == 0) b.stats->cell_chunks = nchunks;
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   if (t < 64) {
@@ -526,6 +598,18 @@ __global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, 
   counts[i] = ce - cs;
 }
 
+// The tile-count scan and the clamp above in one workgroup (n <= kOneGroupScan).
+__global__ __launch_bounds__(1024) void k_tile_ranges_one(uint32_t* counts, uint32_t* offsets, uint32_t n,
+                                                          uint32_t cap) {
+  (void)one_group_scan<uint32_t, 16>(counts, offsets, n);  // ends with a barrier: offsets are written
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+    const uint64_t st = offsets[i], e = st + counts[i];
+    const uint32_t cs = (uint32_t)(st < cap ? st : cap), ce = (uint32_t)(e < cap ? e : cap);
+    offsets[i] = cs;
+    counts[i] = ce - cs;
+  }
+}
+
 // Writing walk, tile-major: the chunk's entries (splat, tile mask, ln depth) are staged in LDS;
 // wave w then takes the cell's tiles 16w .. 16w+15 one at a time and walks all entries of the
 // chunk in order, so each tile's run of this chunk is written as one contiguous range (whole
@@ -535,7 +619,7 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   __shared__ uint32_t s_g[kCellChunk];
   __shared__ float s_ln[kCellChunk];
   const uint32_t q = blockIdx.x;
-  if (q >= b.stats->cell_chunks) return;  // uniform
+  if (q >= cell_chunk_total(b)) return;  // uniform
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -624,20 +708,8 @@ __global__ void k_classify_count(Bufs b, uint4* cls) {
   cls[t] = class_of(c);
 }
 
-__global__ void k_classify_write(Bufs b, const uint4* pre) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.total_tiles) return;
-  uint32_t m, s, c;
-  tile_info(b, t, &m, &s, &c);
-  const uint4 k = class_of(c);
-  const uint4 p = pre[t];
-  if (t == b.total_tiles - 1) {
-    b.stats->small_tiles = p.x + k.x;
-    b.stats->large_tiles = p.y + k.y;
-    b.stats->segments = p.z + k.z;
-    b.stats->chunks = p.z + k.z - (p.x + k.x);
-    b.stats->scratch_used = p.w + k.w;
-  }
+// One tile's list entries and segments, at its class prefix p.
+__device__ __forceinline__ void write_class(const Bufs& b, uint32_t m, uint32_t s, uint32_t c, uint4 k, uint4 p) {
   if (k.x) {
     b.small_list[p.x] = m;
     b.segs[p.z] = Seg{s, s, c, 0xFFFFFFFFu};
@@ -653,6 +725,23 @@ __global__ void k_classify_write(Bufs b, const uint4* pre) {
   }
 }
 
+__global__ void k_classify_write(Bufs b, const uint4* pre) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.total_tiles) return;
+  uint32_t m, s, c;
+  tile_info(b, t, &m, &s, &c);
+  const uint4 k = class_of(c);
+  const uint4 p = pre[t];
+  if (t == b.total_tiles - 1) {
+    b.stats->small_tiles = p.x + k.x;
+    b.stats->large_tiles = p.y + k.y;
+    b.stats->segments = p.z + k.z;
+    b.stats->chunks = p.z + k.z - (p.x + k.x);
+    b.stats->scratch_used = p.w + k.w;
+  }
+  write_class(b, m, s, c, k, p);
+}
+
 }  // namespace
 
 size_t scan_u32_tmp_words(uint32_t n) { return scan_tmp_elems<uint32_t>(n); }
@@ -665,6 +754,7 @@ void scan_u4_exclusive(const uint4* in, uint4* out, uint32_t n, uint4* tmp, hipS
 }
 
 void launch_clear_stats(const Bufs& b, hipStream_t stream) {
+  if (b.splat_blocks) return;  // k_totals clears them
   hipLaunchKernelGGL(k_clear_stats, dim3(1), dim3(64), 0, stream, b.stats);
 }
 
@@ -693,23 +783,23 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
   (void)radix_sort_pairs(b.ckeys_a, b.cvals_a, b.ckeys_b, b.cvals_b, &b.stats->cell_pairs, b.cell_cap,
                          b.cell_key_bits, b.radix_hist, b.radix_tmp, stream);
   // cell ranges and expansion chunks, per-chunk tile counts, per-cell prefixes -> tile counts
-  const uint32_t cg = (b.total_cells + 255) / 256;
+  const uint32_t cg = std::max((b.total_cells + 255) / 256, std::min((b.total_morton + 255) / 256, 1024u));
   hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, sorted_cell_keys(b));
   scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
-  hipLaunchKernelGGL(k_chunk_total, dim3(1), dim3(1), 0, stream, b);
-  const dim3 fg(std::min<uint32_t>((b.total_morton + 255) / 256, 1024u));
-  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_counts, b.total_morton, 0u);
-  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_dmin, b.total_morton, 0xFFFFFFFFu);
-  hipLaunchKernelGGL(k_fill_u32, fg, dim3(256), 0, stream, b.tile_dmax, b.total_morton, 0u);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
   hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
 }
 
 void launch_tile_ranges(const Bufs& b, hipStream_t stream) {
   if (!b.total_morton) return;
-  scan_u32_exclusive(b.tile_counts, b.tile_offsets, b.total_morton, b.radix_tmp, stream);
-  hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
-                     b.tile_counts, b.total_morton, b.cap);
+  if (b.total_morton <= kOneGroupScan) {
+    hipLaunchKernelGGL(k_tile_ranges_one, dim3(1), dim3(1024), 0, stream, b.tile_counts, b.tile_offsets,
+                       b.total_morton, b.cap);
+  } else {
+    scan_u32_exclusive(b.tile_counts, b.tile_offsets, b.total_morton, b.radix_tmp, stream);
+    hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
+                       b.tile_counts, b.total_morton, b.cap);
+  }
   if (b.total_cells) hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
 }
 

@@ -82,12 +82,15 @@ struct Gb {
 };
 
 // One entry into one pixel, without branches: an entry that does not pass the gate leaves every
-// accumulator unchanged (w = 0, T kept).  exp is only consumed where power > -4, so its argument is
-// clamped into the range of the check-free path (the result there is hdsr_expf's, bit for bit).
+// accumulator unchanged (w = 0, T kept).  exp is only consumed where power > -4; there the
+// check-free path gives hdsr_expf's result bit for bit once the argument is capped at 88 (above,
+// both saturate alpha at 0.99).  Below -4 the value is unused: whatever the path returns there
+// (zero, a NaN or an infinity for an extreme argument), alpha = min(0.99, o * gauss) stays finite
+// (min drops a NaN), so w = alpha * T * 0 is zero and T is kept.
 template <bool kG>
 __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
                                       const float4& ga, const float4& gb, float geb) {
-  const float gauss = hdsr_expf_core(fminf(fmaxf(power, -87.0f), 88.0f));
+  const float gauss = hdsr_expf_core(fminf(power, 88.0f));
   const float alpha = fminf(0.99f, o * gauss);
   const float newT = p.T * (1.0f - alpha);
   const bool valid = (power > -4.0f) && (alpha > (1.0f / 255.0f)) && (p.T >= 0.001f);
@@ -117,7 +120,7 @@ __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float 
 template <bool kG>
 __device__ __forceinline__ bool blend(Ph& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
                                       const float4& ga, const float4& gb, float geb) {
-  const float gauss = hdsr_expf_core(fminf(fmaxf(power, -87.0f), 88.0f));
+  const float gauss = hdsr_expf_core(fminf(power, 88.0f));
   const float alpha = fminf(0.99f, o * gauss);
   const _Float16 a16 = (_Float16)alpha;
   const _Float16 newT = p.T * ((_Float16)1.0f - a16);
@@ -192,13 +195,13 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
 template <bool kG, bool kD, bool kH>
 __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_a[kBatch];  // cx, cy, a, b
-  __shared__ float4 s_b[kBatch];  // c, opacity, r, g
-  __shared__ float2 s_c[kBatch];  // b, depth
+  __shared__ float4 s_b[kBatch];  // opacity, r, g, c (the blend's three fields 16-byte aligned)
+  __shared__ float4 s_c[kBatch];  // b, depth (float4: the three records share one slot index)
   __shared__ float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
   __shared__ float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
   __shared__ float s_gc[kG ? kBatch : 1];   // emissive b
   __shared__ uint32_t s_idx[kD ? kBatch : 1];  // entry index of each compacted slot
-  __shared__ uint16_t s_list[4][kBatch];       // per quadrant: the slots that may reach it, in order
+  __shared__ uint16_t s_list[4][kBatch];       // per quadrant: 16 x the slots that may reach it, in order
   __shared__ uint32_t s_cnt[5][4];             // per staging wave: kept entries, then per quadrant
   __shared__ int s_done[4];
 
@@ -217,7 +220,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const uint32_t X = tx * 16 + lx, Y = ty * 16 + ly;
   // the reference's quad origin and the +1 offsets of the quad's odd column / row
   const float p00x = (float)(X & ~1u) + 0.5f, p00y = (float)(Y & ~1u) + 0.5f;
-  const bool odd_x = (lx & 1u) != 0, odd_y = (ly & 1u) != 0;
+  // +1 on the odd column / row, +0 elsewhere: (p00 - c) + 0 equals p00 - c except that -0 turns
+  // into +0, which leaves the power's value unchanged
+  const float add_x = (lx & 1u) ? 1.0f : 0.0f, add_y = (ly & 1u) ? 1.0f : 0.0f;
 
   using PxT = typename std::conditional<kH, Ph, Px>::type;
   PxT p{};
@@ -270,12 +275,12 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
       for (uint32_t w2 = 0; w2 < wave; ++w2) slot += s_cnt[0][w2];
       s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
       if (kD) {
-        s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w));
-        s_c[slot] = make_float2(hdsr_u2f(h2.x), hdsr_u2f(h2.y));
+        s_b[slot] = make_float4(hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w), hdsr_u2f(h1.x) * -0.5f);
+        s_c[slot] = make_float4(hdsr_u2f(h2.x), hdsr_u2f(h2.y), 0.0f, 0.0f);
         s_idx[slot] = i;
       } else {
-        s_b[slot] = make_float4(hdsr_u2f(h1.x) * -0.5f, h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16));
-        s_c[slot] = make_float2(h16(h1.w), hdsr_u2f(h1.y));
+        s_b[slot] = make_float4(h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16), hdsr_u2f(h1.x) * -0.5f);
+        s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), 0.0f, 0.0f);
       }
       if (kG) {
         const uint4 c0 = b.cold[(size_t)g * 2];
@@ -299,10 +304,12 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
         if ((qm >> k) & 1u) {
           uint32_t pos = (uint32_t)__popcll(qb[k] & below);
           for (uint32_t w2 = 0; w2 < wave; ++w2) pos += s_cnt[1 + k][w2];
-          s_list[k][pos] = (uint16_t)slot;
+          s_list[k][pos] = (uint16_t)(slot * 16u);  // byte offset of the slot's float4 records
         }
     }
-    const uint32_t nq = s_cnt[1 + wave][0] + s_cnt[1 + wave][1] + s_cnt[1 + wave][2] + s_cnt[1 + wave][3];
+    // uniform per wave: the walk below is a scalar loop
+    const uint32_t nq = __builtin_amdgcn_readfirstlane(s_cnt[1 + wave][0] + s_cnt[1 + wave][1] +
+                                                       s_cnt[1 + wave][2] + s_cnt[1 + wave][3]);
     __syncthreads();
     {
       const uint32_t i1 = base + kBatch + tid, i2 = base + 2 * kBatch + tid;
@@ -316,15 +323,13 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     if (!__all(saturated(p))) {
       for (uint32_t j = 0; j < nq; ++j) {
         if ((j & 31u) == 31u && __all(saturated(p))) break;  // wave saturated
-        const uint32_t sl = s_list[wave][j];
-        const float4 A = s_a[sl];
-        const float4 B = s_b[sl];
-        float dx = p00x - A.x, dy = p00y - A.y;
-        dx = odd_x ? dx + 1.0f : dx;
-        dy = odd_y ? dy + 1.0f : dy;
-        const float pw = (A.z * dx * dx + A.w * dx * dy) + B.x * dy * dy;
+        const uint32_t off = s_list[wave][j], sl = off >> 4;
+        const float4 A = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_a) + off);
+        const float4 B = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_b) + off);
+        const float dx = (p00x - A.x) + add_x, dy = (p00y - A.y) + add_y;
+        const float pw = (A.z * dx * dx + A.w * dx * dy) + B.w * dy * dy;
         if (!__any(pw > -4.0f)) continue;  // uniform: no pixel of the wave is reached
-        const float2 C = s_c[sl];
+        const float4 C = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_c) + off);
         float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
         float gc = 0.0f;
         if (kG) {
@@ -332,7 +337,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
           gb = s_gb[sl];
           gc = s_gc[sl];
         }
-        const bool v = blend<kG>(p, q, pw, B.y, B.z, B.w, C.x, C.y, ga, gb, gc);
+        const bool v = blend<kG>(p, q, pw, B.x, B.y, B.z, C.x, C.y, ga, gb, gc);
         if (kD && v) last = s_idx[sl] + 1u;
       }
     }

@@ -10,11 +10,11 @@
 //   q20 = min(u32(y * 2^13), 2^20 - 1),  key = (q20 << 12) | i
 // with logMin / logRange from the min and max depth bits of the whole tile.  Keys are unique, so
 // the reference's bitonic network orders them exactly as a stable sort on q20 does; that sort is
-// done here as an LSD radix sort in LDS over q20 - min(q20 of the segment), with digits of 8, 8
-// and 4 bits; a pass whose digit lies above the segment's key span is skipped (a chunk of a large
-// tile spans a fraction of its tile's range).  Equal digits are ranked inside a wave with one ballot
-// per digit bit and a wave-private running count, so a pass needs five barriers of a four-wave
-// workgroup instead of the bitonic network's 78 stages.
+// done here as an LSD radix sort in LDS over q20 - min(q20 of the segment): a span of b bits takes
+// ceil(b / 8) passes of near-equal digit widths (4 to 8 bits), so only the span's bits are ranked
+// (a chunk of a large tile spans a fraction of its tile's range).  Equal digits are ranked inside a
+// wave with one ballot per digit bit and a wave-private running count, so a pass needs five
+// barriers of a four-wave workgroup instead of the bitonic network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
 // Large tiles: one 1024-thread workgroup writes the tile's entries in stable bucket order to
@@ -33,6 +33,22 @@ namespace hdsr {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// The lanes of m whose kBits-bit digit equals this lane's: one ballot per digit bit, folded into
+// the mask as m & ~(ballot ^ s), s = all ones where this lane's bit is set (one three-input bit
+// operation per 32-bit half; truth table 0x90 with the operands in that order).
+template <uint32_t kBits>
+__device__ __forceinline__ unsigned long long match_digit(uint32_t d, unsigned long long m) {
+#pragma unroll
+  for (uint32_t bit = 0; bit < kBits; ++bit) {
+    const uint32_t s = (uint32_t)((int32_t)(d << (31u - bit)) >> 31);
+    const unsigned long long bm = __ballot(s != 0u);
+    const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)m, (uint32_t)bm, s, 0x90);
+    const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(m >> 32), (uint32_t)(bm >> 32), s, 0x90);
+    m = ((unsigned long long)hi << 32) | lo;
+  }
+  return m;
+}
 
 // Running count of a group of lanes with equal digits (m: the group's lanes, this lane among
 // them when valid): the group's highest lane adds the group's size to the wave-private counter
@@ -78,13 +94,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
       for (uint32_t u = 0; u < 4; ++u) {
         const bool valid = r0 + u * 64 + lane < hi;
         const uint32_t k = kk[u];
-        unsigned long long m = __ballot(valid);
-#pragma unroll
-        for (uint32_t bit = 0; bit < 7; ++bit) {
-          const bool s = (k >> bit) & 1u;
-          const unsigned long long bm = __ballot(s);
-          m &= s ? bm : ~bm;
-        }
+        const unsigned long long m = match_digit<7>(k, __ballot(valid));
         if (valid && (m >> lane) == 1ull) wc[w][k] += (uint32_t)__popcll(m);
         __builtin_amdgcn_wave_barrier();
       }
@@ -127,13 +137,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
       for (uint32_t u = 0; u < 4; ++u) {
         const bool valid = r0 + u * 64 + lane < hi;
         const uint32_t k = qq[u] >> 13;
-        unsigned long long m = __ballot(valid);
-#pragma unroll
-        for (uint32_t bit = 0; bit < 7; ++bit) {
-          const bool s = (k >> bit) & 1u;
-          const unsigned long long bm = __ballot(s);
-          m &= s ? bm : ~bm;
-        }
+        const unsigned long long m = match_digit<7>(k, __ballot(valid));
         const uint32_t rank = __popcll(m & below);
         const uint32_t old = group_count(&wc[w][k], m, valid, lane);
         if (valid) {
@@ -169,12 +173,12 @@ constexpr uint32_t kSortPer = HDSR_MAX_TILE_ENTRIES / kSortThreads;
 // Wave w owns positions [w*S, min((w+1)*S, n)) (S a multiple of 64, at most 64 * kSortPer) and
 // walks them in rounds of 64; equal digits inside a round are ranked by ballots, across rounds by
 // a wave-private count.  Every thread of the workgroup must call it (it contains barriers).
+template <uint32_t kBits>
 __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
-                                               uint32_t shift, uint32_t bits, uint32_t (*wc)[256],
-                                               uint32_t* wsum) {
+                                               uint32_t shift, uint32_t (*wc)[256], uint32_t* wsum) {
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  const uint32_t mask = (1u << bits) - 1u;
+  const uint32_t mask = (1u << kBits) - 1u;
   for (uint32_t q = t; q < kSortWaves * 256; q += kSortThreads) wc[q >> 8][q & 255] = 0;
   __syncthreads();
   const uint32_t lo = w * S, hi = min(lo + S, n);
@@ -188,12 +192,7 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
     const bool valid = p < hi;
     const uint32_t key = valid ? cur[p] : 0u;
     const uint32_t d = (key >> shift) & mask;
-    unsigned long long m = __ballot(valid);
-    for (uint32_t bit = 0; bit < bits; ++bit) {
-      const bool s = (d >> bit) & 1u;
-      const unsigned long long bm = __ballot(s);
-      m &= s ? bm : ~bm;
-    }
+    const unsigned long long m = match_digit<kBits>(d, __ballot(valid));
     const uint32_t rank = __popcll(m & below);
     const uint32_t old = group_count(&wc[w][d], m, valid, lane);
     keys[r] = key;
@@ -230,6 +229,18 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   for (uint32_t r = 0; r < kSortPer; ++r)
     if (r < rounds && pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
   __syncthreads();
+}
+
+// One pass of a digit width known only at run time (uniform): 4 to 8 bits.
+__device__ __forceinline__ void radix_pass_w(uint32_t w, const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
+                                             uint32_t shift, uint32_t (*wc)[256], uint32_t* wsum) {
+  switch (w) {
+    case 4: lds_radix_pass<4>(cur, nxt, n, S, shift, wc, wsum); break;
+    case 5: lds_radix_pass<5>(cur, nxt, n, S, shift, wc, wsum); break;
+    case 6: lds_radix_pass<6>(cur, nxt, n, S, shift, wc, wsum); break;
+    case 7: lds_radix_pass<7>(cur, nxt, n, S, shift, wc, wsum); break;
+    default: lds_radix_pass<8>(cur, nxt, n, S, shift, wc, wsum); break;
+  }
 }
 
 // Small tiles and chunks: one workgroup per segment (grid-stride).  Keys are built from the
@@ -286,15 +297,17 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
     uint32_t* cur = ka;
     uint32_t* nxt = kb;
-    const uint32_t lows[3] = {0, 8, 16}, widths[3] = {8, 8, 4};
-#pragma unroll
-    for (uint32_t ps = 0; ps < 3; ++ps) {
-      if (lows[ps] < qbits) {  // uniform: passes above the key span are skipped
-        lds_radix_pass(cur, nxt, count, Sw, 12 + lows[ps], widths[ps], wc, wsum);
-        uint32_t* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-      }
+    // ceil(qbits / 8) passes of near-equal digit widths (4 to 8 bits) cover exactly the key span
+    const uint32_t np = (qbits + 7u) / 8u;
+    const uint32_t w0 = np ? max((qbits + np - 1u) / np, 4u) : 0u;
+    uint32_t low = 0;
+    for (uint32_t ps = 0; ps < np; ++ps) {  // uniform
+      const uint32_t w = ps + 1u < np ? w0 : max(qbits - low, 4u);
+      radix_pass_w(w, cur, nxt, count, Sw, 12u + low, wc, wsum);
+      low += w;
+      uint32_t* tmp = cur;
+      cur = nxt;
+      nxt = tmp;
     }
 #pragma unroll
     for (uint32_t r = 0; r < kSortPer; ++r) {
