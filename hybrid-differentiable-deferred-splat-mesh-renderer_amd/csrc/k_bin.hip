@@ -435,7 +435,8 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, cons
 // tile's offset + the chunk's prefix + the waves before + the rank: the canonical order of the
 // cell list is kept inside every tile.
 
-// Also resets the per-tile counts and depth ranges that the counting walk accumulates into.
+// Also resets the per-tile counts and depth ranges that the counting walk accumulates into; with
+// no pairs to group (keys null: no splats in the batch) it does only that.
 __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t i = c; i < b.total_morton; i += gridDim.x * blockDim.x) {
@@ -443,7 +444,7 @@ __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
     b.tile_dmin[i] = 0xFFFFFFFFu;
     b.tile_dmax[i] = 0u;
   }
-  if (c >= b.total_cells) return;
+  if (!keys || c >= b.total_cells) return;
   const uint32_t n = b.stats->cell_pairs;
   uint32_t lo = 0, hi = n;
   while (lo < hi) {
@@ -778,13 +779,19 @@ static uint32_t cell_passes(const Bufs& b) {
 static const uint32_t* sorted_cell_keys(const Bufs& b) { return cell_passes(b) & 1u ? b.ckeys_b : b.ckeys_a; }
 static const uint32_t* sorted_cell_vals(const Bufs& b) { return cell_passes(b) & 1u ? b.cvals_b : b.cvals_a; }
 
+// The batch has (cell, splat) pairs to group: at least one splat block and one cell.
+static bool has_pairs(const Bufs& b) { return b.splat_blocks && b.cell_cap && b.total_cells; }
+
 void launch_group_pairs(const Bufs& b, hipStream_t stream) {
-  if (!b.cell_cap || !b.total_cells) return;
-  (void)radix_sort_pairs(b.ckeys_a, b.cvals_a, b.ckeys_b, b.cvals_b, &b.stats->cell_pairs, b.cell_cap,
-                         b.cell_key_bits, b.radix_hist, b.radix_tmp, stream);
+  if (!b.total_morton) return;
+  const bool pairs = has_pairs(b);
+  if (pairs)
+    (void)radix_sort_pairs(b.ckeys_a, b.cvals_a, b.ckeys_b, b.cvals_b, &b.stats->cell_pairs, b.cell_cap,
+                           b.cell_key_bits, b.radix_hist, b.radix_tmp, stream);
   // cell ranges and expansion chunks, per-chunk tile counts, per-cell prefixes -> tile counts
   const uint32_t cg = std::max((b.total_cells + 255) / 256, std::min((b.total_morton + 255) / 256, 1024u));
-  hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, sorted_cell_keys(b));
+  hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, pairs ? sorted_cell_keys(b) : nullptr);
+  if (!pairs) return;  // every tile count is zero: the later stages find no entries
   scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
   hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
@@ -800,7 +807,8 @@ void launch_tile_ranges(const Bufs& b, hipStream_t stream) {
     hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
                        b.tile_counts, b.total_morton, b.cap);
   }
-  if (b.total_cells) hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
+  if (has_pairs(b))
+    hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
 }
 
 size_t radix_hist_words(uint32_t cap) {

@@ -1,0 +1,98 @@
+"""Edge cases of the compute path on the GPU against the oracle: a viewport that is not a multiple
+of the tile size, a scene entirely behind the camera, an empty scene, degenerate but finite
+attributes (zero-length quaternions, opacity 0 and 1, vanishing and very large scales), and
+non-finite attributes (NaN / infinity).  The last are checked for a finite image and the finite
+splats' records only: IEEE 754 leaves the sign and payload of a NaN that an operation produces to
+the implementation, and the GPU's differ from the host's, so records and per-tile frames that a NaN
+reaches are not comparable bit for bit."""
+import numpy as np
+import pytest
+
+from hdsr import camera, compositor, scenes
+from oracle import oracle
+from scenes_for_tests import dense_scene, frame_inputs
+from test_gpu_parity import compare_all, render_gpu
+
+pytestmark = pytest.mark.gpu
+
+BG = np.float32([0.02, 0.02, 0.04, 0.0])
+
+
+def test_ragged_viewport(hip_device):
+    s = dense_scene(n=15_000, seed=21, sh_degree=3, core_fraction=0.3)
+    cam = camera.Camera(camera.orbit_eye(2.6, 0.9, 0.35), (0, 0, 0), 310.0, 310.0, 333, 187)
+    plan, srt, u = frame_inputs(s, cam)
+    assert plan["tileColumns"] * 16 > 333 and plan["tileRows"] * 16 > 187
+    orc = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_GBUFFER)
+    res = render_gpu(s, srt, u, plan, gbuffer=True)
+    compare_all(res, orc, plan, gbuffer=True, exact=True)
+    res.destroy()
+
+
+def test_scene_behind_the_camera(hip_device):
+    s = scenes.scene_config0(3000, seed=5)
+    cam = camera.Camera((0.0, 0.0, 3.0), (0, 0, 0), 400.0, 400.0, 160, 96)
+    s.positions[:, 2] += 6.0  # every centre at z >= ~4.5, behind the eye at z = 3 looking down -z
+    plan, srt, u = frame_inputs(s, cam)
+    orc = oracle.run(s, srt, u, cam.width, cam.height)
+    assert orc.stats["visible"] == 0 and orc.stats["refs"] == 0
+    res = render_gpu(s, srt, u, plan)
+    compare_all(res, orc, plan, exact=True)
+    np.testing.assert_array_equal(res.read_rgba().reshape(-1, 4), np.broadcast_to(BG, (160 * 96, 4)))
+    res.destroy()
+
+
+def test_empty_scene(hip_device):
+    cam = camera.Camera((0.0, 0.0, 3.0), (0, 0, 0), 400.0, 400.0, 100, 70)
+    plan = compositor.planTileSplatCompositor({"viewportWidth": 100, "viewportHeight": 70, "tileSizePx": 16,
+                                               "splatCount": 0})
+    s = scenes.scene_config0(1, seed=1).subset(np.arange(0))
+    u = camera.frame_uniforms_for(cam, plan, 0)
+    res = render_gpu(s, np.zeros(0, np.uint32), u, plan, flags_debug=False)
+    st = res.stats()
+    assert st["refs"] == 0 and st["visible_splats"] == 0
+    np.testing.assert_array_equal(res.read_rgba().reshape(-1, 4), np.broadcast_to(BG, (100 * 70, 4)))
+    np.testing.assert_array_equal(res.read_depth(), np.zeros((70, 100), np.float32))
+    res.destroy()
+
+
+def test_degenerate_finite_attributes(hip_device):
+    s = dense_scene(n=8000, seed=33, sh_degree=3, core_fraction=0.4)
+    rng = np.random.default_rng(33)
+    idx = rng.permutation(s.count)[:1200].reshape(6, 200)
+    s.rotations[idx[0]] = 0.0                         # zero-length quaternion (normalised by max(len, 1e-6))
+    s.opacities[idx[1]] = 0.0
+    s.opacities[idx[2]] = 1.0
+    s.scales[idx[3]] = -30.0                          # exp(-30): vanishing footprints
+    s.scales[idx[4]] = np.float32([2.0, -6.0, -6.0])  # needles several tiles long
+    s.scales[idx[5], 0] = 1.5                         # large anisotropic discs
+    cam = camera.Camera(camera.orbit_eye(2.8, 0.3, 0.2), (0, 0, 0), 360.0, 360.0, 320, 240)
+    plan, srt, u = frame_inputs(s, cam)
+    orc = oracle.run(s, srt, u, cam.width, cam.height)
+    res = render_gpu(s, srt, u, plan)
+    compare_all(res, orc, plan, exact=True)
+    res.destroy()
+
+
+def test_non_finite_attributes(hip_device):
+    s = scenes.scene_config0(4000, seed=8)
+    bad = np.arange(40, 4000, 97)
+    s.positions[bad[0::4]] = np.nan
+    s.positions[bad[1::4], 0] = np.inf
+    s.scales[bad[2::4]] = np.nan
+    s.opacities[bad[3::4]] = np.nan
+    cam = camera.Camera(camera.orbit_eye(3.0, 0.4, 0.2), (0, 0, 0), 300.0, 300.0, 256, 160)
+    plan = compositor.planTileSplatCompositor({"viewportWidth": 256, "viewportHeight": 160, "tileSizePx": 16,
+                                               "splatCount": s.count})
+    srt = np.arange(s.count, dtype=np.uint32)  # the host depth sort rejects non-finite depths
+    u = camera.frame_uniforms_for(cam, plan, s.sh_degree)
+    orc = oracle.run(s, srt, u, cam.width, cam.height)
+    res = render_gpu(s, srt, u, plan)
+    rgba = res.read_rgba().reshape(-1, 4)
+    assert np.all(np.isfinite(rgba)) and np.all(np.isfinite(res.read_depth()))
+    assert rgba[:, 3].min() >= 0 and rgba[:, 3].max() <= 1
+    finite = np.ones(s.count, bool)
+    finite[bad] = False
+    np.testing.assert_array_equal(res.debug("projcache").reshape(-1, 14)[finite],
+                                  orc["cache"].reshape(-1, 14)[finite])
+    res.destroy()

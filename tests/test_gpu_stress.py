@@ -33,6 +33,8 @@ def test_stress_reduced_matches_oracle(hip_device):
     orc = oracle.run(s, srt, u, cam.width, cam.height)
     assert orc["tile_counts"].max() > 40_000 and orc.stats["chunks"] > 4 * orc.stats["large_tiles"]
     res = render_gpu(s, srt, u, plan)
+    # the references outgrow the starting pair capacity (16 per splat): the frame is rendered again
+    assert orc.stats["refs"] > 16 * s.count and res.stats()["overflow_reruns"] >= 1
     compare_all(res, orc, plan)
     res.destroy()
 

@@ -65,6 +65,15 @@ def test_create_rejects_unsupported_options_without_gpu_work():
     bad = dict(p, tileSizePx=8)
     rc = _lib.load().hdsr_create(C.byref(compositor._plan_struct(bad)), 0, C.byref(h))
     assert rc == _lib.HDSR_E_INVALID_ARG
+    # more than 255 tiles on an axis does not fit the u8 tile-box packing (gpu_project_splats.wgsl:229-231)
+    for w, hgt in ((4096, 64), (64, 4096)):
+        wide = compositor.planTileSplatCompositor({"viewportWidth": w, "viewportHeight": hgt, "tileSizePx": 16,
+                                                   "splatCount": 1})
+        rc = _lib.load().hdsr_create(C.byref(compositor._plan_struct(wide)), 0, C.byref(h))
+        assert rc == _lib.HDSR_E_INVALID_ARG and not h.value
+    edge = compositor.planTileSplatCompositor({"viewportWidth": 4080, "viewportHeight": 16, "tileSizePx": 16,
+                                               "splatCount": 1})
+    assert edge["tileColumns"] == 255
 
 
 IDENT = np.eye(4, dtype=np.float32).T.reshape(16)
