@@ -120,8 +120,11 @@ def test_autograd_function(setup):
     G = (2.0 * (out.detach() - target)).contiguous()
     r.render(case.uniforms, *dev)
     ref = r.backward(G, *dev)
+    # float atomics add the per-tile partial sums in varying order: two runs agree to the rounding
+    # of those partial sums (a few ulp of the largest gradients), not of each (possibly cancelled) total
     for leaf, g in zip(leaves, ref):
-        np.testing.assert_allclose(leaf.grad.cpu().numpy(), g.cpu().numpy(), rtol=1e-4, atol=1e-7)
+        gn = g.cpu().numpy()
+        np.testing.assert_allclose(leaf.grad.cpu().numpy(), gn, rtol=1e-4, atol=1e-6 * float(np.abs(gn).max()))
 
 
 def test_views_accumulate(setup):
@@ -137,5 +140,6 @@ def test_views_accumulate(setup):
     tot = sum(RenderSplats.apply(r, u, *leaves)[0].sum() for u in (u1, u2))
     tot.backward()
     for k, leaf in enumerate(leaves):
-        np.testing.assert_allclose(leaf.grad.cpu().numpy(), (gs[0][k] + gs[1][k]).cpu().numpy(), rtol=1e-4,
-                                   atol=1e-6)
+        want = (gs[0][k] + gs[1][k]).cpu().numpy()
+        np.testing.assert_allclose(leaf.grad.cpu().numpy(), want, rtol=1e-4,
+                                   atol=max(1e-6, 1e-6 * float(np.abs(want).max())))
