@@ -162,20 +162,23 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 }
 
 // Threads per segment-sort workgroup (16 keys per thread).  Four waves per workgroup keep the
-// passes' barriers cheap, and the register budget is held to four workgroups per CU (the LDS
-// allows four as well), so the global loads of one segment overlap the passes of others.
+// passes' barriers cheap; one 16 KB key array (the passes sort in place) and the register budget
+// let kSortMinWaves workgroups share a CU, so the global loads and the barrier waits of one
+// segment overlap the passes of others.
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortMinWaves = 4;
+constexpr uint32_t kSortMinWaves = 5;
 constexpr uint32_t kSortWaves = kSortThreads / 64;
 constexpr uint32_t kSortPer = HDSR_MAX_TILE_ENTRIES / kSortThreads;
 
-// One LSD pass over cur[0..n): stable by the digit (key >> shift) & (2^bits - 1), into nxt.
+// One LSD pass over keys[0..n), in place: stable by the digit (key >> shift) & (2^bits - 1).
 // Wave w owns positions [w*S, min((w+1)*S, n)) (S a multiple of 64, at most 64 * kSortPer) and
 // walks them in rounds of 64; equal digits inside a round are ranked by ballots, across rounds by
-// a wave-private count.  Every thread of the workgroup must call it (it contains barriers).
+// a wave-private count.  Every key is read into registers before the barriers that precede the
+// scatter, so the scatter can overwrite the same array.  Every thread of the workgroup must call
+// it (it contains barriers).
 template <uint32_t kBits>
-__device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
-                                               uint32_t shift, uint32_t (*wc)[256], uint32_t* wsum) {
+__device__ __forceinline__ void lds_radix_pass(uint32_t* ks, uint32_t n, uint32_t S, uint32_t shift,
+                                               uint32_t (*wc)[256], uint32_t* wsum) {
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t mask = (1u << kBits) - 1u;
@@ -183,20 +186,19 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   __syncthreads();
   const uint32_t lo = w * S, hi = min(lo + S, n);
   const uint32_t rounds = S / 64;  // the same for every wave: a segment of n keys needs ceil(n/256)
-  uint32_t keys[kSortPer], dig[kSortPer], pos[kSortPer];
+  uint32_t keys[kSortPer], pos[kSortPer];
 #pragma unroll
   for (uint32_t r = 0; r < kSortPer; ++r) {
     pos[r] = 0xFFFFFFFFu;
     if (r >= rounds) continue;  // uniform
     const uint32_t p = lo + r * 64 + lane;
     const bool valid = p < hi;
-    const uint32_t key = valid ? cur[p] : 0u;
+    const uint32_t key = valid ? ks[p] : 0u;
     const uint32_t d = (key >> shift) & mask;
     const unsigned long long m = match_digit<kBits>(d, __ballot(valid));
     const uint32_t rank = __popcll(m & below);
     const uint32_t old = group_count(&wc[w][d], m, valid, lane);
     keys[r] = key;
-    dig[r] = d;
     pos[r] = valid ? old + rank : 0xFFFFFFFFu;
   }
   __syncthreads();
@@ -227,29 +229,28 @@ __device__ __forceinline__ void lds_radix_pass(const uint32_t* cur, uint32_t* nx
   __syncthreads();
 #pragma unroll
   for (uint32_t r = 0; r < kSortPer; ++r)
-    if (r < rounds && pos[r] != 0xFFFFFFFFu) nxt[wc[w][dig[r]] + pos[r]] = keys[r];
+    if (r < rounds && pos[r] != 0xFFFFFFFFu) ks[wc[w][(keys[r] >> shift) & mask] + pos[r]] = keys[r];
   __syncthreads();
 }
 
 // One pass of a digit width known only at run time (uniform): 4 to 8 bits.
-__device__ __forceinline__ void radix_pass_w(uint32_t w, const uint32_t* cur, uint32_t* nxt, uint32_t n, uint32_t S,
-                                             uint32_t shift, uint32_t (*wc)[256], uint32_t* wsum) {
+__device__ __forceinline__ void radix_pass_w(uint32_t w, uint32_t* ks, uint32_t n, uint32_t S, uint32_t shift,
+                                             uint32_t (*wc)[256], uint32_t* wsum) {
   switch (w) {
-    case 4: lds_radix_pass<4>(cur, nxt, n, S, shift, wc, wsum); break;
-    case 5: lds_radix_pass<5>(cur, nxt, n, S, shift, wc, wsum); break;
-    case 6: lds_radix_pass<6>(cur, nxt, n, S, shift, wc, wsum); break;
-    case 7: lds_radix_pass<7>(cur, nxt, n, S, shift, wc, wsum); break;
-    default: lds_radix_pass<8>(cur, nxt, n, S, shift, wc, wsum); break;
+    case 4: lds_radix_pass<4>(ks, n, S, shift, wc, wsum); break;
+    case 5: lds_radix_pass<5>(ks, n, S, shift, wc, wsum); break;
+    case 6: lds_radix_pass<6>(ks, n, S, shift, wc, wsum); break;
+    case 7: lds_radix_pass<7>(ks, n, S, shift, wc, wsum); break;
+    default: lds_radix_pass<8>(ks, n, S, shift, wc, wsum); break;
   }
 }
 
 // Small tiles and chunks: one workgroup per segment (grid-stride).  Keys are built from the
-// streamed (depth, ln depth) pairs; the entries wait in registers during the passes and are then
-// staged in the free ping-pong buffer, where each sorted key's low 12 bits index them (small
-// tiles are sorted in place, so the source range cannot be read after the first write).
+// streamed q20 keys; the entries wait in registers during the passes and are then staged in the
+// key array, where each sorted key's low 12 bits index them (small tiles are sorted in place in
+// the entry list, so the source range cannot be read after the first write).
 __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b) {
   __shared__ uint32_t ka[HDSR_MAX_TILE_ENTRIES];
-  __shared__ uint32_t kb[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t wc[kSortWaves][256];
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t red[2][kSortWaves];
@@ -295,31 +296,40 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     const uint32_t span = qmax - qmin;  // uniform; 0 for a segment of one entry or equal keys
     const uint32_t qbits = span ? 32u - (uint32_t)__clz(span) : 0u;
     const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
-    uint32_t* cur = ka;
-    uint32_t* nxt = kb;
     // ceil(qbits / 8) passes of near-equal digit widths (4 to 8 bits) cover exactly the key span
     const uint32_t np = (qbits + 7u) / 8u;
     const uint32_t w0 = np ? max((qbits + np - 1u) / np, 4u) : 0u;
     uint32_t low = 0;
     for (uint32_t ps = 0; ps < np; ++ps) {  // uniform
       const uint32_t w = ps + 1u < np ? w0 : max(qbits - low, 4u);
-      radix_pass_w(w, cur, nxt, count, Sw, 12u + low, wc, wsum);
+      radix_pass_w(w, ka, count, Sw, 12u + low, wc, wsum);
       low += w;
-      uint32_t* tmp = cur;
-      cur = nxt;
-      nxt = tmp;
     }
+    if (np == 0) __syncthreads();  // the keys written above are read below
+    // each sorted key's low 12 bits index the segment's entries: take this thread's sorted keys,
+    // stage the entries (waiting in registers since the first load) in the same array, then
+    // gather them
+    uint32_t sk[kSortPer];
 #pragma unroll
     for (uint32_t r = 0; r < kSortPer; ++r) {
       const uint32_t i = r * kSortThreads + t;
-      if (i < count) nxt[i] = vv[r];
+      sk[r] = i < count ? ka[i] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kSortPer; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      if (i < count) ka[i] = vv[r];
     }
     __syncthreads();
     uint32_t* dst = b.entries + S.dst;
-    for (uint32_t i = t; i < count; i += kSortThreads) {
-      const uint32_t key = cur[i];
-      dst[i] = nxt[key & 0xFFFu];
-      if (b.keys20) b.keys20[S.dst + i] = key + (qmin << 12);
+#pragma unroll
+    for (uint32_t r = 0; r < kSortPer; ++r) {
+      const uint32_t i = r * kSortThreads + t;
+      if (i < count) {
+        dst[i] = ka[sk[r] & 0xFFFu];
+        if (b.keys20) b.keys20[S.dst + i] = sk[r] + (qmin << 12);
+      }
     }
     __syncthreads();
   }
