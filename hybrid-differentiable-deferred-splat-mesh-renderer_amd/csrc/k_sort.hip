@@ -287,6 +287,8 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
       qmin = min(qmin, red[0][q]);
       qmax = max(qmax, red[1][q]);
     }
+    qmin = __builtin_amdgcn_readfirstlane(qmin);  // uniform: scalar registers
+    qmax = __builtin_amdgcn_readfirstlane(qmax);
     // keys relative to the segment's smallest q20: only the bits of qmax - qmin are ranked
 #pragma unroll
     for (uint32_t r = 0; r < kSortPer; ++r) {
