@@ -17,7 +17,7 @@
 // barriers of a four-wave workgroup instead of the bitonic network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
-// Large tiles: one 1024-thread workgroup writes the tile's entries in stable bucket order to
+// Large tiles: one 512-thread workgroup writes the tile's entries in stable bucket order to
 // scratch (ranks by ballots, per-bucket prefix across waves).  The bucket-ordered list is then cut
 // into consecutive 4096-entry chunks (the reference's greedy packing splits buckets exactly there)
 // and each chunk is sorted like a small tile, reading scratch and writing the tile's range of the
@@ -61,14 +61,19 @@ __device__ __forceinline__ uint32_t group_count(uint32_t* counter, unsigned long
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
 }
 
-// Large tiles.  Wave w of the 1024-thread workgroup owns the contiguous slice [w*S, (w+1)*S)
-// of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the entry's key;
+// Large tiles.  Wave w of the kBucketThreads-thread workgroup owns the contiguous slice
+// [w*S, (w+1)*S) of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the entry's key;
 // ranks inside a round from seven ballots, a wave-private running count across rounds).  Then one
 // scan over (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 2 walks the
 // slice in the same order and writes each entry (and its key) at slot + its running rank: a
 // stable bucket order with two workgroup barriers per tile.
-__global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
-  __shared__ uint32_t wc[16][HDSR_NUM_BUCKETS];
+// Eight waves per tile: fewer waves per tile than 16 give each wave longer slices between the
+// barriers and let four workgroups share a CU (256 threads was slower again: 0.245 / 0.205 /
+// 0.237 ms per 1080p frame at 1024 / 512 / 256 threads).
+constexpr uint32_t kBucketThreads = 512;
+constexpr uint32_t kBucketWaves = kBucketThreads / 64;
+__global__ __launch_bounds__(kBucketThreads) void k_bucket(Bufs b) {
+  __shared__ uint32_t wc[kBucketWaves][HDSR_NUM_BUCKETS];
   __shared__ uint32_t wsum[2];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -77,10 +82,9 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     const LargeTile L = b.large_tiles[l];
     const uint32_t* src = b.entries + L.start;
     const uint32_t* qsrc = b.qkey + L.start;
-    const uint32_t S = ((L.count + 1023u) / 1024u) * 64u;
+    const uint32_t S = ((L.count + kBucketThreads - 1u) / kBucketThreads) * 64u;
     const uint32_t lo = w * S, hi = min(lo + S, L.count);
-    wc[t >> 7][t & 127] = 0;
-    wc[8 + (t >> 7)][t & 127] = 0;
+    for (uint32_t q = t; q < kBucketWaves * HDSR_NUM_BUCKETS; q += kBucketThreads) wc[q >> 7][q & 127] = 0;
     __syncthreads();
     // pass 1: wave-private bucket counts, four rounds of loads in flight
     for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     uint32_t tot = 0, inc = 0;
     if (t < HDSR_NUM_BUCKETS) {
 #pragma unroll
-      for (uint32_t q = 0; q < 16; ++q) tot += wc[q][t];
+      for (uint32_t q = 0; q < kBucketWaves; ++q) tot += wc[q][t];
       inc = tot;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(1024) void k_bucket(Bufs b) {
     if (t < HDSR_NUM_BUCKETS) {
       uint32_t run = inc - tot + (w == 1 ? wsum[0] : 0u);
 #pragma unroll
-      for (uint32_t q = 0; q < 16; ++q) {
+      for (uint32_t q = 0; q < kBucketWaves; ++q) {
         const uint32_t c = wc[q][t];
         wc[q][t] = run;
         run += c;
@@ -341,8 +345,8 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
 
 void launch_bucket(const Bufs& b, hipStream_t stream) {
   if (!b.total_tiles) return;
-  const uint32_t grid = b.total_tiles < 1024u ? b.total_tiles : 1024u;
-  hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(1024), 0, stream, b);
+  const uint32_t grid = b.total_tiles < 4096u ? b.total_tiles : 4096u;
+  hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(kBucketThreads), 0, stream, b);
 }
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {
