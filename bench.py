@@ -176,15 +176,20 @@ def main() -> int:
     per_stage = {}
     cells = -(-plan["tileColumns"] // 8) * -(-plan["tileRows"] // 8)
     ckb = max(1, (cells - 1).bit_length())
+    full_size = a.n == 1_000_000 and len(cams) == 8
     for name, ms in stage_ms.items():
         byts = algorithmic_bytes(name, scene.count, k, mean_st, pixels, comp_refs, ckb)
         gbs = (byts / (ms * 1e-3) / 1e9) if ms > 0 and byts else None
         per_stage[name] = {"ms": round(ms, 4), "GBps": None if gbs is None else round(gbs, 1)}
+        # the counters' bytes for single-kernel stages: the bucket cascade's per-unit figure (the
+        # reference's three global passes, 44 B per large-tile reference) exceeds what this kernel moves
+        pmc = PMC_TRAFFIC_BYTES.get(SINGLE_KERNEL_STAGES.get(name, ""))
+        if full_size and pmc and ms > 0:
+            per_stage[name]["traffic_GBps"] = round(pmc / (ms * 1e-3) / 1e9, 1)
     dom = max(SINGLE_KERNEL_STAGES, key=lambda s: stage_ms[s])
     dom_kernel = SINGLE_KERNEL_STAGES[dom]
     dom_bytes = algorithmic_bytes(dom, scene.count, k, mean_st, pixels, comp_refs)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
-    full_size = a.n == 1_000_000 and len(cams) == 8
 
     # the same frames with the back-to-front order produced on the GPU each frame (the reference's
     # encodeSort before the compositor) instead of resident host-made orders
