@@ -75,6 +75,11 @@ constexpr float kTCut16 = 1049.0f / 1048576.0f;     // f16(0.001)
 constexpr float kBgRG16 = 1311.0f / 65536.0f;       // f16(0.02)
 constexpr float kBgB16 = 1311.0f / 32768.0f;        // f16(0.04)
 
+// The float4 record at byte offset off of a staged array.
+__device__ __forceinline__ float4 rec16(const float4* a, uint32_t off) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + off);
+}
+
 __device__ __forceinline__ bool saturated(const Px& p) { return p.T < 0.001f; }
 __device__ __forceinline__ bool saturated(const Ph& p) { return p.T < (_Float16)kTCut16; }
 struct Gb {
@@ -87,11 +92,13 @@ struct Gb {
 // both saturate alpha at 0.99).  Below -4 the value is unused: whatever the path returns there
 // (zero, a NaN or an infinity for an extreme argument), alpha = min(0.99, o * gauss) stays finite
 // (min drops a NaN), so w = alpha * T * 0 is zero and T is kept.
+__device__ __forceinline__ float alpha_of(float power, float o) {
+  return fminf(0.99f, o * hdsr_expf_core(fminf(power, 88.0f)));
+}
+
 template <bool kG>
-__device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
-                                      const float4& ga, const float4& gb, float geb) {
-  const float gauss = hdsr_expf_core(fminf(power, 88.0f));
-  const float alpha = fminf(0.99f, o * gauss);
+__device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float alpha, float cr, float cg, float cb,
+                                      float dep, const float4& ga, const float4& gb, float geb) {
   const float newT = p.T * (1.0f - alpha);
   const bool valid = (power > -4.0f) && (alpha > (1.0f / 255.0f)) && (p.T >= 0.001f);
   const float w = (alpha * p.T) * (valid ? 1.0f : 0.0f);
@@ -118,10 +125,8 @@ __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float o, float 
 
 // The f16 variant's blend: the same gate, with T, C and w carried in binary16.
 template <bool kG>
-__device__ __forceinline__ bool blend(Ph& p, Gb& q, float power, float o, float cr, float cg, float cb, float dep,
-                                      const float4& ga, const float4& gb, float geb) {
-  const float gauss = hdsr_expf_core(fminf(power, 88.0f));
-  const float alpha = fminf(0.99f, o * gauss);
+__device__ __forceinline__ bool blend(Ph& p, Gb& q, float power, float alpha, float cr, float cg, float cb,
+                                      float dep, const float4& ga, const float4& gb, float geb) {
   const _Float16 a16 = (_Float16)alpha;
   const _Float16 newT = p.T * ((_Float16)1.0f - a16);
   const bool valid = (power > -4.0f) && (alpha > kAlphaTh16) && (p.T >= (_Float16)kTCut16);
@@ -146,6 +151,24 @@ __device__ __forceinline__ bool blend(Ph& p, Gb& q, float power, float o, float 
     q.eb = q.eb + geb * w32;
   }
   return valid;
+}
+
+// One staged entry (byte offset off, records A / B already read) into this lane's pixel.
+template <bool kG, bool kD, typename P>
+__device__ __forceinline__ void blend_slot(P& p, Gb& q, float pw, float alpha, const float4& B, uint32_t off,
+                                           const float4* s_c, const float4* s_ga, const float4* s_gb,
+                                           const float* s_gc, const uint32_t* s_idx, uint32_t* last) {
+  const float4 C = rec16(s_c, off);
+  const uint32_t sl = off >> 4;
+  float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
+  float gc = 0.0f;
+  if (kG) {
+    ga = s_ga[sl];
+    gb = s_gb[sl];
+    gc = s_gc[sl];
+  }
+  const bool v = blend<kG>(p, q, pw, alpha, B.y, B.z, C.x, C.y, ga, gb, gc);
+  if (kD && v) *last = s_idx[sl] + 1u;
 }
 
 __device__ __forceinline__ float4 out_rgba(const Px& p) {
@@ -321,24 +344,23 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     }
     loaded += min(kBatch, count - base);
     if (!__all(saturated(p))) {
-      for (uint32_t j = 0; j < nq; ++j) {
-        if ((j & 31u) == 31u && __all(saturated(p))) break;  // wave saturated
-        const uint32_t off = s_list[wave][j], sl = off >> 4;
-        const float4 A = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_a) + off);
-        const float4 B = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_b) + off);
-        const float dx = (p00x - A.x) + add_x, dy = (p00y - A.y) + add_y;
-        const float pw = (A.z * dx * dx + A.w * dx * dy) + B.w * dy * dy;
-        if (!__any(pw > -4.0f)) continue;  // uniform: no pixel of the wave is reached
-        const float4 C = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_c) + off);
-        float4 ga = make_float4(0, 0, 0, 0), gb = make_float4(0, 0, 0, 0);
-        float gc = 0.0f;
-        if (kG) {
-          ga = s_ga[sl];
-          gb = s_gb[sl];
-          gc = s_gc[sl];
-        }
-        const bool v = blend<kG>(p, q, pw, B.x, B.y, B.z, C.x, C.y, ga, gb, gc);
-        if (kD && v) last = s_idx[sl] + 1u;
+      // two entries per step: their exponentials are independent chains, blended in list order
+      const uint16_t* lst = s_list[wave];
+      for (uint32_t j = 0; j < nq; j += 2) {
+        if ((j & 30u) == 30u && __all(saturated(p))) break;  // wave saturated
+        const bool hb = j + 1u < nq;  // uniform
+        const uint32_t oa = lst[j], ob = hb ? lst[j + 1u] : oa;
+        const float4 Aa = rec16(s_a, oa), Ba = rec16(s_b, oa);
+        const float4 Ab = rec16(s_a, ob), Bb = rec16(s_b, ob);
+        const float dxa = (p00x - Aa.x) + add_x, dya = (p00y - Aa.y) + add_y;
+        const float dxb = (p00x - Ab.x) + add_x, dyb = (p00y - Ab.y) + add_y;
+        const float pwa = (Aa.z * dxa * dxa + Aa.w * dxa * dya) + Ba.w * dya * dya;
+        const float pwb = (Ab.z * dxb * dxb + Ab.w * dxb * dyb) + Bb.w * dyb * dyb;
+        const bool ra = __any(pwa > -4.0f), rb = hb && __any(pwb > -4.0f);  // uniform
+        if (!ra && !rb) continue;  // no pixel of the wave is reached by either
+        const float ala = alpha_of(pwa, Ba.x), alb = alpha_of(pwb, Bb.x);
+        if (ra) blend_slot<kG, kD>(p, q, pwa, ala, Ba, oa, s_c, s_ga, s_gb, s_gc, s_idx, &last);
+        if (rb) blend_slot<kG, kD>(p, q, pwb, alb, Bb, ob, s_c, s_ga, s_gb, s_gc, s_idx, &last);
       }
     }
     const int wave_done = __all(saturated(p));
