@@ -578,10 +578,17 @@ __global__ __launch_bounds__(64) void k_cell_prefix(Bufs b) {
   const uint32_t cx = lc % F.cells_x, cy = lc / F.cells_x;
   const uint32_t q0 = b.cell_chunk0[c], nq = b.cell_nchunk[c];
   uint32_t run = 0;
-  for (uint32_t q = q0; q < q0 + nq; ++q) {
-    const uint32_t v = b.chunk_cnt[(size_t)q * 64 + t];
-    b.chunk_cnt[(size_t)q * 64 + t] = run;
-    run += v;
+  // eight chunks' counts are loaded before any of their prefixes is stored, so the loads of a
+  // group are in flight together instead of one round trip per chunk
+  for (uint32_t g = 0; g < nq; g += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) v[k] = g + k < nq ? b.chunk_cnt[(size_t)(q0 + g + k) * 64 + t] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      if (g + k < nq) b.chunk_cnt[(size_t)(q0 + g + k) * 64 + t] = run;
+      run += v[k];
+    }
   }
   const uint32_t tx = cx * kCellTiles + (t & 7u), ty = cy * kCellTiles + (t >> 3);
   if (tx < F.tiles_x && ty < F.tiles_y) b.tile_counts[F.morton_base + hdsr_morton2(tx, ty)] = run;
