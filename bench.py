@@ -35,8 +35,8 @@ SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_b
 # HBM bytes per launch of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a 1 GiB copy (FETCH x2.0, WRITE
 # x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  Valid for the kernels as profiled.
-PMC_TRAFFIC_BYTES = {"k_project": 672.2e6, "k_emit": 30.6e6, "k_bucket": 714.1e6, "k_segsort": 493.3e6,
-                     "k_composite": 198.9e6}
+PMC_TRAFFIC_BYTES = {"k_project": 672.1e6, "k_emit": 30.6e6, "k_bucket": 741.0e6, "k_segsort": 506.4e6,
+                     "k_composite": 194.4e6}
 
 
 def parse():
@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--diff-only", action="store_true", help="run only the differentiation workload")
     ap.add_argument("--no-stress", action="store_true", help="skip the 6M-splat 3840x2160 stress workload")
     ap.add_argument("--stress-only", action="store_true", help="run only the stress workload")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -112,6 +113,9 @@ def main() -> int:
         return 0
     if a.stress_only:
         print(json.dumps({"stress": stress_bench()}))
+        return 0
+    if a.serving_only:
+        print(json.dumps({"serving": serving_bench()}))
         return 0
 
     t_setup = time.time()
@@ -416,7 +420,8 @@ def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
     return out
 
 
-def serving_bench(rates=(100, 200, 400, 600, 800, 1200, 3000, 5000), duration_s=1.0, max_requests=1500, seed=3) -> dict:
+def serving_bench(rates=(100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 3000, 5000), duration_s=1.0,
+                  max_requests=1500, seed=3) -> dict:
     """Serving workload: 32 resident scenes with sizes log-uniform in [50k, 4M] splats, SH0 or SH3
     by coin (each a distinct slice of a 4M-splat seeded scene of that degree), 256 requests of
     random scene / orbit pose / resolution in {640x480, 1280x720, 1920x1080}, submitted as a Poisson
