@@ -97,6 +97,7 @@ class Engine {
   uint64_t want_cap_ = 0;
   bool diff_ = false;
   bool f16_ = false;
+  bool bucket_wide_ = false;  // the last settled frame's large tiles averaged more than 16k entries
   size_t n_diff_ = 0;
   // allocation sizes (elements)
   size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;

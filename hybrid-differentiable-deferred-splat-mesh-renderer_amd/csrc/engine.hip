@@ -400,7 +400,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     tab[2 * (V + 1) + v] = L.cell_start[v];
   }
   const uint64_t sig[7] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.total_cells, L.cell_key_bits,
-                           (uint64_t)gbuffer | ((uint64_t)L.packed << 1)};
+                           (uint64_t)gbuffer | ((uint64_t)L.packed << 1) | ((uint64_t)bucket_wide_ << 2)};
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& g : graphs_)
     if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
@@ -418,6 +418,7 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
   }
   if (!exec) {
     Bufs b = b_;
+    b.bucket_wide = bucket_wide_;
     b.keys20 = nullptr;
     b.entries_canon = nullptr;
     b.packed = L.packed;
@@ -487,6 +488,7 @@ int Engine::run(const FrameDesc* frames_in, uint32_t V, hipStream_t stream, bool
   if ((rc = upload_frames(frames.data(), L, stream))) return rc;
   Bufs b = b_;
   b.packed = L.packed;
+  b.bucket_wide = bucket_wide_;
   if (!debug) {
     b.keys20 = nullptr;
     b.entries_canon = nullptr;
@@ -573,6 +575,11 @@ int Engine::wait() {
     err = std::string("device error: ") + hipGetErrorString(hipGetLastError());
     return HDSR_E_HIP;
   }
+  // the bucket cascade's workgroup size for the next frames, from this frame's large tiles: both
+  // sizes give the same order; 512 threads are faster for the usual tiles of a few thousand
+  // entries, 1024 when tiles of 100k entries would leave one 512-thread workgroup as the tail
+  const DevStats& s = *hstats_;
+  bucket_wide_ = s.large_tiles != 0 && s.scratch_used / s.large_tiles > 16384u;
   return 0;
 }
 

@@ -172,6 +172,7 @@ struct Bufs {
   uint32_t seg_cap;
   DevStats* stats;
   uint32_t packed;           // every frame reads its scene from packed records (host-set, selects the kernel)
+  uint32_t bucket_wide;      // large tiles average > 16k entries: one 1024-thread workgroup per tile (host-set)
 };
 
 // Packed splat record, in floats: [0..3] rotation wxyz, [4..6] position, [7..9] log-scale,

@@ -17,11 +17,11 @@
 // barriers of a four-wave workgroup instead of the bitonic network's 78 stages.
 // The bucket of an entry is min(u32(y), 127) = q20 >> 13.
 //
-// Large tiles: one 512-thread workgroup writes the tile's entries in stable bucket order to
-// scratch (ranks by ballots, per-bucket prefix across waves).  The bucket-ordered list is then cut
-// into consecutive 4096-entry chunks (the reference's greedy packing splits buckets exactly there)
-// and each chunk is sorted like a small tile, reading scratch and writing the tile's range of the
-// entry list.
+// Large tiles: one workgroup (512 or 1024 threads) writes the tile's entries in stable bucket
+// order to scratch (ranks by ballots, per-bucket prefix across waves).  The bucket-ordered list is
+// then cut into consecutive 4096-entry chunks (the reference's greedy packing splits buckets
+// exactly there) and each chunk is sorted like a small tile, reading scratch and writing the
+// tile's range of the entry list.
 //
 // Every entry's q20 arrives with it (qkey[]): the tile-list expansion evaluates it from the tile's
 // depth range (min / max folded during the counting walk) and ln(max(depth, 1e-6)) (computed once
@@ -67,12 +67,13 @@ __device__ __forceinline__ uint32_t group_count(uint32_t* counter, unsigned long
 // scan over (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 2 walks the
 // slice in the same order and writes each entry (and its key) at slot + its running rank: a
 // stable bucket order with two workgroup barriers per tile.
-// Eight waves per tile: fewer waves per tile than 16 give each wave longer slices between the
-// barriers and let four workgroups share a CU (256 threads was slower again: 0.245 / 0.205 /
-// 0.237 ms per 1080p frame at 1024 / 512 / 256 threads).
-constexpr uint32_t kBucketThreads = 512;
-constexpr uint32_t kBucketWaves = kBucketThreads / 64;
+// kBucketThreads = 512 (eight waves per tile) gives each wave longer slices between the barriers
+// and lets four workgroups share a CU: 0.245 / 0.205 / 0.237 ms per 1080p frame at 1024 / 512 /
+// 256 threads.  Frames whose large tiles average more than 16k entries (a 4K view into a dense
+// cluster: tiles of 100k entries) use 1024 threads, so the biggest tiles do not form the tail.
+template <uint32_t kBucketThreads>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket(Bufs b) {
+  constexpr uint32_t kBucketWaves = kBucketThreads / 64;
   __shared__ uint32_t wc[kBucketWaves][HDSR_NUM_BUCKETS];
   __shared__ uint32_t wsum[2];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
@@ -345,8 +346,13 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
 
 void launch_bucket(const Bufs& b, hipStream_t stream) {
   if (!b.total_tiles) return;
-  const uint32_t grid = b.total_tiles < 4096u ? b.total_tiles : 4096u;
-  hipLaunchKernelGGL(k_bucket, dim3(grid), dim3(kBucketThreads), 0, stream, b);
+  if (b.bucket_wide) {
+    const uint32_t grid = b.total_tiles < 1024u ? b.total_tiles : 1024u;
+    hipLaunchKernelGGL(k_bucket<1024>, dim3(grid), dim3(1024), 0, stream, b);
+  } else {
+    const uint32_t grid = b.total_tiles < 4096u ? b.total_tiles : 4096u;
+    hipLaunchKernelGGL(k_bucket<512>, dim3(grid), dim3(512), 0, stream, b);
+  }
 }
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {

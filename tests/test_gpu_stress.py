@@ -17,7 +17,7 @@ would need minutes, so the checks are the size-independent ones:
 import numpy as np
 import pytest
 
-from hdsr import scenes
+from hdsr import compositor, scenes
 from oracle import oracle
 from scenes_for_tests import frame_inputs
 from stress_checks import invariants, stress_camera
@@ -35,6 +35,13 @@ def test_stress_reduced_matches_oracle(hip_device):
     res = render_gpu(s, srt, u, plan)
     # the references outgrow the starting pair capacity (16 per splat): the frame is rendered again
     assert orc.stats["refs"] > 16 * s.count and res.stats()["overflow_reruns"] >= 1
+    compare_all(res, orc, plan)
+    # its large tiles average more than 16k entries, so the next render takes the bucket cascade's
+    # 1024-thread variant: the same lists and pixels
+    st = res.stats()
+    assert st["large_tile_refs"] > 16384 * st["large_tiles"]
+    compositor.encodeFullComputeCompositorPipeline(None, res)
+    res.sync()
     compare_all(res, orc, plan)
     res.destroy()
 
