@@ -373,3 +373,62 @@ def test_f16_composite_vs_fp32_and_fp64(small_frame, small_frame_f16):
     # depth votes gate on f32(w) > 0.01, so a pixel's voter set can change with the f16 weight
     dd = np.abs(g["out_depth"] - f["out_depth"])
     assert np.mean(dd > 1e-4) < 0.01
+
+
+def test_quaternion_sign_flip_changes_nothing():
+    # q and -q are one rotation: the rotation matrix is quadratic in q, so every product, and with
+    # it every projection record and list, is the same bit for bit (the counterpart, for this
+    # path's projection, of the covariance sign-flip invariance that
+    # tests/renderer/conicProjection.test.mjs:17-33 checks on the reference's JS conic module)
+    s = scenes.scene_config0(n=3000, seed=17)
+    cam = scenes.cameras_config0()[0]
+    plan, srt, u = frame_inputs(s, cam)
+    a = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_NO_COMPOSITE)
+    s.rotations = -s.rotations
+    b = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_NO_COMPOSITE)
+    for k in ("cache", "depth", "entries", "keys"):
+        np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_footprint_radius_clamp():
+    # the tile box uses min(radius, 8 tiles) (gpu_project_splats.wgsl:210-218; the clamp the
+    # reference's lifecycle test requires, tests/renderer/computeCompositorLifecycle.test.mjs:48-64):
+    # no box spans more than 17 tiles per axis, and interior splats with a larger radius span 16 or 17
+    s = scenes.scene_config0(n=400, seed=23)
+    s.scales[:] = np.float32(-0.5)  # exp(-0.5) * 3 sigma: radii of hundreds of pixels
+    cam = camera.Camera((0.0, 0.0, 3.0), (0, 0, 0), 525.0, 525.0, 640, 480)
+    plan, srt, u = frame_inputs(s, cam)
+    f = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_NO_COMPOSITE)
+    c = f["cache"].reshape(-1, 14)
+    vis = c[:, 7] != 0xFFFFFFFF
+    x0, y0, x1, y1 = (c[vis, 7] & 0xFF), (c[vis, 7] >> 8) & 0xFF, (c[vis, 7] >> 16) & 0xFF, c[vis, 7] >> 24
+    assert np.all(x1 - x0 + 1 <= 17) and np.all(y1 - y0 + 1 <= 17)
+    px, py = c[vis, 0].view(np.float32), c[vis, 1].view(np.float32)
+    rad = np.array([float(np.uint16(w & 0xFFFF).view(np.float16)) for w in c[vis, 6]])
+    inner = (rad > 128) & (px > 200) & (px < 440) & (py > 200) & (py < 280)
+    assert inner.sum() > 20
+    assert np.all(np.isin(x1[inner] - x0[inner] + 1, (16, 17))) and np.all(np.isin(y1[inner] - y0[inner] + 1, (16, 17)))
+
+
+def test_splat_scale_scales_the_covariance():
+    # splatScale multiplies the three scales (uniform word 24): the projected covariance, recovered
+    # from the conic as inv(conic) - 0.3 I, grows by splatScale^2
+    s = scenes.scene_config0(n=2000, seed=29)
+    cam = scenes.cameras_config0()[0]
+    plan, srt, u = frame_inputs(s, cam)
+    u2 = u.copy()
+    u2[24] = np.float32(2.0) * u[24]
+    a = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_NO_COMPOSITE)["cache"].reshape(-1, 14)
+    b = oracle.run(s, srt, u2, cam.width, cam.height, oracle.FLAG_NO_COMPOSITE)["cache"].reshape(-1, 14)
+    both = (a[:, 7] != 0xFFFFFFFF) & (b[:, 7] != 0xFFFFFFFF)
+
+    def cov(c):
+        k = c[:, 2:5].view(np.float32).astype(np.float64)
+        m = np.stack([np.stack([k[:, 0], k[:, 1]], -1), np.stack([k[:, 1], k[:, 2]], -1)], -2)
+        return np.linalg.inv(m) - 0.3 * np.eye(2)
+
+    ca, cb = cov(a[both]), cov(b[both])
+    big = np.abs(ca[:, 0, 0]) > 1.0  # away from the 0.3 low-pass, where fp32 keeps the ratio
+    assert big.sum() > 100
+    np.testing.assert_allclose(cb[big, 0, 0] / ca[big, 0, 0], 4.0, rtol=2e-3)
+    np.testing.assert_allclose(cb[big, 1, 1] / ca[big, 1, 1], 4.0, rtol=2e-3)
