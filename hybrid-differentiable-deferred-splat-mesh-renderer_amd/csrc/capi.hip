@@ -1,7 +1,6 @@
 // capi.hip - the C ABI of include/hdsr.h over one Engine (a batch of one frame).
 #include <algorithm>
 #include <cstring>
-#include <memory>
 #include <string>
 
 #include "hdsr.h"
@@ -116,7 +115,16 @@ extern "C" {
 
 const char* hdsr_version(void) { return "hdsr-mi355x 0.1 (gfx950)"; }
 
-const char* hdsr_last_error(hdsr_ctx* c) { return c ? c->err.c_str() : "null context"; }
+// Failures before a context exists (hdsr_create) leave their message here, per host thread.
+static thread_local std::string g_create_err = "null context";
+
+static int create_fail(int code, const char* what, hipError_t e) {
+  g_create_err = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+  (void)hipGetLastError();
+  return code;
+}
+
+const char* hdsr_last_error(hdsr_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out) {
   if (!in || !out) return HDSR_E_INVALID_ARG;
@@ -145,30 +153,32 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   if (plan->tile_size_px != HDSR_TILE_PX) return HDSR_E_INVALID_ARG;
   if (plan->tile_columns == 0 || plan->tile_rows == 0 || plan->tile_columns > 255 || plan->tile_rows > 255)
     return HDSR_E_INVALID_ARG;  // u8 tile-box packing (gpu_project_splats.wgsl:229-231)
-  std::unique_ptr<hdsr_ctx> c(new hdsr_ctx());
+  hdsr_ctx* c = new hdsr_ctx();
   c->plan = *plan;
   c->flags = flags;
-  if (hipGetDevice(&c->device) != hipSuccess) return HDSR_E_HIP;
+  if (hipError_t e = hipGetDevice(&c->device); e != hipSuccess) {
+    delete c;
+    return create_fail(HDSR_E_HIP, "hipGetDevice", e);
+  }
   const size_t px = (size_t)plan->viewport_width * plan->viewport_height;
-  if (hipMalloc((void**)&c->rgba, px * 16) != hipSuccess) return HDSR_E_OOM;
-  if (hipMalloc((void**)&c->depth, px * 4) != hipSuccess) return HDSR_E_OOM;
-  if (flags & HDSR_GBUFFER) {
-    if (hipMalloc((void**)&c->gbn, px * 4) != hipSuccess) return HDSR_E_OOM;
-    if (hipMalloc((void**)&c->gbm, px * 16) != hipSuccess) return HDSR_E_OOM;
+  // every output the flags ask for; on a failure the ones already allocated are freed
+  bool ok = hipMalloc((void**)&c->rgba, px * 16) == hipSuccess && hipMalloc((void**)&c->depth, px * 4) == hipSuccess;
+  if (ok && (flags & HDSR_GBUFFER))
+    ok = hipMalloc((void**)&c->gbn, px * 4) == hipSuccess && hipMalloc((void**)&c->gbm, px * 16) == hipSuccess;
+  if (ok && (flags & HDSR_OUT_RGBA16)) ok = hipMalloc((void**)&c->rgba16, px * 8) == hipSuccess;
+  if (ok && (flags & HDSR_DIFFERENTIABLE))
+    ok = hipMalloc((void**)&c->t_final, px * 4) == hipSuccess && hipMalloc((void**)&c->last_idx, px * 4) == hipSuccess;
+  if (!ok) {
+    const hipError_t e = hipGetLastError();
+    hdsr_destroy(c);
+    return create_fail(HDSR_E_OOM, "hipMalloc of the outputs", e);
   }
-  if (flags & HDSR_OUT_RGBA16) {
-    if (hipMalloc((void**)&c->rgba16, px * 8) != hipSuccess) return HDSR_E_OOM;
-  }
-  if (flags & HDSR_DIFFERENTIABLE) {
-    if (hipMalloc((void**)&c->t_final, px * 4) != hipSuccess) return HDSR_E_OOM;
-    if (hipMalloc((void**)&c->last_idx, px * 4) != hipSuccess) return HDSR_E_OOM;
-    c->eng.set_diff(true);
-  }
+  if (flags & HDSR_DIFFERENTIABLE) c->eng.set_diff(true);
   c->eng.set_f16((flags & HDSR_F16_COMPOSITE) != 0);
   const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
   c->eng.set_min_capacity(base_cap);
   c->eng.set_profile((flags & HDSR_PROFILE) != 0);
-  *out = c.release();
+  *out = c;
   return HDSR_OK;
 }
 
@@ -286,9 +296,19 @@ int hdsr_sort_back_to_front(hdsr_ctx* c, void* stream) {
   return HDSR_OK;
 }
 
+// The context's device current on the calling thread (a context may be driven from another host
+// thread than the one that created it, one thread at a time, e.g. the N-API addon's worker).
+static int use_device(hdsr_ctx* c) {
+  int cur = -1;
+  if (hipGetDevice(&cur) == hipSuccess && cur == c->device) return HDSR_OK;
+  if (hipSetDevice(c->device) != hipSuccess) return fail(c, HDSR_E_HIP, "hipSetDevice failed");
+  return HDSR_OK;
+}
+
 int hdsr_render(hdsr_ctx* c, void* stream) {
   if (!c) return HDSR_E_INVALID_ARG;
   if (!c->have_splats || !c->have_frame) return fail(c, HDSR_E_STATE, "bind splats and set a frame first");
+  if (int rc = use_device(c)) return rc;
   fill_frame_scene(c);
   const int rc = c->eng.run(&c->frame, 1, static_cast<hipStream_t>(stream), (c->flags & HDSR_GBUFFER) != 0,
                             (c->flags & HDSR_DEBUG) != 0);
@@ -311,6 +331,7 @@ int hdsr_composite_only(hdsr_ctx* c, void* stream) {
 int hdsr_sync(hdsr_ctx* c) {
   if (!c) return HDSR_E_INVALID_ARG;
   if (!c->rendered) return HDSR_OK;
+  if (int rc = use_device(c)) return rc;
   for (int attempt = 0; attempt < 4; ++attempt) {
     int rc = c->eng.wait();
     if (rc) return fail(c, rc, c->eng.err);

@@ -172,6 +172,15 @@ def load() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"HIP compositor library not built: {LIB_PATH} (run build.py)")
+    # One HIP runtime per process.  torch ships its own libamdhip64 and loads it by path; loaded
+    # before torch, libhdsr would map /opt/rocm's copy as well, and of two runtimes the one that
+    # initialises second finds no device (hipErrorNoDevice).  With torch imported first, libhdsr's
+    # libamdhip64.so.7 dependency binds to torch's copy (same soname), so torch's device pointers
+    # and streams are the library's too.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

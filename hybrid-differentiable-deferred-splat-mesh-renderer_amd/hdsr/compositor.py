@@ -66,7 +66,7 @@ class Resources:
         h = C.c_void_p()
         rc = self._lib.hdsr_create(C.byref(_plan_struct(plan)), flags, C.byref(h))
         if rc != 0:
-            raise HdsrError(rc, "hdsr_create failed")
+            raise HdsrError(rc, "hdsr_create failed: " + (self._lib.hdsr_last_error(None) or b"").decode())
         self.handle = h
         self._keep = []
 

@@ -64,6 +64,9 @@ function createTileSplatCompositor(device, plan, options) {
     flags: flags,
     destroy: function () { addon.destroy(handle); },
     sync: function () { addon.sync(handle); },
+    // the full pipeline on a worker thread: a Promise settled when the frame is complete (the
+    // context must not be used until then)
+    renderAsync: function () { return addon.renderAsync(handle); },
     readRGBA: function () { var a = new Float32Array(px * 4); addon.readback(handle, OUT.rgba, a); return a; },
     readDepth: function () { var a = new Float32Array(px); addon.readback(handle, OUT.depth, a); return a; },
     readOutput: function (name, array) { addon.readback(handle, OUT[name], array); return array; },

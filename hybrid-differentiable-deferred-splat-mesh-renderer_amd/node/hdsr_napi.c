@@ -27,7 +27,9 @@
 
 static napi_value throw_rc(napi_env env, int rc, hdsr_ctx* ctx, const char* what) {
   char buf[512];
-  snprintf(buf, sizeof buf, "%s failed (%d): %s", what, rc, ctx ? hdsr_last_error(ctx) : "");
+  /* with no context, only hdsr_create leaves a message (hdsr_last_error(NULL)) */
+  const char* msg = ctx ? hdsr_last_error(ctx) : strcmp(what, "hdsr_create") == 0 ? hdsr_last_error(NULL) : "";
+  snprintf(buf, sizeof buf, "%s failed (%d): %s", what, rc, msg);
   napi_throw_error(env, NULL, buf);
   return NULL;
 }
@@ -301,6 +303,74 @@ static napi_value js_render(napi_env env, napi_callback_info info) {
   const int rc = hdsr_render(ctx, NULL);
   if (rc) return throw_rc(env, rc, ctx, "hdsr_render");
   return NULL;
+}
+
+/* renderAsync(h): hdsr_render and hdsr_sync on a libuv worker thread; returns a Promise that
+ * resolves once the frame is complete (capacity re-runs included) or rejects with the error.
+ * The context must not be used from JS until the Promise settles: a context serves one host
+ * thread at a time. */
+typedef struct {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref handle_ref; /* keeps the context's external alive while the worker uses it */
+  hdsr_ctx* ctx;
+  int rc;
+  const char* what;
+  char err[512];
+} render_job;
+
+static void render_execute(napi_env env, void* data) {
+  render_job* j = (render_job*)data;
+  (void)env;
+  j->what = "hdsr_render";
+  j->rc = hdsr_render(j->ctx, NULL);
+  if (!j->rc) {
+    j->what = "hdsr_sync";
+    j->rc = hdsr_sync(j->ctx);
+  }
+  if (j->rc) snprintf(j->err, sizeof j->err, "%s", hdsr_last_error(j->ctx));
+}
+
+static void render_complete(napi_env env, napi_status status, void* data) {
+  render_job* j = (render_job*)data;
+  if (status == napi_ok && j->rc == 0) {
+    napi_value undef;
+    napi_get_undefined(env, &undef);
+    napi_resolve_deferred(env, j->deferred, undef);
+  } else {
+    char buf[640];
+    napi_value msg, err;
+    if (status != napi_ok)
+      snprintf(buf, sizeof buf, "renderAsync: the work was not completed (status %d)", (int)status);
+    else
+      snprintf(buf, sizeof buf, "%s failed (%d): %s", j->what, j->rc, j->err);
+    napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, NULL, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_reference(env, j->handle_ref);
+  napi_delete_async_work(env, j->work);
+  free(j);
+}
+
+static napi_value js_render_async(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1], promise, name;
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  render_job* j = (render_job*)calloc(1, sizeof *j);
+  if (!j) {
+    napi_throw_error(env, NULL, "renderAsync: out of host memory");
+    return NULL;
+  }
+  j->ctx = ctx;
+  CALL(env, napi_create_promise(env, &j->deferred, &promise));
+  CALL(env, napi_create_reference(env, argv[0], 1, &j->handle_ref));
+  CALL(env, napi_create_string_utf8(env, "hdsr.renderAsync", NAPI_AUTO_LENGTH, &name));
+  CALL(env, napi_create_async_work(env, NULL, name, render_execute, render_complete, j, &j->work));
+  CALL(env, napi_queue_async_work(env, j->work));
+  return promise;
 }
 
 static napi_value js_sort_back_to_front(napi_env env, napi_callback_info info) {
@@ -589,6 +659,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"updateSortedIndices", NULL, js_update_sorted, NULL, NULL, NULL, napi_default, NULL},
       {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_default, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},
+      {"renderAsync", NULL, js_render_async, NULL, NULL, NULL, napi_default, NULL},
       {"compositeOnly", NULL, js_composite_only, NULL, NULL, NULL, napi_default, NULL},
       {"sortBackToFront", NULL, js_sort_back_to_front, NULL, NULL, NULL, napi_default, NULL},
       {"sync", NULL, js_sync, NULL, NULL, NULL, napi_default, NULL},

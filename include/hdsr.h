@@ -263,6 +263,8 @@ typedef struct hdsr_grads {
 /* Waits for the last render (as hdsr_sync), then enqueues the backward on 'stream'. */
 int hdsr_backward(hdsr_ctx* ctx, const float* grad_rgba, const hdsr_grads* grads, void* stream);
 
+/* The message of the last failure on 'ctx'; with ctx NULL, of the last hdsr_create that failed on
+   the calling thread. */
 const char* hdsr_last_error(hdsr_ctx* ctx);
 const char* hdsr_version(void);
 

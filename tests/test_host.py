@@ -4,6 +4,8 @@ declares, the plan arithmetic, and the reference's own known-answer tests for th
 import ctypes as C
 import os
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -23,6 +25,7 @@ def header_functions():
 
 
 def test_library_exports_every_declared_symbol():
+    _lib.load()  # through the binding first: it puts torch's HIP runtime in place
     lib = C.CDLL(_lib.LIB_PATH)
     names = header_functions()
     assert len(names) >= 17
@@ -31,6 +34,19 @@ def test_library_exports_every_declared_symbol():
     declared = {n for n, _, _ in _lib.SIGNATURES}
     assert declared == set(names), f"binding table differs from header: {declared ^ set(names)}"
     assert _lib.load().hdsr_version().startswith(b"hdsr")
+
+
+def test_one_hip_runtime_when_loaded_before_torch():
+    """The binding loaded before anything imports torch still leaves one libamdhip64 mapped: torch
+    loads its own copy by path, and of two HIP runtimes in one process the one that initialises
+    second finds no device."""
+    pkg = os.path.dirname(os.path.dirname(os.path.abspath(_lib.__file__)))
+    code = ("import sys\nsys.path.insert(0, %r)\nfrom hdsr import _lib\n_lib.load()\nimport torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(sorted({l.split()[-1] for l in maps if 'libamdhip64' in l}))" % pkg)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert len(eval(r.stdout.strip().splitlines()[-1])) == 1, r.stdout
 
 
 def test_plan_matches_reference_arithmetic():

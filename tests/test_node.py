@@ -92,6 +92,33 @@ def test_facade_renders_like_oracle(hip_device, gbuffer):
             np.testing.assert_array_equal(bil, oracle.bilateral_normals(want, depth.reshape(320, 480)))
 
 
+@pytest.mark.gpu
+def test_render_async_matches_sync_and_oracle(hip_device):
+    """resources.renderAsync() (render + sync on a worker thread, a Promise to the JS host) gives
+    the image of the synchronous call and of the oracle."""
+    s = scenes.scene_config0(n=9_000, seed=22)
+    cam = camera.Camera(camera.orbit_eye(3.0, 0.5, 0.2), (0, 0, 0), 500.0, 500.0, 400, 272)
+    plan = compositor.planTileSplatCompositor({"viewportWidth": 400, "viewportHeight": 272, "tileSizePx": 16,
+                                               "splatCount": s.count})
+    srt = scenes.sort_back_to_front(s.positions, cam.view)
+    u = camera.frame_uniforms_for(cam, plan, s.sh_degree)
+    with tempfile.TemporaryDirectory() as d:
+        arrays = {"positions.bin": s.positions, "scales.bin": s.scales, "rotations.bin": s.rotations,
+                  "opacities.bin": s.opacities, "materials.bin": s.materials, "shdata.bin": s.sh_data(),
+                  "sorted.bin": srt, "viewproj.bin": cam.view_proj, "campos.bin": cam.camera_pos,
+                  "view.bin": cam.view, "proj.bin": cam.proj}
+        for k, v in arrays.items():
+            np.ascontiguousarray(v).tofile(os.path.join(d, k))
+        json.dump({"width": 400, "height": 272, "count": s.count, "shDegree": s.sh_degree},
+                  open(os.path.join(d, "meta.json"), "w"))
+        out = run_node("render_async.js", d)
+        assert out["equal"]
+        rgba = np.fromfile(os.path.join(d, "rgba_async.bin"), np.float32)
+        orc = oracle.run(s, srt, u, 400, 272)
+        assert out["stats"]["refs"] == orc.stats["refs"]
+        assert np.abs(rgba - orc["rgba"]).max() <= 1e-5
+
+
 def test_node_asset_loaders_match_python(tmp_path):
     """The Node facade's loaders (splatAssets.js: PLY through the addon's hdsr_ply_decode, the
     first-smoke manifest in JS) give the same arrays, bind buffers and bounds as hdsr.assets."""
