@@ -96,3 +96,22 @@ def test_non_finite_attributes(hip_device):
     np.testing.assert_array_equal(res.debug("projcache").reshape(-1, 14)[finite],
                                   orc["cache"].reshape(-1, 14)[finite])
     res.destroy()
+
+
+def test_tied_keys_and_near_depth_order(hip_device):
+    """Tile lists with equal keys (every splat three times at one position) under the depth order,
+    and under the depth order with 1% of neighbours swapped: per-tile entries and keys (ties by
+    list position) equal the oracle's."""
+    base = dense_scene(n=6000, seed=41, sh_degree=0, core_fraction=0.3)
+    s = base.subset(np.repeat(np.arange(base.count), 3))
+    cam = camera.Camera(camera.orbit_eye(2.7, 0.5, 0.3), (0, 0, 0), 380.0, 380.0, 320, 224)
+    plan, srt, u = frame_inputs(s, cam)
+    swapped = srt.copy()
+    rng = np.random.default_rng(41)
+    for i in rng.choice(s.count - 1, s.count // 100, replace=False):
+        swapped[i], swapped[i + 1] = swapped[i + 1], swapped[i]
+    for order in (srt, swapped):
+        orc = oracle.run(s, order, u, cam.width, cam.height)
+        res = render_gpu(s, order, u, plan)
+        compare_all(res, orc, plan, exact=True)
+        res.destroy()
