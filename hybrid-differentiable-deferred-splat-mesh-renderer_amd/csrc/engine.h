@@ -97,6 +97,7 @@ class Engine {
   uint64_t want_cap_ = 0;
   bool diff_ = false;
   bool f16_ = false;
+  bool deferred_ = false;     // the last enqueued frame left chunks unsorted (deferred chunk sort)
   bool bucket_wide_ = false;  // the last settled frame's large tiles averaged more than 16k entries
   size_t n_diff_ = 0;
   // allocation sizes (elements)
@@ -117,6 +118,7 @@ class Engine {
   // profiling
   static constexpr int kStages = HDSR_STAGE_COUNT;
   static constexpr int kProfRing = 128;
+  static constexpr int kProfEvents = kStages + 3;  // stage boundaries + the two of the deferred rounds
   bool profile_ = false;
   std::vector<hipEvent_t> prof_ev_;  // kProfRing * (kStages + 1)
   int prof_head_ = 0, prof_count_ = 0;

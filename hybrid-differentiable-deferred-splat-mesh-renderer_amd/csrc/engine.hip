@@ -86,7 +86,7 @@ void Engine::release() {
                   b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
-                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, d_frames_};
+                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, b_.tile_pend, b_.px_state, d_frames_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b_ = Bufs{};
@@ -227,6 +227,8 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.tile_offsets, &have, L.total_morton))) return rc;
     have = n_morton_;
     if ((rc = grow(&b_.tile_counts, &have, L.total_morton))) return rc;
+    have = n_morton_;
+    if ((rc = grow(&b_.tile_pend, &have, L.total_morton))) return rc;
     have = n_morton_ * 2;
     if ((rc = grow(&b_.tile_dmin, &have, (size_t)L.total_morton * 2))) return rc;
     n_morton_ = L.total_morton;
@@ -241,6 +243,8 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.large_bases, &have, L.total_tiles))) return rc;
     have = n_tiles_;
     if ((rc = grow(&b_.large_tiles, &have, L.total_tiles))) return rc;
+    have = n_tiles_ * 256 * 6;
+    if ((rc = grow(&b_.px_state, &have, (size_t)L.total_tiles * 256 * 6))) return rc;
     n_tiles_ = L.total_tiles;
   }
   // segments: every small tile, plus at most one 4096-entry chunk per 4096 pairs and per large tile
@@ -349,9 +353,26 @@ void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, boo
     (void)hipMemcpyAsync(b.entries_canon, b.entries, (size_t)b.cap * 4, hipMemcpyDeviceToDevice, stream);
   launch_bucket(b, stream);
   if (ev) (void)hipEventRecord(ev[7], stream);
-  launch_segment_sort(b, stream);
+  // Deferred chunk sort when nothing reads the whole lists: no debug readback, no G-buffer (its
+  // votes are not part of the saved pixel state), no backward pass.
+  deferred_ = !debug && !gbuffer && !b.drec;
+  Bufs r1 = b, r2 = b;
+  r1.round = deferred_ ? 1u : 0u;
+  r2.round = 2u;
+  // Stage events: sort = [7, 8] + [10, 11], composite = [8, 10] + [11, 9] (the rounds interleave).
+  launch_segment_sort(r1, stream);
   if (ev) (void)hipEventRecord(ev[8], stream);
-  launch_composite(b, stream, gbuffer, f16_);
+  if (deferred_) {
+    launch_composite(r1, stream, gbuffer, f16_);
+    if (ev) (void)hipEventRecord(ev[kStages + 1], stream);
+    launch_segment_sort(r2, stream);
+    if (ev) (void)hipEventRecord(ev[kStages + 2], stream);
+    launch_composite(r2, stream, gbuffer, f16_);
+  } else {
+    if (ev) (void)hipEventRecord(ev[kStages + 1], stream);
+    if (ev) (void)hipEventRecord(ev[kStages + 2], stream);
+    launch_composite(r1, stream, gbuffer, f16_);
+  }
   if (ev) (void)hipEventRecord(ev[9], stream);
 }
 
@@ -521,7 +542,20 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
   Layout L = layout(frames.data(), V);
   int rc = upload_frames(frames.data(), L, stream);
   if (rc) return rc;
-  launch_composite(b_, stream, gbuffer, f16_);
+  if (deferred_ && gbuffer) {  // the G-buffer reads whole lists; a deferred frame sorted only a part
+    err = "composite_only with the G-buffer after a frame rendered without it";
+    return HDSR_E_STATE;
+  }
+  if (deferred_) {
+    // the lists as the last render left them: the rounds meet the same saturation again
+    Bufs r1 = b_, r2 = b_;
+    r1.round = 1;
+    r2.round = 2;
+    launch_composite(r1, stream, gbuffer, f16_);
+    launch_composite(r2, stream, gbuffer, f16_);
+  } else {
+    launch_composite(b_, stream, gbuffer, f16_);
+  }
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) {
     err = std::string("kernel launch failed: ") + hipGetErrorString(le);
@@ -532,7 +566,7 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
 
 hipEvent_t* Engine::prof_slot() {
   if (prof_ev_.empty()) {
-    prof_ev_.resize((size_t)kProfRing * (kStages + 1));
+    prof_ev_.resize((size_t)kProfRing * kProfEvents);
     for (hipEvent_t& e : prof_ev_)
       if (hipEventCreate(&e) != hipSuccess) {
         prof_ev_.clear();
@@ -543,16 +577,20 @@ hipEvent_t* Engine::prof_slot() {
   if (prof_count_ == kProfRing) prof_harvest_one();
   const int slot = (prof_head_ + prof_count_) % kProfRing;
   ++prof_count_;
-  return &prof_ev_[(size_t)slot * (kStages + 1)];
+  return &prof_ev_[(size_t)slot * kProfEvents];
 }
 
 void Engine::prof_harvest_one() {
-  hipEvent_t* ev = &prof_ev_[(size_t)prof_head_ * (kStages + 1)];
+  hipEvent_t* ev = &prof_ev_[(size_t)prof_head_ * kProfEvents];
   (void)hipEventSynchronize(ev[kStages]);
-  for (int s = 0; s < kStages; ++s) {
+  auto span = [&](int a, int b) {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev[s], ev[s + 1]) == hipSuccess) prof_acc_[s] += ms;
-  }
+    return hipEventElapsedTime(&ms, ev[a], ev[b]) == hipSuccess ? (double)ms : 0.0;
+  };
+  for (int s = 0; s < kStages - 2; ++s) prof_acc_[s] += span(s, s + 1);
+  const int sort = kStages - 2, comp = kStages - 1, e1 = kStages + 1, e2 = kStages + 2;
+  prof_acc_[sort] += span(sort, sort + 1) + span(e1, e2);
+  prof_acc_[comp] += span(comp, e1) + span(e2, kStages);
   ++prof_runs_;
   prof_head_ = (prof_head_ + 1) % kProfRing;
   --prof_count_;

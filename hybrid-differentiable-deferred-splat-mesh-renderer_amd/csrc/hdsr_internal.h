@@ -173,6 +173,13 @@ struct Bufs {
   DevStats* stats;
   uint32_t packed;           // every frame reads its scene from packed records (host-set, selects the kernel)
   uint32_t bucket_wide;      // large tiles average > 16k entries: one 1024-thread workgroup per tile (host-set)
+  // Deferred chunk sort (host-set round; 0 = the whole frame in one pass).  Round 1 sorts the small
+  // tiles and the first chunk of every large tile, and rasterizes each large tile up to the end of
+  // that chunk; a tile left unsaturated there gets tile_pend[morton] = 1 and its pixel state saved.
+  // Round 2 sorts the other chunks of those tiles only and resumes them.
+  uint32_t round;
+  uint32_t* tile_pend;       // [total_morton] large tiles only: 1 = resumed in round 2
+  float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)
 };
 
 // Packed splat record, in floats: [0..3] rotation wxyz, [4..6] position, [7..9] log-scale,

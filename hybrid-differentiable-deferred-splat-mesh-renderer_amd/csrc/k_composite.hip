@@ -82,6 +82,27 @@ __device__ __forceinline__ float4 rec16(const float4* a, uint32_t off) {
 
 __device__ __forceinline__ bool saturated(const Px& p) { return p.T < 0.001f; }
 __device__ __forceinline__ bool saturated(const Ph& p) { return p.T < (_Float16)kTCut16; }
+
+// A deferred tile's pixel state between the rounds (planar: field f of pixel i at s[f*256 + i]);
+// binary16 values convert to fp32 and back exactly.
+template <class P>
+__device__ __forceinline__ void save_px(float* s, const P& p) {
+  s[0] = (float)p.T;
+  s[256] = (float)p.r;
+  s[512] = (float)p.g;
+  s[768] = (float)p.b;
+  s[1024] = p.dw;
+  s[1280] = p.ws;
+}
+template <class P>
+__device__ __forceinline__ void load_px(const float* s, P& p) {
+  p.T = s[0];
+  p.r = s[256];
+  p.g = s[512];
+  p.b = s[768];
+  p.dw = s[1024];
+  p.ws = s[1280];
+}
 struct Gb {
   float mr, mm, nx, ny, nz, cf, er, eg, eb;
 };
@@ -235,8 +256,20 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const uint32_t tx = lt % F.tiles_x, ty = lt / F.tiles_x;
   const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
   const uint32_t start = b.tile_offsets[m];
-  const uint32_t count = b.tile_counts[m];
+  const uint32_t full = b.tile_counts[m];
   const uint32_t tid = threadIdx.x;
+  // Deferred chunk sort: round 1 stops a large tile at the end of its first chunk (the only one
+  // sorted yet); round 2 resumes the tiles left unsaturated there from their saved state.  The
+  // batches are those of one pass (4096 is a multiple of kBatch), so the pixels are the same.
+  const bool deferrable = b.round != 0 && full > HDSR_MAX_TILE_ENTRIES;  // uniform
+  uint32_t count = full, base0 = 0;
+  if (b.round == 2) {
+    if (!deferrable || !b.tile_pend[m]) return;
+    base0 = HDSR_MAX_TILE_ENTRIES;
+  } else if (deferrable) {
+    count = HDSR_MAX_TILE_ENTRIES;
+  }
+  float* const pst = b.px_state + (size_t)t * (256 * 6) + tid;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t lx = (wave & 1u) * 8u + (lane & 7u), ly = (wave >> 1) * 8u + (lane >> 3);
@@ -250,6 +283,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   using PxT = typename std::conditional<kH, Ph, Px>::type;
   PxT p{};
   p.T = 1.0f;
+  if (base0) load_px(pst, p);
   uint32_t last = 0;
   Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t loaded = 0;
@@ -260,12 +294,14 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const uint4 z4 = make_uint4(0, 0, 0, 0);
   const uint4* rec = kD ? reinterpret_cast<const uint4*>(b.drec) : b.hot;
   constexpr uint32_t kRec = kD ? 3 : 2;  // 16-byte words per record
-  uint32_t g_cur = tid < count ? b.entries[start + tid] : 0u;
-  uint4 h0n = tid < count ? rec[(size_t)g_cur * kRec] : z4;
-  uint4 h1n = tid < count ? rec[(size_t)g_cur * kRec + 1] : z4;
-  uint4 h2n = (kD && tid < count) ? rec[(size_t)g_cur * kRec + 2] : z4;
-  uint32_t g_next = kBatch + tid < count ? b.entries[start + kBatch + tid] : 0u;
-  for (uint32_t base = 0; base < count; base += kBatch) {
+  const uint32_t i0 = base0 + tid;
+  uint32_t g_cur = i0 < count ? b.entries[start + i0] : 0u;
+  uint4 h0n = i0 < count ? rec[(size_t)g_cur * kRec] : z4;
+  uint4 h1n = i0 < count ? rec[(size_t)g_cur * kRec + 1] : z4;
+  uint4 h2n = (kD && i0 < count) ? rec[(size_t)g_cur * kRec + 2] : z4;
+  uint32_t g_next = i0 + kBatch < count ? b.entries[start + i0 + kBatch] : 0u;
+  bool finished = false;  // every pixel saturated
+  for (uint32_t base = base0; base < count; base += kBatch) {
     const uint32_t i = base + tid;
     const uint4 h0 = h0n, h1 = h1n, h2 = h2n;
     const uint32_t g = g_cur;
@@ -366,7 +402,18 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     const int wave_done = __all(saturated(p));
     if (lane == 0) s_done[wave] = wave_done;
     __syncthreads();
-    if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) break;
+    if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) {
+      finished = true;
+      break;
+    }
+  }
+  if (b.round == 1 && deferrable) {
+    if (tid == 0) b.tile_pend[m] = finished ? 0u : 1u;
+    if (!finished) {  // uniform: resumed in round 2
+      save_px(pst, p);
+      if (tid == 0 && b.stats) atomicAdd(&b.stats->composite_refs, loaded);
+      return;
+    }
   }
   store_px<kG>(F, X, Y, p, q);
   if (kD && X < F.width && Y < F.height) {
@@ -376,7 +423,7 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   }
   if (tid == 0 && b.stats) {
     atomicAdd(&b.stats->composite_refs, loaded);
-    atomicMax(&b.stats->composite_max_refs, loaded);
+    atomicMax(&b.stats->composite_max_refs, base0 + loaded);
   }
 }
 

@@ -263,6 +263,17 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
   const uint32_t nseg = b.stats->segments;
   for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
     const Seg S = b.segs[s];
+    if (b.round) {  // uniform: deferred chunk sort
+      bool take;
+      if (S.large == kNone) {
+        take = b.round == 1;
+      } else {  // round 1: the first chunk; round 2: the other chunks of the tiles resumed
+        const LargeTile& L = b.large_tiles[S.large];
+        const bool first = S.dst == L.start;
+        take = b.round == 1 ? first : !first && b.tile_pend[L.morton] != 0u;
+      }
+      if (!take) continue;
+    }
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
     const uint32_t count = S.count;
