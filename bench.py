@@ -32,11 +32,15 @@ HBM_PEAK_GBS = 8000.0
 # Stages that are one kernel each: the roofline is taken on the slowest of them.
 SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_bucket", "sort": "k_segsort",
                         "composite": "k_composite"}
-# HBM bytes per launch of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
+# HBM bytes per frame of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a 1 GiB copy (FETCH x2.0, WRITE
-# x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  Valid for the kernels as profiled.
-PMC_TRAFFIC_BYTES = {"k_project": 672.1e6, "k_emit": 30.6e6, "k_bucket": 741.0e6, "k_segsort": 506.4e6,
-                     "k_composite": 194.4e6}
+# x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  k_segsort and k_composite run twice per frame
+# (the deferred chunk sort's two rounds): theirs are the sums of both launches.  Valid for the
+# kernels as profiled.
+PMC_TRAFFIC_BYTES = {"k_project": 670.5e6, "k_emit": 30.6e6, "k_bucket": 741.0e6, "k_segsort": 218.0e6,
+                     "k_composite": 197.2e6}
+# launches per frame of the single-kernel stages (the deferred rounds)
+LAUNCHES_PER_FRAME = {"k_segsort": 2, "k_composite": 2}
 
 
 def parse():
@@ -249,7 +253,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": dom_kernel, "stage": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": PMC_TRAFFIC_BYTES.get(dom_kernel) if full_size else None,
-                     "algorithmic_bytes_per_launch": dom_bytes, "ms_per_launch": round(stage_ms[dom], 4)},
+                     "algorithmic_bytes_per_frame": dom_bytes, "ms_per_frame": round(stage_ms[dom], 4),
+                     "launches_per_frame": LAUNCHES_PER_FRAME.get(dom_kernel, 1)},
         "stages_ms_per_frame": per_stage,
         "frame_stats": {k2: round(v2, 1) for k2, v2 in mean_st.items()},
         "cpu_baseline": cpu,
