@@ -597,7 +597,11 @@ int hdsr_debug_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
     case HDSR_DBG_DEPTH: HIPX(c, hipMemcpy(host, b.depth, need, hipMemcpyDeviceToHost)); return HDSR_OK;
     case HDSR_DBG_TILE_COUNTS: HIPX(c, hipMemcpy(host, b.tile_counts, need, hipMemcpyDeviceToHost)); return HDSR_OK;
     case HDSR_DBG_TILE_OFFSETS: HIPX(c, hipMemcpy(host, b.tile_offsets, need, hipMemcpyDeviceToHost)); return HDSR_OK;
-    case HDSR_DBG_ENTRIES: HIPX(c, hipMemcpy(host, b.entries, need, hipMemcpyDeviceToHost)); return HDSR_OK;
+    case HDSR_DBG_ENTRIES:
+      // a frame that sorted only the chunks its pixels reach gets the others sorted first
+      if (int e = c->eng.complete_lists(nullptr)) return fail(c, e, c->eng.err);
+      HIPX(c, hipMemcpy(host, b.entries, need, hipMemcpyDeviceToHost));
+      return HDSR_OK;
     case HDSR_DBG_ENTRIES_CANON: HIPX(c, hipMemcpy(host, b.entries_canon, need, hipMemcpyDeviceToHost)); return HDSR_OK;
     case HDSR_DBG_KEYS20: HIPX(c, hipMemcpy(host, b.keys20, need, hipMemcpyDeviceToHost)); return HDSR_OK;
     case HDSR_DBG_SMALL_LIST: HIPX(c, hipMemcpy(host, b.small_list, need, hipMemcpyDeviceToHost)); return HDSR_OK;

@@ -67,6 +67,9 @@ class Engine {
   // Per-stage HIP events (one set per run, kept in a ring until harvested).
   void set_profile(bool on) { profile_ = on; }
   int stage_times(float* ms, uint32_t* runs);
+  // Sorts the chunks a deferred frame left unsorted (no-op otherwise), so that the entry lists
+  // are whole: for readers of the lists (debug readback).
+  int complete_lists(hipStream_t stream);
 
   const Bufs& bufs() const { return b_; }
   const Layout& last_layout() const { return lay_; }

@@ -564,6 +564,20 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
   return 0;
 }
 
+int Engine::complete_lists(hipStream_t stream) {
+  if (!deferred_) return 0;
+  Bufs r3 = b_;
+  r3.round = 3;
+  launch_segment_sort(r3, stream);
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    err = std::string("kernel launch failed: ") + hipGetErrorString(le);
+    return HDSR_E_HIP;
+  }
+  deferred_ = false;
+  return 0;
+}
+
 hipEvent_t* Engine::prof_slot() {
   if (prof_ev_.empty()) {
     prof_ev_.resize((size_t)kProfRing * kProfEvents);

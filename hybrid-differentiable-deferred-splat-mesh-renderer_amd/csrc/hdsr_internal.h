@@ -176,7 +176,8 @@ struct Bufs {
   // Deferred chunk sort (host-set round; 0 = the whole frame in one pass).  Round 1 sorts the small
   // tiles and the first chunk of every large tile, and rasterizes each large tile up to the end of
   // that chunk; a tile left unsaturated there gets tile_pend[morton] = 1 and its pixel state saved.
-  // Round 2 sorts the other chunks of those tiles only and resumes them.
+  // Round 2 sorts the other chunks of those tiles only and resumes them.  Round 3 sorts every chunk
+  // after the first: it completes the lists of a deferred frame for a reader of the whole lists.
   uint32_t round;
   uint32_t* tile_pend;       // [total_morton] large tiles only: 1 = resumed in round 2
   float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)

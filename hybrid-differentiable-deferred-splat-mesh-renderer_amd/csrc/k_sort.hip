@@ -267,10 +267,11 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
       bool take;
       if (S.large == kNone) {
         take = b.round == 1;
-      } else {  // round 1: the first chunk; round 2: the other chunks of the tiles resumed
+      } else {  // round 1: the first chunk; round 2: the other chunks of the tiles resumed;
+                // round 3: every other chunk (completes the lists of a deferred frame)
         const LargeTile& L = b.large_tiles[S.large];
         const bool first = S.dst == L.start;
-        take = b.round == 1 ? first : !first && b.tile_pend[L.morton] != 0u;
+        take = b.round == 1 ? first : b.round == 2 ? !first && b.tile_pend[L.morton] != 0u : !first;
       }
       if (!take) continue;
     }

@@ -43,6 +43,11 @@ def test_deferred_equals_one_pass_and_oracle(hip_device, faint, f16):
     lazy.sync()
     np.testing.assert_array_equal(lazy.read_rgba(), rgba)
     np.testing.assert_array_equal(lazy.read_depth(), depth)
+    # reading the entry lists back sorts the chunks the frame skipped: whole lists, as one pass
+    np.testing.assert_array_equal(lazy.debug("entries"), full.debug("entries"))
+    compositor.encodeCompositeOnly(None, lazy)  # now in one pass over the whole lists
+    lazy.sync()
+    np.testing.assert_array_equal(lazy.read_rgba(), rgba)
     # further frames on the context (its per-tile flags and saved state are rewritten each frame):
     # the same view again, then another view
     compositor.encodeFullComputeCompositorPipeline(None, lazy)
