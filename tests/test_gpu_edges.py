@@ -11,7 +11,7 @@ import pytest
 from hdsr import camera, compositor, scenes
 from oracle import oracle
 from scenes_for_tests import dense_scene, frame_inputs
-from test_gpu_parity import compare_all, render_gpu
+from test_gpu_parity import check_deferred, compare_all, render_gpu
 
 pytestmark = pytest.mark.gpu
 
@@ -26,6 +26,7 @@ def test_ragged_viewport(hip_device):
     orc = oracle.run(s, srt, u, cam.width, cam.height, oracle.FLAG_GBUFFER)
     res = render_gpu(s, srt, u, plan, gbuffer=True)
     compare_all(res, orc, plan, gbuffer=True, exact=True)
+    check_deferred(s, srt, u, plan, res, orc)
     res.destroy()
 
 
@@ -39,6 +40,7 @@ def test_scene_behind_the_camera(hip_device):
     res = render_gpu(s, srt, u, plan)
     compare_all(res, orc, plan, exact=True)
     np.testing.assert_array_equal(res.read_rgba().reshape(-1, 4), np.broadcast_to(BG, (160 * 96, 4)))
+    check_deferred(s, srt, u, plan, res, orc)
     res.destroy()
 
 
@@ -71,6 +73,7 @@ def test_degenerate_finite_attributes(hip_device):
     orc = oracle.run(s, srt, u, cam.width, cam.height)
     res = render_gpu(s, srt, u, plan)
     compare_all(res, orc, plan, exact=True)
+    check_deferred(s, srt, u, plan, res, orc)
     res.destroy()
 
 
@@ -95,6 +98,7 @@ def test_non_finite_attributes(hip_device):
     finite[bad] = False
     np.testing.assert_array_equal(res.debug("projcache").reshape(-1, 14)[finite],
                                   orc["cache"].reshape(-1, 14)[finite])
+    check_deferred(s, srt, u, plan, res)
     res.destroy()
 
 
@@ -114,4 +118,5 @@ def test_tied_keys_and_near_depth_order(hip_device):
         orc = oracle.run(s, order, u, cam.width, cam.height)
         res = render_gpu(s, order, u, plan)
         compare_all(res, orc, plan, exact=True)
+        check_deferred(s, order, u, plan, res, orc)
         res.destroy()
