@@ -24,6 +24,7 @@ struct hdsr_ctx {
   bool own_sorted = false;
   // frame
   bool have_frame = false;
+  uint32_t frame_sh_degree = 0;   // word 25 of the last uniforms
   FrameDesc frame{};
   // outputs
   float* rgba = nullptr;
@@ -79,7 +80,17 @@ int dev_copy(hdsr_ctx* c, T** dst, const T* src, size_t count) {
   return 0;
 }
 
-uint32_t sh_stride(uint32_t deg) { return 3u + 3u * ((deg + 1u) * (deg + 1u) - 1u) + 3u; }
+uint32_t sh_stride(uint32_t deg) { return sh_row_floats(deg); }
+
+// The frame's uniform shDegree against the bound shData (the shader derives the row stride from
+// the uniform, gpu_project_splats.wgsl:294-296, so a mismatch would read other splats' rows).
+int check_frame_degree(hdsr_ctx* c) {
+  if (c->have_splats && c->have_frame && c->frame_sh_degree != c->sh_degree)
+    return fail(c, HDSR_E_INVALID_ARG,
+                "uniform shDegree (" + std::to_string(c->frame_sh_degree) + ") differs from the bound shData's degree (" +
+                    std::to_string(c->sh_degree) + ")");
+  return HDSR_OK;
+}
 
 void fill_frame_scene(hdsr_ctx* c) {
   FrameDesc& F = c->frame;
@@ -195,7 +206,9 @@ void hdsr_destroy(hdsr_ctx* c) {
 int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
   if (!c || !s) return HDSR_E_INVALID_ARG;
   if (s->count != c->plan.splat_count) return fail(c, HDSR_E_INVALID_ARG, "splat count differs from the plan");
-  if (s->sh_degree > 3) return fail(c, HDSR_E_INVALID_ARG, "sh_degree must be 0..3");
+  uint32_t deg = 0;
+  const char* why = "";
+  if (!resolve_sh_degree(s->count, s->sh_degree, s->sh_data_len, &deg, &why)) return fail(c, HDSR_E_INVALID_ARG, why);
   if (s->count && (!s->positions || !s->scales || !s->rotations || !s->opacities || !s->materials || !s->sh_data ||
                    !s->sorted_indices))
     return fail(c, HDSR_E_INVALID_ARG, "missing splat buffer");
@@ -203,7 +216,7 @@ int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
   free_scene(c);
   const size_t n = s->count;
   c->n = s->count;
-  c->sh_degree = s->sh_degree;
+  c->sh_degree = deg;
   c->normals_len = s->normals_len;
   if (s->memory == HDSR_MEM_DEVICE) {
     c->pos = const_cast<float*>(s->positions);
@@ -226,11 +239,11 @@ int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
     if ((rc = dev_copy(c, &c->opac, s->opacities, n))) return rc;
     if ((rc = dev_copy(c, &c->material, s->materials, n))) return rc;
     if ((rc = dev_copy(c, &c->normals, s->normals, (size_t)s->normals_len))) return rc;
-    if ((rc = dev_copy(c, &c->sh, s->sh_data, n * sh_stride(s->sh_degree)))) return rc;
+    if ((rc = dev_copy(c, &c->sh, s->sh_data, n * sh_stride(deg)))) return rc;
     if ((rc = dev_copy(c, &c->sorted, s->sorted_indices, n))) return rc;
     if (n) {
       // one 128-byte-aligned record per splat for the projection's gathers
-      const uint32_t stride = sh_stride(s->sh_degree);
+      const uint32_t stride = sh_stride(deg);
       HIPX(c, hipMalloc((void**)&c->packed, n * (size_t)packed_stride4(stride) * sizeof(float4)));
       launch_pack_scene(c->pos, c->scale, c->rot, c->opac, c->material, c->sh, (uint32_t)n, stride, c->packed,
                         nullptr);
@@ -241,7 +254,7 @@ int hdsr_bind_splats(hdsr_ctx* c, const hdsr_splats* s) {
     return fail(c, HDSR_E_INVALID_ARG, "memory must be HDSR_MEM_HOST or HDSR_MEM_DEVICE");
   }
   c->have_splats = true;
-  return HDSR_OK;
+  return check_frame_degree(c);
 }
 
 int hdsr_update_sorted_indices(hdsr_ctx* c, const uint32_t* sorted, uint32_t memory) {
@@ -274,7 +287,12 @@ int hdsr_set_frame(hdsr_ctx* c, const void* u) {
     return fail(c, HDSR_E_INVALID_ARG, "uniform tile grid differs from the plan");
   if (w[22] != c->plan.splat_count) return fail(c, HDSR_E_INVALID_ARG, "uniform splatCount differs from the plan");
   if (w[25] > 3) return fail(c, HDSR_E_INVALID_ARG, "uniform shDegree must be 0..3");
+  if (c->have_splats && w[25] != c->sh_degree)
+    return fail(c, HDSR_E_INVALID_ARG,
+                "uniform shDegree (" + std::to_string(w[25]) + ") differs from the bound shData's degree (" +
+                    std::to_string(c->sh_degree) + ")");
   c->frame = f;
+  c->frame_sh_degree = w[25];
   c->have_frame = true;
   return HDSR_OK;
 }
@@ -308,6 +326,7 @@ static int use_device(hdsr_ctx* c) {
 int hdsr_render(hdsr_ctx* c, void* stream) {
   if (!c) return HDSR_E_INVALID_ARG;
   if (!c->have_splats || !c->have_frame) return fail(c, HDSR_E_STATE, "bind splats and set a frame first");
+  if (int rc = check_frame_degree(c)) return rc;
   if (int rc = use_device(c)) return rc;
   fill_frame_scene(c);
   const int rc = c->eng.run(&c->frame, 1, static_cast<hipStream_t>(stream), (c->flags & HDSR_GBUFFER) != 0,

@@ -37,6 +37,40 @@ constexpr uint32_t kCellTiles = 8;            // a cell is 8 x 8 tiles
 constexpr uint32_t kCellChunk = 1024;         // cell-list entries per expansion workgroup
 constexpr uint32_t kMaxCellsPerSplat = 9;     // a tile box spans at most 17 tiles per axis
 
+// Floats per splat of a shData row at SH degree deg: dc (3) + 3K coefficients + emissive (3),
+// K = (deg+1)^2 - 1 (gpu_project_splats.wgsl:294-296).
+inline uint32_t sh_row_floats(uint32_t deg) { return 3u + 3u * ((deg + 1u) * (deg + 1u) - 1u) + 3u; }
+
+// The SH degree of a bind: deg_in 0..3, or 0xFFFFFFFF (take it from sh_len, the floats in shData,
+// which must be count * sh_row_floats(d) for exactly one d).  A stated sh_len must hold the rows.
+// Returns false with *why set when the buffers cannot be read at that degree.
+inline bool resolve_sh_degree(uint32_t count, uint32_t deg_in, uint64_t sh_len, uint32_t* deg_out,
+                              const char** why) {
+  if (deg_in == 0xFFFFFFFFu) {
+    if (count == 0) {
+      *deg_out = 0;
+      return true;
+    }
+    for (uint32_t d = 0; d <= 3; ++d)
+      if (sh_len == (uint64_t)count * sh_row_floats(d)) {
+        *deg_out = d;
+        return true;
+      }
+    *why = "shData length matches no SH degree: it must hold count * (6 + 3K) floats, K = (d+1)^2 - 1";
+    return false;
+  }
+  if (deg_in > 3) {
+    *why = "sh_degree must be 0..3";
+    return false;
+  }
+  if (sh_len != 0 && sh_len < (uint64_t)count * sh_row_floats(deg_in)) {
+    *why = "shData is shorter than count * (6 + 3K) floats for the stated sh_degree";
+    return false;
+  }
+  *deg_out = deg_in;
+  return true;
+}
+
 struct FrameDesc {
   // scene (device pointers)
   const float* pos;

@@ -35,7 +35,7 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-uint32_t sh_stride_of(uint32_t deg) { return 3u + 3u * ((deg + 1u) * (deg + 1u) - 1u) + 3u; }
+uint32_t sh_stride_of(uint32_t deg) { return sh_row_floats(deg); }
 
 struct Scene {
   hdsr_splats host{};
@@ -383,7 +383,10 @@ void hdsr_server_destroy(hdsr_server* s) {
 int hdsr_server_add_scene(hdsr_server* s, uint32_t id, const hdsr_splats* sp) {
   if (!s || !sp) return HDSR_E_INVALID_ARG;
   if (sp->memory != HDSR_MEM_HOST) return s->fail(HDSR_E_INVALID_ARG, "server scenes are registered from host memory");
-  if (sp->sh_degree > 3) return s->fail(HDSR_E_INVALID_ARG, "sh_degree must be 0..3");
+  uint32_t deg = 0;
+  const char* why = "";
+  if (!resolve_sh_degree(sp->count, sp->sh_degree, sp->sh_data_len, &deg, &why))
+    return s->fail(HDSR_E_INVALID_ARG, why);
   if (sp->count && (!sp->positions || !sp->scales || !sp->rotations || !sp->opacities || !sp->materials ||
                     !sp->sh_data))
     return s->fail(HDSR_E_INVALID_ARG, "missing splat array");
@@ -396,9 +399,10 @@ int hdsr_server_add_scene(hdsr_server* s, uint32_t id, const hdsr_splats* sp) {
   }
   Scene sc;
   sc.host = *sp;
+  sc.host.sh_degree = deg;
   sc.n = sp->count;
-  sc.sh_degree = sp->sh_degree;
-  sc.bytes = (uint64_t)sp->count * (4 + 16ull * packed_stride4(sh_stride_of(sp->sh_degree))) + sp->normals_len * 4;
+  sc.sh_degree = deg;
+  sc.bytes = (uint64_t)sp->count * (4 + 16ull * packed_stride4(sh_stride_of(deg))) + sp->normals_len * 4;
   if (s->cfg.hbm_budget_bytes && sc.bytes > s->cfg.hbm_budget_bytes)
     return s->fail(HDSR_E_CAPACITY, "scene larger than the whole budget");
   s->scenes[id] = sc;

@@ -30,6 +30,7 @@ STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "s
 
 HDSR_MEM_HOST = 0
 HDSR_MEM_DEVICE = 1
+HDSR_SH_DEGREE_AUTO = 0xFFFFFFFF  # take the SH degree from the shData length
 
 OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX, OUT_FILTERED_NORMAL = range(8)
 
@@ -55,7 +56,8 @@ class Splats(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("scales", C.c_void_p), ("rotations", C.c_void_p),
                 ("opacities", C.c_void_p), ("materials", C.c_void_p), ("normals", C.c_void_p),
                 ("normals_len", C.c_uint64), ("sh_data", C.c_void_p), ("sorted_indices", C.c_void_p),
-                ("count", C.c_uint32), ("sh_degree", C.c_uint32), ("memory", C.c_uint32)]
+                ("count", C.c_uint32), ("sh_degree", C.c_uint32), ("memory", C.c_uint32),
+                ("sh_data_len", C.c_uint64)]
 
 
 class Outputs(C.Structure):

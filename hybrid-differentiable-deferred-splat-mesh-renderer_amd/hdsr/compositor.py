@@ -155,6 +155,9 @@ class Resources:
         check(self._lib.hdsr_set_frame(self.handle, u.ctypes.data), self.handle)
 
     def update_sorted_indices(self, sorted_indices) -> None:
+        if _numel(sorted_indices) != self.plan["splatCount"]:
+            raise ValueError(f"sortedIndices holds {_numel(sorted_indices)} elements; splatCount is "
+                             f"{self.plan['splatCount']}")
         if hasattr(sorted_indices, "data_ptr"):
             check(self._lib.hdsr_update_sorted_indices(self.handle, sorted_indices.data_ptr(), _lib.HDSR_MEM_DEVICE),
                   self.handle)
@@ -208,17 +211,30 @@ def _host_or_device(arr, dtype):
     return a.ctypes.data, _lib.HDSR_MEM_HOST, a
 
 
+def _numel(arr) -> int:
+    return int(arr.numel()) if hasattr(arr, "numel") else int(np.asarray(arr).size)
+
+
 def createTileSplatBindGroups(device, resources: Resources, splatBuffers: dict, outputTexture=None,
                               gbufferTextures=None, *, shDegree: int | None = None) -> BindGroups:
     """splatBuffers: positionBuffer, scaleBuffer, rotationBuffer, opacityBuffer, materialBuffer,
-    normalBuffer (optional), shDataBuffer, sortedIndexBuffer, shDegree (or keyword)."""
-    deg = int(shDegree if shDegree is not None else splatBuffers.get("shDegree", 0))
-    keys = [("positionBuffer", np.float32), ("scaleBuffer", np.float32), ("rotationBuffer", np.float32),
-            ("opacityBuffer", np.float32), ("materialBuffer", np.uint32), ("normalBuffer", np.float32),
-            ("shDataBuffer", np.float32), ("sortedIndexBuffer", np.uint32)]
+    normalBuffer (optional), shDataBuffer, sortedIndexBuffer; the keys of the reference's call
+    (src/splatRenderer.ts:946).  The SH degree is not part of that call: the reference reads it
+    from the frame uniforms.  Here it is taken from the shDataBuffer length (N * (6 + 3K) floats)
+    unless shDegree is given (keyword or splatBuffers key); the frame uniforms' shDegree must then
+    agree.  Every buffer's length is checked against splatCount (ValueError on a mismatch)."""
+    n = int(resources.plan["splatCount"])
+    deg = shDegree if shDegree is not None else splatBuffers.get("shDegree")
+    deg = _lib.HDSR_SH_DEGREE_AUTO if deg is None else int(deg)
+    keys = [("positionBuffer", np.float32, 3), ("scaleBuffer", np.float32, 3), ("rotationBuffer", np.float32, 4),
+            ("opacityBuffer", np.float32, 1), ("materialBuffer", np.uint32, 1), ("normalBuffer", np.float32, None),
+            ("shDataBuffer", np.float32, None), ("sortedIndexBuffer", np.uint32, 1)]
     ptrs, kinds, keep = {}, set(), []
-    for k, dt in keys:
-        p, kind, ka = _host_or_device(splatBuffers.get(k), dt)
+    for k, dt, per in keys:
+        arr = splatBuffers.get(k)
+        if arr is not None and per is not None and _numel(arr) != per * n:
+            raise ValueError(f"{k} holds {_numel(arr)} elements; splatCount {n} needs {per * n}")
+        p, kind, ka = _host_or_device(arr, dt)
         ptrs[k] = p
         if kind is not None:
             kinds.add(kind)
@@ -227,10 +243,11 @@ def createTileSplatBindGroups(device, resources: Resources, splatBuffers: dict, 
         raise ValueError("splat buffers must all be host arrays or all device tensors")
     mem = kinds.pop() if kinds else _lib.HDSR_MEM_HOST
     nb = splatBuffers.get("normalBuffer")
-    nlen = 0 if nb is None else int(np.prod(nb.shape))
+    nlen = 0 if nb is None else _numel(nb)  # any length: normal reads are bounded by it
+    sb = splatBuffers.get("shDataBuffer")
     s = _lib.Splats(ptrs["positionBuffer"], ptrs["scaleBuffer"], ptrs["rotationBuffer"], ptrs["opacityBuffer"],
                     ptrs["materialBuffer"], ptrs["normalBuffer"], nlen, ptrs["shDataBuffer"],
-                    ptrs["sortedIndexBuffer"], resources.plan["splatCount"], deg, mem)
+                    ptrs["sortedIndexBuffer"], n, deg, mem, 0 if sb is None else _numel(sb))
     check(resources._lib.hdsr_bind_splats(resources.handle, C.byref(s)), resources.handle)
     if mem == _lib.HDSR_MEM_DEVICE:
         resources._keep = keep
@@ -260,4 +277,4 @@ def bind_scene(resources: Resources, scene, sorted_indices) -> BindGroups:
         "positionBuffer": scene.positions, "scaleBuffer": scene.scales, "rotationBuffer": scene.rotations,
         "opacityBuffer": scene.opacities, "materialBuffer": scene.materials, "normalBuffer": scene.normals,
         "shDataBuffer": scene.sh_data(), "sortedIndexBuffer": sorted_indices,
-    }, shDegree=scene.sh_degree)
+    })

@@ -79,7 +79,7 @@ class DiffRenderer:
         means, log_scales, quats, logits, sh = ps
         s = _lib.Splats(means.data_ptr(), log_scales.data_ptr(), quats.data_ptr(), logits.data_ptr(),
                         self.materials.data_ptr(), None, 0, sh.data_ptr(), self.sorted.data_ptr(), self.n,
-                        self.sh_degree, _lib.HDSR_MEM_DEVICE)
+                        self.sh_degree, _lib.HDSR_MEM_DEVICE, sh.numel())
         check(self._lib.hdsr_bind_splats(self.handle, C.byref(s)), self.handle)
         self._bound = key
 

@@ -64,7 +64,8 @@ class Server:
         srt = None if sorted_indices is None else np.ascontiguousarray(sorted_indices, np.uint32)
         s = _lib.Splats(arrays[0].ctypes.data, arrays[1].ctypes.data, arrays[2].ctypes.data, arrays[3].ctypes.data,
                         arrays[4].ctypes.data, None, 0, arrays[5].ctypes.data,
-                        None if srt is None else srt.ctypes.data, scene.count, scene.sh_degree, _lib.HDSR_MEM_HOST)
+                        None if srt is None else srt.ctypes.data, scene.count, scene.sh_degree, _lib.HDSR_MEM_HOST,
+                        arrays[5].size)
         self._check(self._lib.hdsr_server_add_scene(self.handle, scene_id, C.byref(s)))
         self._keep[scene_id] = (arrays, srt)
         self._meta[scene_id] = (scene.count, scene.sh_degree)

@@ -128,9 +128,13 @@ static void plan_from_js(napi_env env, napi_value obj, hdsr_plan* p) {
 }
 
 /* The external holds a box so that destroy() can release the context early and the finalizer
- * (run at garbage collection) then only frees the box. */
+ * (run at garbage collection) then only frees the box.  While a renderAsync() frame is in flight
+ * the worker thread owns the context: 'busy' makes every other call on it throw, and a destroy()
+ * in that window is deferred to the frame's completion ('destroy_pending'). */
 typedef struct {
   hdsr_ctx* ctx;
+  int busy;
+  int destroy_pending;
 } ctx_box;
 
 static ctx_box* box_arg(napi_env env, napi_value v) {
@@ -145,8 +149,12 @@ static ctx_box* box_arg(napi_env env, napi_value v) {
 static hdsr_ctx* ctx_arg(napi_env env, napi_value v) {
   ctx_box* b = box_arg(env, v);
   if (!b) return NULL;
-  if (!b->ctx) {
+  if (!b->ctx || b->destroy_pending) {
     napi_throw_error(env, NULL, "compositor handle already destroyed");
+    return NULL;
+  }
+  if (b->busy) {
+    napi_throw_error(env, NULL, "compositor is busy: a renderAsync() frame is in flight; wait for its Promise");
     return NULL;
   }
   return b->ctx;
@@ -207,6 +215,8 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     return NULL;
   }
   box->ctx = ctx;
+  box->busy = 0;
+  box->destroy_pending = 0;
   napi_value h;
   if (napi_create_external(env, box, finalize_ctx, NULL, &h) != napi_ok) {
     hdsr_destroy(ctx);
@@ -217,49 +227,161 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
   return h;
 }
 
-/* destroy(handle): releases now; the handle must not be used again. */
+/* destroy(handle): releases now, or - while a renderAsync() frame is in flight - as soon as that
+ * frame completes; the handle must not be used again either way. */
 static napi_value js_destroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value argv[1];
   CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   ctx_box* b = box_arg(env, argv[0]);
   if (!b) return NULL;
+  if (b->busy) {
+    b->destroy_pending = 1;
+    return NULL;
+  }
   if (b->ctx) hdsr_destroy(b->ctx);
   b->ctx = NULL;
   return NULL;
 }
 
-/* bindSplats(handle, {positions, scales, rotations, opacities, materials, normals?, shData,
- *                     sortedIndices, shDegree})  -- host typed arrays, copied to the device */
+/* Element count of typed array property 'key' against 'want' elements ('exact': equal, else at
+ * least); throws a RangeError naming the property on a mismatch.  Returns 0 when it threw. */
+static int check_len(napi_env env, const char* key, size_t bytes, size_t elem, uint64_t want, int exact) {
+  const uint64_t have = bytes / elem;
+  if (exact ? have == want : have >= want) return 1;
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s holds %llu elements; %s%llu expected for the plan's splatCount", key,
+           (unsigned long long)have, exact ? "" : "at least ", (unsigned long long)want);
+  napi_throw_range_error(env, NULL, buf);
+  return 0;
+}
+
+/* bindSplats(handle, count, {positions, scales, rotations, opacities, materials, normals?, shData,
+ *                            sortedIndices, shDegree?})  -- host typed arrays, copied to the
+ * device.  Every array's length is checked against 'count' (the plan's splatCount) before
+ * anything is read: positions / scales 3N, rotations 4N, opacities / materials / sortedIndices N,
+ * shData N * (6 + 3K); normals any length (reads are bounded by it).  Without shDegree the degree is taken from the
+ * shData length, as the reference's bind names no degree (src/splatRenderer.ts:946). */
 static napi_value js_bind(napi_env env, napi_callback_info info) {
-  size_t argc = 2;
-  napi_value argv[2];
+  size_t argc = 3;
+  napi_value argv[3];
   CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  if (argc < 3) {
+    napi_throw_type_error(env, NULL, "bindSplats(handle, count, buffers)");
+    return NULL;
+  }
   hdsr_ctx* ctx = ctx_arg(env, argv[0]);
   if (!ctx) return NULL;
-  size_t nb = 0, tmp = 0;
+  uint32_t n = 0;
+  if (napi_get_value_uint32(env, argv[1], &n) != napi_ok) {
+    napi_throw_type_error(env, NULL, "bindSplats: count must be a number");
+    return NULL;
+  }
+  size_t b = 0;
   hdsr_splats s;
   memset(&s, 0, sizeof s);
-  s.positions = (const float*)typed_prop(env, argv[1], "positions", &nb);
-  s.count = (uint32_t)(nb / 12);
-  s.scales = (const float*)typed_prop(env, argv[1], "scales", &tmp);
-  s.rotations = (const float*)typed_prop(env, argv[1], "rotations", &tmp);
-  s.opacities = (const float*)typed_prop(env, argv[1], "opacities", &tmp);
-  s.materials = (const uint32_t*)typed_prop(env, argv[1], "materials", &tmp);
-  s.normals = (const float*)typed_prop(env, argv[1], "normals", &tmp);
-  s.normals_len = tmp / 4;
-  s.sh_data = (const float*)typed_prop(env, argv[1], "shData", &tmp);
-  s.sorted_indices = (const uint32_t*)typed_prop(env, argv[1], "sortedIndices", &tmp);
-  s.sh_degree = get_u32(env, argv[1], "shDegree", 0);
+  s.count = n;
+  s.positions = (const float*)typed_prop(env, argv[2], "positions", &b);
+  if (!check_len(env, "positionBuffer", b, 4, 3ull * n, 1)) return NULL;
+  s.scales = (const float*)typed_prop(env, argv[2], "scales", &b);
+  if (!check_len(env, "scaleBuffer", b, 4, 3ull * n, 1)) return NULL;
+  s.rotations = (const float*)typed_prop(env, argv[2], "rotations", &b);
+  if (!check_len(env, "rotationBuffer", b, 4, 4ull * n, 1)) return NULL;
+  s.opacities = (const float*)typed_prop(env, argv[2], "opacities", &b);
+  if (!check_len(env, "opacityBuffer", b, 4, n, 1)) return NULL;
+  s.materials = (const uint32_t*)typed_prop(env, argv[2], "materials", &b);
+  if (!check_len(env, "materialBuffer", b, 4, n, 1)) return NULL;
+  s.sorted_indices = (const uint32_t*)typed_prop(env, argv[2], "sortedIndices", &b);
+  if (!check_len(env, "sortedIndexBuffer", b, 4, n, 1)) return NULL;
+  /* any length: the projection reads a splat's normal only where the buffer holds it, as the
+   * shader bounds its reads by arrayLength (gpu_project_splats.wgsl:237,252) */
+  s.normals = (const float*)typed_prop(env, argv[2], "normals", &b);
+  s.normals_len = b / 4;
+  s.sh_data = (const float*)typed_prop(env, argv[2], "shData", &b);
+  s.sh_data_len = b / 4;
+  s.sh_degree = get_u32(env, argv[2], "shDegree", HDSR_SH_DEGREE_AUTO);
+  if (s.sh_degree == HDSR_SH_DEGREE_AUTO && n) {
+    int found = 0;
+    for (uint64_t d = 0; d <= 3; ++d) found |= s.sh_data_len == (6 + 3 * ((d + 1) * (d + 1) - 1)) * (uint64_t)n;
+    if (!found) {
+      char buf[256];
+      snprintf(buf, sizeof buf,
+               "shDataBuffer holds %llu floats, which matches no SH degree: %u splats need 6, 15, 30 or 51 each",
+               (unsigned long long)s.sh_data_len, n);
+      napi_throw_range_error(env, NULL, buf);
+      return NULL;
+    }
+  }
+  if (s.sh_degree != HDSR_SH_DEGREE_AUTO && s.sh_degree <= 3) {
+    const uint64_t k = (uint64_t)(s.sh_degree + 1) * (s.sh_degree + 1) - 1;
+    if (!check_len(env, "shDataBuffer", b, 4, (6 + 3 * k) * (uint64_t)n, 0)) return NULL;
+  }
   s.memory = HDSR_MEM_HOST;
   const int rc = hdsr_bind_splats(ctx, &s);
   if (rc) return throw_rc(env, rc, ctx, "hdsr_bind_splats");
   return NULL;
 }
 
+/* bindOutputs(handle, rgba, depth): caller-owned device buffers (BigInt device addresses, or
+ * null for the context's own) that later renders write - hdsr_bind_outputs. */
+static napi_value js_bind_outputs(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  void* ptrs[2] = {NULL, NULL};
+  for (int i = 0; i < 2; ++i) {
+    if (argc <= (size_t)(i + 1)) break;
+    napi_valuetype t;
+    napi_typeof(env, argv[i + 1], &t);
+    if (t == napi_null || t == napi_undefined) continue;
+    if (t != napi_bigint) {
+      napi_throw_type_error(env, NULL, "bindOutputs: device addresses must be BigInt (or null)");
+      return NULL;
+    }
+    uint64_t v = 0;
+    bool lossless = false;
+    CALL(env, napi_get_value_bigint_uint64(env, argv[i + 1], &v, &lossless));
+    ptrs[i] = (void*)(uintptr_t)v;
+  }
+  const int rc = hdsr_bind_outputs(ctx, (float*)ptrs[0], (float*)ptrs[1]);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_bind_outputs");
+  return NULL;
+}
+
+/* outputs(handle) -> {rgba, depth, gbufferNormal, gbufferMaterial, rgba16} device addresses
+ * (BigInt, 0n when absent), width, height - hdsr_get_outputs. */
+static napi_value js_outputs(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  hdsr_outputs o;
+  const int rc = hdsr_get_outputs(ctx, &o);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_get_outputs");
+  napi_value out, v;
+  CALL(env, napi_create_object(env, &out));
+  const struct {
+    const char* key;
+    const void* p;
+  } f[] = {{"rgba", o.rgba}, {"depth", o.depth}, {"gbufferNormal", o.gbuffer_normal},
+           {"gbufferMaterial", o.gbuffer_material}, {"rgba16", o.rgba16}};
+  for (size_t i = 0; i < sizeof f / sizeof f[0]; ++i) {
+    CALL(env, napi_create_bigint_uint64(env, (uint64_t)(uintptr_t)f[i].p, &v));
+    CALL(env, napi_set_named_property(env, out, f[i].key, v));
+  }
+  set_u32(env, out, "width", o.width);
+  set_u32(env, out, "height", o.height);
+  return out;
+}
+
+/* updateSortedIndices(handle, Uint32Array, count): hdsr_update_sorted_indices after a length
+ * check against count (the plan's splatCount). */
 static napi_value js_update_sorted(napi_env env, napi_callback_info info) {
-  size_t argc = 2;
-  napi_value argv[2];
+  size_t argc = 3;
+  napi_value argv[3];
   CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
   hdsr_ctx* ctx = ctx_arg(env, argv[0]);
   if (!ctx) return NULL;
@@ -268,6 +390,15 @@ static napi_value js_update_sorted(napi_env env, napi_callback_info info) {
   CALL(env, napi_set_named_property(env, holder, "a", argv[1]));
   size_t bytes = 0;
   const void* p = typed_prop(env, holder, "a", &bytes);
+  uint32_t n = 0;
+  if (argc < 3 || napi_get_value_uint32(env, argv[2], &n) != napi_ok) {
+    napi_throw_type_error(env, NULL, "updateSortedIndices(handle, sortedIndices, count)");
+    return NULL;
+  }
+  if (!p || !check_len(env, "sortedIndices", bytes, 4, n, 1)) {
+    if (!p) napi_throw_type_error(env, NULL, "sortedIndices must be a Uint32Array");
+    return NULL;
+  }
   const int rc = hdsr_update_sorted_indices(ctx, (const uint32_t*)p, HDSR_MEM_HOST);
   if (rc) return throw_rc(env, rc, ctx, "hdsr_update_sorted_indices");
   return NULL;
@@ -313,6 +444,7 @@ typedef struct {
   napi_async_work work;
   napi_deferred deferred;
   napi_ref handle_ref; /* keeps the context's external alive while the worker uses it */
+  ctx_box* box;
   hdsr_ctx* ctx;
   int rc;
   const char* what;
@@ -333,6 +465,11 @@ static void render_execute(napi_env env, void* data) {
 
 static void render_complete(napi_env env, napi_status status, void* data) {
   render_job* j = (render_job*)data;
+  j->box->busy = 0;
+  if (j->box->destroy_pending) {  /* destroy() was called while the frame ran */
+    if (j->box->ctx) hdsr_destroy(j->box->ctx);
+    j->box->ctx = NULL;
+  }
   if (status == napi_ok && j->rc == 0) {
     napi_value undef;
     napi_get_undefined(env, &undef);
@@ -365,11 +502,24 @@ static napi_value js_render_async(napi_env env, napi_callback_info info) {
     return NULL;
   }
   j->ctx = ctx;
-  CALL(env, napi_create_promise(env, &j->deferred, &promise));
-  CALL(env, napi_create_reference(env, argv[0], 1, &j->handle_ref));
-  CALL(env, napi_create_string_utf8(env, "hdsr.renderAsync", NAPI_AUTO_LENGTH, &name));
-  CALL(env, napi_create_async_work(env, NULL, name, render_execute, render_complete, j, &j->work));
-  CALL(env, napi_queue_async_work(env, j->work));
+  j->box = box_arg(env, argv[0]);
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_reference(env, argv[0], 1, &j->handle_ref) != napi_ok ||
+      napi_create_string_utf8(env, "hdsr.renderAsync", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, NULL, name, render_execute, render_complete, j, &j->work) != napi_ok) {
+    free(j);
+    napi_throw_error(env, NULL, "renderAsync: N-API setup failed");
+    return NULL;
+  }
+  j->box->busy = 1;
+  if (napi_queue_async_work(env, j->work) != napi_ok) {
+    j->box->busy = 0;
+    napi_delete_reference(env, j->handle_ref);
+    napi_delete_async_work(env, j->work);
+    free(j);
+    napi_throw_error(env, NULL, "renderAsync: could not queue the work");
+    return NULL;
+  }
   return promise;
 }
 
@@ -420,6 +570,10 @@ static napi_value js_readback(napi_env env, napi_callback_info info) {
   CALL(env, napi_set_named_property(env, holder, "a", argv[2]));
   size_t bytes = 0;
   void* p = (void*)typed_prop(env, holder, "a", &bytes);
+  if (!p) {
+    napi_throw_type_error(env, NULL, "readback: the destination must be a typed array");
+    return NULL;
+  }
   const int rc = hdsr_readback(ctx, which, p, bytes);
   if (rc) return throw_rc(env, rc, ctx, "hdsr_readback");
   return NULL;
@@ -656,6 +810,8 @@ static napi_value init(napi_env env, napi_value exports) {
       {"create", NULL, js_create, NULL, NULL, NULL, napi_default, NULL},
       {"destroy", NULL, js_destroy, NULL, NULL, NULL, napi_default, NULL},
       {"bindSplats", NULL, js_bind, NULL, NULL, NULL, napi_default, NULL},
+      {"bindOutputs", NULL, js_bind_outputs, NULL, NULL, NULL, napi_default, NULL},
+      {"outputs", NULL, js_outputs, NULL, NULL, NULL, napi_default, NULL},
       {"updateSortedIndices", NULL, js_update_sorted, NULL, NULL, NULL, napi_default, NULL},
       {"setFrame", NULL, js_set_frame, NULL, NULL, NULL, napi_default, NULL},
       {"render", NULL, js_render, NULL, NULL, NULL, napi_default, NULL},

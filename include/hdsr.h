@@ -84,7 +84,15 @@ void hdsr_destroy(hdsr_ctx* ctx);
  *   materials N u32 (pack2x16float(roughness, metalness)), normals: normals_len f32
  *   ([geometric 3N | detail 3N], or 0 = derive from the covariance), sh_data N*(3+3K+3) f32
  *   (dc rgb, K = (d+1)^2-1 coefficients rgb-interleaved, emissive rgb),
- *   sorted_indices N u32 (a permutation; the identity is allowed). */
+ *   sorted_indices N u32 (a permutation; the identity is allowed).
+ * The reference binds shData without a degree and reads the degree from the frame uniforms
+ * (u32 word 25, gpu_project_splats.wgsl:294-296; the bind call src/splatRenderer.ts:946 names no
+ * degree).  Here sh_degree = HDSR_SH_DEGREE_AUTO takes it from sh_data_len: N*(6+3K) floats
+ * must match exactly one degree 0..3 (6, 15, 30 or 51 floats per splat).  With an explicit
+ * sh_degree, a nonzero sh_data_len must hold at least N*(6+3K) floats.  Either way the degree of
+ * every frame's uniforms (hdsr_set_frame) must equal the bound degree, or the call fails with
+ * HDSR_E_INVALID_ARG: a WebGPU host with mismatched buffers would read other splats' rows. */
+#define HDSR_SH_DEGREE_AUTO 0xFFFFFFFFu
 typedef struct hdsr_splats {
   const float* positions;
   const float* scales;
@@ -96,8 +104,9 @@ typedef struct hdsr_splats {
   const float* sh_data;
   const uint32_t* sorted_indices;
   uint32_t count;
-  uint32_t sh_degree;
+  uint32_t sh_degree;             /* 0..3 or HDSR_SH_DEGREE_AUTO */
   uint32_t memory;                /* HDSR_MEM_HOST or HDSR_MEM_DEVICE */
+  uint64_t sh_data_len;           /* floats in sh_data; 0 = not stated (needs an explicit degree) */
 } hdsr_splats;
 
 /* createTileSplatBindGroups (gpuTileSplatCompositor.ts:792-995). */
@@ -108,7 +117,9 @@ int hdsr_bind_splats(hdsr_ctx* ctx, const hdsr_splats* splats);
 int hdsr_update_sorted_indices(hdsr_ctx* ctx, const uint32_t* sorted_indices, uint32_t memory);
 
 /* queue.writeBuffer(frameUniformBuffer, ...) of writeTileSplatFrameUniforms
- * (gpuTileSplatCompositor.ts:730-774; src/splatRenderer.ts:1030): the 256-byte block verbatim. */
+ * (gpuTileSplatCompositor.ts:730-774; src/splatRenderer.ts:1030): the 256-byte block verbatim.
+ * Its tile grid and splatCount must equal the plan's, and its shDegree (word 25) the bound
+ * splats' degree (checked here once splats are bound, else by hdsr_bind_splats and hdsr_render). */
 int hdsr_set_frame(hdsr_ctx* ctx, const void* uniforms256);
 
 /* The sortedIndices producer on the GPU: createViewDepthSortInput + encodeGpuSortPrototype

@@ -910,6 +910,7 @@ enum {
   ORC_RGBA64,
   ORC_DEPTH64,
   ORC_RAW_INVERSIONS,
+  ORC_DIFFREC,  // differentiable mode: (opacity, r, g, b) fp32 per rank, as projected
   ORC_NUM
 };
 
@@ -1009,6 +1010,7 @@ uint64_t orc_size(void* h, int which) {
     case ORC_RGBA64: return F->rgba64.size() * 8;
     case ORC_DEPTH64: return F->depth64.size() * 8;
     case ORC_RAW_INVERSIONS: return F->raw_order_inversions.size() * 4;
+    case ORC_DIFFREC: return F->diffrec.size() * 4;
     default: return 0;
   }
 }
@@ -1039,6 +1041,7 @@ int orc_get(void* h, int which, void* dst, uint64_t bytes) {
     case ORC_RGBA64: src = F->rgba64.data(); break;
     case ORC_DEPTH64: src = F->depth64.data(); break;
     case ORC_RAW_INVERSIONS: src = F->raw_order_inversions.data(); break;
+    case ORC_DIFFREC: src = F->diffrec.data(); break;
     default: return -2;
   }
   if (n) memcpy(dst, src, n);

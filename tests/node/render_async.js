@@ -15,7 +15,7 @@ var resources = C.createTileSplatCompositor(device, plan, {});
 var bindGroups = C.createTileSplatBindGroups(device, resources, {
   positionBuffer: f32('positions.bin'), scaleBuffer: f32('scales.bin'), rotationBuffer: f32('rotations.bin'),
   opacityBuffer: f32('opacities.bin'), materialBuffer: u32('materials.bin'), shDataBuffer: f32('shdata.bin'),
-  sortedIndexBuffer: u32('sorted.bin'), shDegree: meta.shDegree,
+  sortedIndexBuffer: u32('sorted.bin'),
 }, null, null);
 var target = new Float32Array(C.TILE_SPLAT_FRAME_UNIFORM_BYTES / 4);
 C.writeTileSplatFrameUniforms(target, f32('viewproj.bin'), plan, 1.0, meta.shDegree, f32('campos.bin'), f32('view.bin'), f32('proj.bin'));

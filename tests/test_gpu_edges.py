@@ -120,3 +120,33 @@ def test_tied_keys_and_near_depth_order(hip_device):
         compare_all(res, orc, plan, exact=True)
         check_deferred(s, order, u, plan, res, orc)
         res.destroy()
+
+
+def test_sh_degree_from_shdata_and_uniform_mismatch(hip_device):
+    """The bind takes the SH degree from the shData length when none is given (the reference's bind
+    names none, src/splatRenderer.ts:946); frame uniforms with another shDegree are rejected, a
+    degree-less bind whose shData matches no degree is rejected, and an explicit degree whose rows
+    the shData cannot hold is rejected."""
+    from hdsr._lib import HdsrError
+    s = dense_scene(n=3000, seed=44, sh_degree=3, core_fraction=0.3)
+    cam = camera.Camera(camera.orbit_eye(2.8, 0.3, 0.2), (0, 0, 0), 360.0, 360.0, 192, 128)
+    plan, srt, u = frame_inputs(s, cam)
+    res = compositor.createTileSplatCompositor(None, plan)
+    compositor.bind_scene(res, s, srt)  # no degree given
+    bad = u.copy()
+    bad.view(np.uint32)[25] = 0
+    with pytest.raises(HdsrError, match="differs from the bound shData"):
+        res.set_frame(bad)
+    res.set_frame(u)
+    compositor.encodeFullComputeCompositorPipeline(None, res)
+    res.sync()
+    orc = oracle.run(s, srt, u, cam.width, cam.height)
+    assert np.abs(res.read_rgba().reshape(-1) - orc["rgba"]).max() <= 1e-5
+    bufs = {"positionBuffer": s.positions, "scaleBuffer": s.scales, "rotationBuffer": s.rotations,
+            "opacityBuffer": s.opacities, "materialBuffer": s.materials, "sortedIndexBuffer": srt}
+    with pytest.raises(HdsrError, match="matches no SH degree"):
+        compositor.createTileSplatBindGroups(None, res, {**bufs, "shDataBuffer": s.sh_data()[:-7]})
+    with pytest.raises(HdsrError, match="shorter than"):
+        compositor.createTileSplatBindGroups(None, res, {**bufs, "shDataBuffer": s.sh_data()[:30 * s.count]},
+                                              shDegree=3)
+    res.destroy()

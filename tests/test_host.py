@@ -190,3 +190,74 @@ def test_gpu_sort_order_restatement():
     a = orc.gpu_sort_order(orc.view_depth_keys_f32(sc.positions, cam.view))
     b = scenes.sort_back_to_front(sc.positions, cam.view)
     assert np.mean(a != b) < 0.01
+
+
+# FrameUniforms (src/shaders/gpu_project_splats.wgsl:27-44): WGSL field, byte offset, word type,
+# words.  mat3x3f columns are vec3 with a 16-byte stride (a pad word after each column).
+FRAME_UNIFORMS_LAYOUT = [
+    ("viewProj", 0, "f32", 16), ("viewport", 64, "f32", 2), ("tileSizePx", 72, "f32", 1),
+    ("debugMode", 76, "f32", 1), ("tileGrid", 80, "u32", 2), ("splatCount", 88, "u32", 1),
+    ("totalTileRefs", 92, "u32", 1), ("splatScale", 96, "f32", 1), ("shDegree", 100, "u32", 1),
+    ("_pad0", 104, "u32", 2), ("cameraPos", 112, "f32", 3), ("_pad1", 124, "f32", 1),
+    ("viewMatrix", 128, "f32", 16), ("focal", 192, "f32", 2), ("_pad2", 200, "f32", 2),
+    ("normalMatrix.c0", 208, "f32", 3), ("_padc0", 220, "f32", 1), ("normalMatrix.c1", 224, "f32", 3),
+    ("_padc1", 236, "f32", 1), ("normalMatrix.c2", 240, "f32", 3), ("_padc2", 252, "f32", 1),
+]
+
+
+def test_frame_uniform_layout_word_by_word():
+    """The 256-byte block against the WGSL struct's offsets, field by field, with the values
+    writeTileSplatFrameUniforms (src/gpuTileSplatCompositor.ts:730-774) stores there: viewport and
+    tile size as f32, the grid / counts / degree as u32, focal = (W * P[0] * 0.5, H * P[5] * 0.5)
+    rounded once to f32, normalMatrix columns at a 16-byte stride, every pad word zero.  The JS
+    writer equals this one word for word (test_node.py::test_facade_cpu)."""
+    words = [w for _, _, _, n in FRAME_UNIFORMS_LAYOUT for w in range(n)]
+    assert len(words) == 64
+    off = 0
+    for name, byte, _, n in FRAME_UNIFORMS_LAYOUT:  # contiguous, no overlap, 256 bytes
+        assert byte == off, name
+        off += 4 * n
+    assert off == 256
+    plan = compositor.planTileSplatCompositor({"viewportWidth": 1000, "viewportHeight": 563, "tileSizePx": 16,
+                                               "splatCount": 123457})
+    vp = np.float32(np.arange(16) * 0.25 - 1.5)
+    view = np.float32(np.arange(16) * -0.125 + 3.0)
+    proj = np.zeros(16, np.float32)
+    proj[0], proj[5] = np.float32(1.7320508), np.float32(2.4142137)
+    nm = [0.5, -0.25, 0.125, 2.0, 3.0, -4.0, 7.0, 8.0, 9.5]
+    t = camera.write_frame_uniforms(vp, plan, 1.25, 2, [1.5, -2.5, 3.5], view, proj, nm)
+    assert t.dtype == np.float32 and t.nbytes == 256
+    want = {
+        "viewProj": vp, "viewport": [1000.0, 563.0], "tileSizePx": [16.0], "debugMode": [0.0],
+        "tileGrid": [63, 36], "splatCount": [123457], "totalTileRefs": [min(16 * 123457, 65535 * 256)],
+        "splatScale": [1.25], "shDegree": [2], "_pad0": [0, 0], "cameraPos": [1.5, -2.5, 3.5], "_pad1": [0.0],
+        "viewMatrix": view, "focal": [1000 * float(proj[0]) * 0.5, 563 * float(proj[5]) * 0.5], "_pad2": [0.0, 0.0],
+        "normalMatrix.c0": nm[0:3], "_padc0": [0.0], "normalMatrix.c1": nm[3:6], "_padc1": [0.0],
+        "normalMatrix.c2": nm[6:9], "_padc2": [0.0],
+    }
+    u = t.view(np.uint32)
+    for name, byte, kind, n in FRAME_UNIFORMS_LAYOUT:
+        w = byte // 4
+        exp = np.asarray(want[name], np.uint32 if kind == "u32" else np.float32)
+        got = u[w:w + n] if kind == "u32" else t[w:w + n]
+        np.testing.assert_array_equal(got, exp, err_msg=name)
+    assert plan["tileColumns"] == 63 and plan["tileRows"] == 36
+
+
+class _FakeResources:
+    """Stands in for a context: the facade's length checks run before any call into the library."""
+    plan = {"splatCount": 10}
+    handle = None
+
+
+@pytest.mark.parametrize("key,per", [("positionBuffer", 3), ("scaleBuffer", 3), ("rotationBuffer", 4),
+                                     ("opacityBuffer", 1), ("materialBuffer", 1), ("sortedIndexBuffer", 1)])
+def test_bind_rejects_short_buffers(key, per):
+    n = 10
+    bufs = {"positionBuffer": np.zeros(3 * n, np.float32), "scaleBuffer": np.zeros(3 * n, np.float32),
+            "rotationBuffer": np.zeros(4 * n, np.float32), "opacityBuffer": np.zeros(n, np.float32),
+            "materialBuffer": np.zeros(n, np.uint32), "shDataBuffer": np.zeros(6 * n, np.float32),
+            "sortedIndexBuffer": np.arange(n, dtype=np.uint32)}
+    bufs[key] = bufs[key][:-1]
+    with pytest.raises(ValueError, match=key):
+        compositor.createTileSplatBindGroups(None, _FakeResources(), bufs)

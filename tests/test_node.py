@@ -50,38 +50,52 @@ def test_facade_cpu():
     assert out["version"].startswith("hdsr")
 
 
+def write_scene(d, s, srt, cam, **meta):
+    arrays = {"positions.bin": s.positions, "scales.bin": s.scales, "rotations.bin": s.rotations,
+              "opacities.bin": s.opacities, "materials.bin": s.materials, "shdata.bin": s.sh_data(),
+              "sorted.bin": srt, "viewproj.bin": cam.view_proj, "campos.bin": cam.camera_pos,
+              "view.bin": cam.view, "proj.bin": cam.proj}
+    for k, v in arrays.items():
+        np.ascontiguousarray(v).tofile(os.path.join(d, k))
+    json.dump({"width": cam.width, "height": cam.height, "count": s.count, "shDegree": s.sh_degree, **meta},
+              open(os.path.join(d, "meta.json"), "w"))
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("gbuffer", [False, True])
-def test_facade_renders_like_oracle(hip_device, gbuffer):
-    """One frame through the JS facade in the reference's call order; with the G-buffer on, the
-    screen-space normal and bilateral passes run after it through the same facade."""
-    s = scenes.scene_config0(n=12_000, seed=21)
+@pytest.mark.parametrize("gbuffer,sh3", [(False, False), (True, False), (False, True)])
+def test_facade_renders_like_oracle(hip_device, gbuffer, sh3):
+    """One frame through the JS facade in the reference's call order and bind shape (no SH degree
+    in the bind: it is taken from the shData length; output textures filled at queue.submit);
+    with the G-buffer on, the screen-space normal and bilateral passes run after it through the
+    same facade.  The SH3 case is the one a degree-less bind must not get wrong."""
+    from scenes_for_tests import dense_scene
+    s = dense_scene(n=12_000, seed=21, sh_degree=3, core_fraction=0.3) if sh3 else scenes.scene_config0(n=12_000,
+                                                                                                        seed=21)
     cam = camera.Camera(camera.orbit_eye(3.0, -0.6, 0.25), (0, 0, 0), 600.0, 600.0, 480, 320)
     plan = compositor.planTileSplatCompositor({"viewportWidth": 480, "viewportHeight": 320, "tileSizePx": 16,
                                                "splatCount": s.count})
     srt = scenes.sort_back_to_front(s.positions, cam.view)
     u = camera.frame_uniforms_for(cam, plan, s.sh_degree)
     with tempfile.TemporaryDirectory() as d:
-        arrays = {"positions.bin": s.positions, "scales.bin": s.scales, "rotations.bin": s.rotations,
-                  "opacities.bin": s.opacities, "materials.bin": s.materials, "shdata.bin": s.sh_data(),
-                  "sorted.bin": srt, "viewproj.bin": cam.view_proj, "campos.bin": cam.camera_pos,
-                  "view.bin": cam.view, "proj.bin": cam.proj}
-        for k, v in arrays.items():
-            np.ascontiguousarray(v).tofile(os.path.join(d, k))
-        json.dump({"width": 480, "height": 320, "count": s.count, "shDegree": s.sh_degree, "gbuffer": gbuffer,
-                   "recoveryMode": 1}, open(os.path.join(d, "meta.json"), "w"))
+        write_scene(d, s, srt, cam, gbuffer=gbuffer, recoveryMode=1)
         st = run_node("render_scene.js", d)
         uj = np.fromfile(os.path.join(d, "uniforms_js.bin"), np.uint32)
         np.testing.assert_array_equal(uj, u.view(np.uint32))
-        orc = oracle.run(s, srt, u, 480, 320)
+        orc = oracle.run(s, srt, u, 480, 320, oracle.FLAG_GBUFFER if gbuffer else 0)
         assert st["refs"] == orc.stats["refs"]
         np.testing.assert_array_equal(np.fromfile(os.path.join(d, "entries_js.bin"), np.uint32), orc["entries"])
         np.testing.assert_array_equal(np.fromfile(os.path.join(d, "keys_js.bin"), np.uint32), orc["keys"])
         rgba = np.fromfile(os.path.join(d, "rgba_js.bin"), np.float32)
         assert np.abs(rgba - orc["rgba"]).max() <= 1e-5
+        # the rgba16float texture holds the fp32 colour rounded to binary16 (numpy's RNE)
+        np.testing.assert_array_equal(np.fromfile(os.path.join(d, "rgba16_tex.bin"), np.uint16),
+                                      rgba.astype(np.float16).view(np.uint16))
         depth = np.fromfile(os.path.join(d, "depth_js.bin"), np.float32)
         assert np.abs(depth - orc["out_depth"]).max() <= 1e-5
+        np.testing.assert_array_equal(np.fromfile(os.path.join(d, "depth_tex.bin"), np.float32), depth)
         if gbuffer:
+            np.testing.assert_array_equal(np.fromfile(os.path.join(d, "gbn_tex.bin"), np.uint32), orc["gb_normal"])
+            np.testing.assert_array_equal(np.fromfile(os.path.join(d, "gbm_tex.bin"), np.uint32), orc["gb_material"])
             gbn = np.fromfile(os.path.join(d, "gbn_js.bin"), np.uint32).reshape(320, 480)
             inv, ok = oracle.mat4_inverse(u[:16])
             assert ok
@@ -90,6 +104,30 @@ def test_facade_renders_like_oracle(hip_device, gbuffer):
             np.testing.assert_array_equal(ssn, want)
             bil = np.fromfile(os.path.join(d, "bil_js.bin"), np.uint32).reshape(320, 480)
             np.testing.assert_array_equal(bil, oracle.bilateral_normals(want, depth.reshape(320, 480)))
+
+
+@pytest.mark.gpu
+def test_boundary_errors_and_ownership(hip_device):
+    """Short or degree-less buffers, a short sortedIndices update and a uniform shDegree that
+    differs from the bound shData are rejected; calls during a renderAsync() frame throw and
+    destroy() waits for it; device-address output textures receive the frame."""
+    from scenes_for_tests import dense_scene
+    s = dense_scene(n=5_000, seed=23, sh_degree=3, core_fraction=0.3)
+    cam = camera.Camera(camera.orbit_eye(3.0, 0.5, 0.2), (0, 0, 0), 500.0, 500.0, 256, 176)
+    srt = scenes.sort_back_to_front(s.positions, cam.view)
+    with tempfile.TemporaryDirectory() as d:
+        write_scene(d, s, srt, cam)
+        out = run_node("boundary_checks.js", d)
+    for k, e in out["short"].items():
+        assert e is not None and e["name"] == "RangeError" and "Buffer holds" in e["message"], (k, e)
+    assert out["noDegree"] is not None and "matches no SH degree" in out["noDegree"]["message"]
+    assert out["shortSorted"]["name"] == "RangeError"
+    assert "differs from the bound shData" in out["wrongDegree"]["message"]
+    for k in ("busyRender", "busyRead", "busySecondAsync"):
+        assert out[k] is not None and "busy" in out[k]["message"], k
+    assert out["destroyDuringFlight"] is None and out["asyncResolved"]
+    assert "already destroyed" in out["afterDestroy"]["message"]
+    assert out["deviceOutputEqual"] and out["deviceOutputNonzero"]
 
 
 @pytest.mark.gpu
@@ -103,14 +141,7 @@ def test_render_async_matches_sync_and_oracle(hip_device):
     srt = scenes.sort_back_to_front(s.positions, cam.view)
     u = camera.frame_uniforms_for(cam, plan, s.sh_degree)
     with tempfile.TemporaryDirectory() as d:
-        arrays = {"positions.bin": s.positions, "scales.bin": s.scales, "rotations.bin": s.rotations,
-                  "opacities.bin": s.opacities, "materials.bin": s.materials, "shdata.bin": s.sh_data(),
-                  "sorted.bin": srt, "viewproj.bin": cam.view_proj, "campos.bin": cam.camera_pos,
-                  "view.bin": cam.view, "proj.bin": cam.proj}
-        for k, v in arrays.items():
-            np.ascontiguousarray(v).tofile(os.path.join(d, k))
-        json.dump({"width": 400, "height": 272, "count": s.count, "shDegree": s.sh_degree},
-                  open(os.path.join(d, "meta.json"), "w"))
+        write_scene(d, s, srt, cam)
         out = run_node("render_async.js", d)
         assert out["equal"]
         rgba = np.fromfile(os.path.join(d, "rgba_async.bin"), np.float32)
