@@ -1,6 +1,7 @@
 // capi.hip - the C ABI of include/hdsr.h over one Engine (a batch of one frame).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "hdsr.h"
@@ -311,6 +312,60 @@ int hdsr_sort_back_to_front(hdsr_ctx* c, void* stream) {
   }
   launch_depth_sort(c->pos, n, c->frame.view, c->sorted, c->ds_work, static_cast<hipStream_t>(stream));
   HIPX(c, hipGetLastError());
+  return HDSR_OK;
+}
+
+// ---- allocation-free depth order -----------------------------------------------------------
+// Scratch of hdsr_sort_back_to_front_noalloc: the padded (key, index) pairs of the bitonic network,
+// HDSR_NOALLOC_MAX_SPLATS of them (32 MiB), in a file-scope device array that every GPU the
+// library runs on holds until the process ends.  Its device address is looked up once per device
+// (hipGetSymbolAddress with that device current) and only that address is handed to kernels.
+// k_depth_keys writes every pair of the padded range before the network reads it, so nothing in
+// it has to start at zero.  All calls on one GPU share it: calls that overlap on one GPU race.
+static __device__ __align__(256) uint2 scratch[HDSR_NOALLOC_MAX_SPLATS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on 'device' (current on this thread), looked up on first use.
+hipError_t scratch_on(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+}  // namespace
+
+int hdsr_sort_back_to_front_noalloc(const float* positions, uint32_t count, const void* uniforms256,
+                                    uint32_t* sorted_out, void* stream) {
+  if (!uniforms256 || (count && (!positions || !sorted_out))) {
+    g_create_err = "hdsr_sort_back_to_front_noalloc: null argument";
+    return HDSR_E_INVALID_ARG;
+  }
+  if (count > HDSR_NOALLOC_MAX_SPLATS) {
+    g_create_err = "hdsr_sort_back_to_front_noalloc: " + std::to_string(count) + " splats exceed the scratch of " +
+                   std::to_string(HDSR_NOALLOC_MAX_SPLATS);
+    return HDSR_E_CAPACITY;
+  }
+  if (!count) return HDSR_OK;
+  int device = -1;
+  hipError_t e = hipGetDevice(&device);
+  if (e == hipSuccess && (device < 0 || device >= kMaxDevices)) e = hipErrorInvalidDevice;
+  void* work = nullptr;
+  if (e == hipSuccess) e = scratch_on(device, &work);
+  if (e != hipSuccess) return create_fail(HDSR_E_HIP, "hdsr_sort_back_to_front_noalloc: scratch lookup", e);
+  FrameDesc f{};
+  parse_uniforms(uniforms256, &f);
+  launch_depth_sort(positions, count, f.view, sorted_out, static_cast<uint2*>(work), static_cast<hipStream_t>(stream));
+  e = hipGetLastError();
+  if (e != hipSuccess) return create_fail(HDSR_E_HIP, "hdsr_sort_back_to_front_noalloc: launch", e);
   return HDSR_OK;
 }
 

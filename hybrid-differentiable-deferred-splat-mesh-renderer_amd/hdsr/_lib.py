@@ -31,6 +31,7 @@ STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "s
 HDSR_MEM_HOST = 0
 HDSR_MEM_DEVICE = 1
 HDSR_SH_DEGREE_AUTO = 0xFFFFFFFF  # take the SH degree from the shData length
+HDSR_NOALLOC_MAX_SPLATS = 4194304  # scratch capacity of hdsr_sort_back_to_front_noalloc
 
 OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX, OUT_FILTERED_NORMAL = range(8)
 
@@ -131,6 +132,7 @@ SIGNATURES = [
     ("hdsr_set_frame", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_render", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_sort_back_to_front", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hdsr_sort_back_to_front_noalloc", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("hdsr_composite_only", C.c_int, [C.c_void_p, C.c_void_p]),
     ("hdsr_sync", C.c_int, [C.c_void_p]),
     ("hdsr_get_outputs", C.c_int, [C.c_void_p, C.POINTER(Outputs)]),

@@ -129,6 +129,19 @@ int hdsr_set_frame(hdsr_ctx* ctx, const void* uniforms256);
  * following hdsr_render on the same stream uses it.  Needs hdsr_bind_splats and hdsr_set_frame. */
 int hdsr_sort_back_to_front(hdsr_ctx* ctx, void* stream);
 
+/* Allocation-free variant of hdsr_sort_back_to_front.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * hosts that forbid allocation).  Same network and result as hdsr_sort_back_to_front, for
+ * 'count' splats whose positions (3*count f32) are in device memory, with the view matrix of the
+ * 256-byte frame uniforms; writes count sort ranks' splat ids to the device buffer sorted_out.
+ * Needs no context and calls no allocator: its scratch is a fixed device array of
+ * HDSR_NOALLOC_MAX_SPLATS pairs per GPU, held until the process ends.  count above that returns
+ * HDSR_E_CAPACITY.  Calls that overlap on one GPU (other streams or threads) share that array
+ * and race.  Enqueued on 'stream'; errors are reported by hdsr_last_error(NULL). */
+#define HDSR_NOALLOC_MAX_SPLATS 4194304u
+int hdsr_sort_back_to_front_noalloc(const float* positions, uint32_t count, const void* uniforms256,
+                                    uint32_t* sorted_out, void* stream);
+
 /* encodeFullComputeCompositorPipeline (gpuTileSplatCompositor.ts:1023-1179): enqueues the whole
  * path on 'stream' and returns.  Outputs are complete once the stream reaches this point, unless
  * the pair capacity overflowed; hdsr_sync() detects that, grows the capacity and re-renders. */
@@ -274,8 +287,8 @@ typedef struct hdsr_grads {
 /* Waits for the last render (as hdsr_sync), then enqueues the backward on 'stream'. */
 int hdsr_backward(hdsr_ctx* ctx, const float* grad_rgba, const hdsr_grads* grads, void* stream);
 
-/* The message of the last failure on 'ctx'; with ctx NULL, of the last hdsr_create that failed on
-   the calling thread. */
+/* The message of the last failure on 'ctx'; with ctx NULL, of the last hdsr_create (or
+   hdsr_sort_back_to_front_noalloc) that failed on the calling thread. */
 const char* hdsr_last_error(hdsr_ctx* ctx);
 const char* hdsr_version(void);
 
