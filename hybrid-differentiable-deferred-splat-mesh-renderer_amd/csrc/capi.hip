@@ -602,6 +602,11 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->composite_max_refs = s.composite_max_refs;
   o->large_tile_refs = s.scratch_used;
   o->cell_pairs = s.cell_pairs;
+  // a deferred frame sorts the small tiles, the first 4096-entry chunk of every large tile and the
+  // other chunks of the tiles resumed; a one-pass frame sorts every entry
+  o->sorted_refs = c->eng.last_frame_deferred()
+                       ? (s.refs - s.scratch_used) + HDSR_MAX_TILE_ENTRIES * s.large_tiles + s.resumed_refs
+                       : s.refs;
   return HDSR_OK;
 }
 
@@ -624,7 +629,12 @@ int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {
   const DevStats& s = c->eng.host_stats();
   const uint64_t M = c->plan.morton_tile_count;
   switch (which) {
-    case HDSR_DBG_PROJCACHE: *bytes = (uint64_t)c->n * 56; break;
+    case HDSR_DBG_PROJCACHE:
+      // the G-buffer-only words of the record are written by renders that need them
+      if (!(c->flags & (HDSR_DEBUG | HDSR_GBUFFER)))
+        return fail(c, HDSR_E_STATE, "create the context with HDSR_DEBUG (or HDSR_GBUFFER) to read the projection cache");
+      *bytes = (uint64_t)c->n * 56;
+      break;
     case HDSR_DBG_DEPTH: *bytes = (uint64_t)c->n * 4; break;
     case HDSR_DBG_TILE_COUNTS:
     case HDSR_DBG_TILE_OFFSETS: *bytes = M * 4; break;

@@ -52,6 +52,9 @@ class Engine {
 
   // Host copy of the device counters of the last run (valid after wait()).
   const DevStats& host_stats() const { return *hstats_; }
+  // whether the last enqueued frame ran the deferred chunk sort (first chunks, then the resumed
+  // tiles' other chunks) rather than sorting every chunk
+  bool last_frame_deferred() const { return last_deferred_; }
   int wait();
   bool overflowed() const { return hstats_ && hstats_->overflow != 0; }
   // Grows the pair capacity to hold the last run's references (with headroom).
@@ -101,6 +104,7 @@ class Engine {
   bool diff_ = false;
   bool f16_ = false;
   bool deferred_ = false;     // the last enqueued frame left chunks unsorted (deferred chunk sort)
+  bool last_deferred_ = false;  // as deferred_, but not cleared when the lists are completed
   bool bucket_wide_ = false;  // the last settled frame's large tiles averaged more than 16k entries
   size_t n_diff_ = 0;
   // allocation sizes (elements)

@@ -337,7 +337,11 @@ int Engine::upload_frames(const FrameDesc* frames, const Layout& L, hipStream_t 
 void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, bool debug, hipEvent_t* ev) {
   launch_clear_stats(b, stream);
   if (ev) (void)hipEventRecord(ev[0], stream);
-  launch_project(b, stream);
+  {
+    Bufs bp = b;
+    bp.write_cold = (gbuffer || debug) ? 1u : 0u;  // the G-buffer-only words
+    launch_project(bp, stream);
+  }
   if (ev) (void)hipEventRecord(ev[1], stream);
   launch_scan_blocks(b, stream);
   if (ev) (void)hipEventRecord(ev[2], stream);
@@ -356,6 +360,7 @@ void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, boo
   // Deferred chunk sort when nothing reads the whole lists: no debug readback, no G-buffer (its
   // votes are not part of the saved pixel state), no backward pass.
   deferred_ = !debug && !gbuffer && !b.drec;
+  last_deferred_ = deferred_;
   Bufs r1 = b, r2 = b;
   r1.round = deferred_ ? 1u : 0u;
   r2.round = 2u;

@@ -130,7 +130,7 @@ struct DevStats {
   uint32_t composite_max_refs;  // largest per-tile count of those
   uint32_t cell_pairs;          // (cell, rank) pairs
   uint32_t cell_chunks;         // expansion workgroups needed (sum over cells of ceil(count / kCellChunk))
-  uint32_t pad;
+  uint32_t resumed_refs;        // deferred frames: entries after the first chunk of the resumed tiles
 };
 
 struct Seg {
@@ -215,6 +215,9 @@ struct Bufs {
   uint32_t round;
   uint32_t* tile_pend;       // [total_morton] large tiles only: 1 = resumed in round 2
   float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)
+  // The projection writes cold[] (read only by the G-buffer rasterizer and the 14-word debug
+  // readback) only when set.
+  uint32_t write_cold;
 };
 
 // Packed splat record, in floats: [0..3] rotation wxyz, [4..6] position, [7..9] log-scale,

@@ -511,59 +511,126 @@ __device__ __forceinline__ unsigned long long cover_mask(uint32_t bd, uint32_t c
   return m;
 }
 
-// Counting walk; also folds each entry's depth bits into its covered tiles' min / max (LDS, then
-// one global atomic pair per tile and chunk): the depth range that sets every tile's log frame.
+// The rectangle [lx0, lx1] x [ly0, ly1] of cell (cx, cy)'s tiles (0..7 each) inside tile box bd;
+// false when they do not meet.
+__device__ __forceinline__ bool cell_rect(uint32_t bd, uint32_t cx, uint32_t cy, uint32_t* lx0, uint32_t* lx1,
+                                          uint32_t* ly0, uint32_t* ly1) {
+  if (bd == HDSR_SENTINEL) return false;
+  uint32_t x0, y0, x1, y1;
+  box_of(bd, &x0, &y0, &x1, &y1);
+  const int bx = (int)(cx * kCellTiles), by = (int)(cy * kCellTiles);
+  const int ax0 = max((int)x0 - bx, 0), ax1 = min((int)x1 - bx, 7);
+  const int ay0 = max((int)y0 - by, 0), ay1 = min((int)y1 - by, 7);
+  if (ax0 > ax1 || ay0 > ay1) return false;
+  *lx0 = (uint32_t)ax0;
+  *lx1 = (uint32_t)ax1;
+  *ly0 = (uint32_t)ay0;
+  *ly1 = (uint32_t)ay1;
+  return true;
+}
+
+// Dyadic intervals over the 8 tiles of a cell row or column: node (L, i) covers [i 2^L, (i+1) 2^L),
+// L = 0..3, stored at kDyOff[L] + i (15 nodes).  [a, b] splits into at most four disjoint nodes,
+// returned as 4-bit node numbers packed low first; *n gets their count.
+constexpr uint32_t kDyNodes = 15;
+__device__ __forceinline__ uint32_t dyadic_pieces(uint32_t a, uint32_t b, uint32_t* n) {
+  uint32_t packed = 0, k = 0, lo = a;
+  const uint32_t hi = b + 1u;
+  while (lo < hi) {
+    uint32_t L = lo ? (uint32_t)__builtin_ctz(lo) : 3u;
+    L = min(L, 3u);
+    while ((1u << L) > hi - lo) --L;
+    const uint32_t off = L == 0 ? 0u : L == 1 ? 8u : L == 2 ? 12u : 14u;
+    packed |= (off + (lo >> L)) << (4u * k);
+    ++k;
+    lo += 1u << L;
+  }
+  *n = k;
+  return packed;
+}
+
+// Counting walk: for the chunk's entries, the number covering each of the cell's 64 tiles, and each
+// tile's min / max depth bits over them (the depth range that sets the tile's log frame).  Counts:
+// every entry adds its tile rectangle to a 9 x 9 difference array (four LDS atomics), whose 2D
+// prefix sums are the counts.  Depth range: every rectangle is split into dyadic x-intervals times
+// dyadic y-intervals (at most 4 x 4 disjoint pieces; one for a rectangle spanning the cell) and the
+// depth folded into each piece's min / max slot; a tile's min / max is then taken over the 16
+// dyadic rectangles that contain it, exactly the pieces of the entries covering it.  Then one
+// global atomic pair per tile and chunk.
 __global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) {
-  __shared__ uint32_t wcnt[4][64];
-  __shared__ uint32_t s_dmin[64], s_dmax[64];
+  __shared__ int32_t s_diff[9 * 9];
+  __shared__ uint32_t s_dmin[kDyNodes * kDyNodes], s_dmax[kDyNodes * kDyNodes];
   const uint32_t q = blockIdx.x;
   const uint32_t nchunks = cell_chunk_total(b);
   if (q >= nchunks) return;  // uniform
   if (q == 0 && threadIdx.x == 0) b.stats->cell_chunks = nchunks;
   const ChunkInfo ci = chunk_info(b, q);
-  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
-  if (t < 64) {
+  const uint32_t t = threadIdx.x;
+  if (t < 81) s_diff[t] = 0;
+  if (t < kDyNodes * kDyNodes) {
     s_dmin[t] = 0xFFFFFFFFu;
     s_dmax[t] = 0u;
   }
   __syncthreads();
-  uint32_t my = 0;
 #pragma unroll 1
-  for (uint32_t r = 0; r < 4; ++r) {
-    const uint32_t i = w * 256 + r * 64 + lane;
-    unsigned long long mk = 0ull;
-    uint32_t d = 0;
-    if (i < ci.count) {
-      const uint32_t g = cv[ci.first + i];
-      mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
-      d = b.depth[g];
-    }
-#pragma unroll
-    for (uint32_t tt = 0; tt < 64; ++tt) {
-      const uint32_t pc = (uint32_t)__popcll(__ballot((mk >> tt) & 1ull));
-      my += lane == tt ? pc : 0u;
-    }
-    // each lane walks its tiles from a lane-dependent rotation, so that lanes with similar boxes
-    // hit different LDS words in the same step
-    const uint32_t rot = (lane * 11u) & 63u;
-    const unsigned long long rm = rot ? (mk >> rot) | (mk << (64u - rot)) : mk;
-    for (unsigned long long mm = rm; mm; mm &= mm - 1ull) {
-      const uint32_t tt = ((uint32_t)__ffsll((long long)mm) - 1u + rot) & 63u;
-      atomicMin(&s_dmin[tt], d);
-      atomicMax(&s_dmax[tt], d);
+  for (uint32_t i = t; i < ci.count; i += 256) {
+    const uint32_t g = cv[ci.first + i];
+    uint32_t lx0, lx1, ly0, ly1;
+    if (!cell_rect(b.bounds[g], ci.cx, ci.cy, &lx0, &lx1, &ly0, &ly1)) continue;
+    const uint32_t d = b.depth[g];
+    atomicAdd(&s_diff[ly0 * 9 + lx0], 1);
+    atomicAdd(&s_diff[ly0 * 9 + lx1 + 1], -1);
+    atomicAdd(&s_diff[(ly1 + 1) * 9 + lx0], -1);
+    atomicAdd(&s_diff[(ly1 + 1) * 9 + lx1 + 1], 1);
+    uint32_t nxp, nyp;
+    const uint32_t xp = dyadic_pieces(lx0, lx1, &nxp), yp = dyadic_pieces(ly0, ly1, &nyp);
+    for (uint32_t c = 0; c < nyp; ++c) {
+      const uint32_t row = ((yp >> (4u * c)) & 15u) * kDyNodes;
+      for (uint32_t a = 0; a < nxp; ++a) {
+        const uint32_t slot = row + ((xp >> (4u * a)) & 15u);
+        atomicMin(&s_dmin[slot], d);
+        atomicMax(&s_dmax[slot], d);
+      }
     }
   }
-  wcnt[w][lane] = my;
+  __syncthreads();
+  if (t < 9) {  // prefix along x of row t
+    int32_t run = 0;
+    for (uint32_t x = 0; x < 9; ++x) {
+      run += s_diff[t * 9 + x];
+      s_diff[t * 9 + x] = run;
+    }
+  }
+  __syncthreads();
+  if (t < 9) {  // then along y of column t
+    int32_t run = 0;
+    for (uint32_t y = 0; y < 9; ++y) {
+      run += s_diff[y * 9 + t];
+      s_diff[y * 9 + t] = run;
+    }
+  }
   __syncthreads();
   if (t < 64) {
-    const uint32_t c = (wcnt[0][t] + wcnt[1][t]) + (wcnt[2][t] + wcnt[3][t]);
+    const uint32_t x = t & 7u, y = t >> 3;
+    const uint32_t c = (uint32_t)s_diff[y * 9 + x];
     b.chunk_cnt[(size_t)q * 64 + t] = c;
     const FrameDesc& F = *ci.F;
-    const uint32_t tx = ci.cx * kCellTiles + (t & 7u), ty = ci.cy * kCellTiles + (t >> 3);
+    const uint32_t tx = ci.cx * kCellTiles + x, ty = ci.cy * kCellTiles + y;
     if (c && tx < F.tiles_x && ty < F.tiles_y) {
+      uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+#pragma unroll
+      for (uint32_t Ly = 0; Ly < 4; ++Ly) {
+        const uint32_t row = ((Ly == 0 ? 0u : Ly == 1 ? 8u : Ly == 2 ? 12u : 14u) + (y >> Ly)) * kDyNodes;
+#pragma unroll
+        for (uint32_t Lx = 0; Lx < 4; ++Lx) {
+          const uint32_t slot = row + (Lx == 0 ? 0u : Lx == 1 ? 8u : Lx == 2 ? 12u : 14u) + (x >> Lx);
+          mn = min(mn, s_dmin[slot]);
+          mx = max(mx, s_dmax[slot]);
+        }
+      }
       const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
-      atomicMin(&b.tile_dmin[m], s_dmin[t]);
-      atomicMax(&b.tile_dmax[m], s_dmax[t]);
+      atomicMin(&b.tile_dmin[m], mn);
+      atomicMax(&b.tile_dmax[m], mx);
     }
   }
 }

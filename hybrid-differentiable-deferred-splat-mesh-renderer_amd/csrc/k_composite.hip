@@ -122,7 +122,9 @@ __device__ __forceinline__ bool blend(Px& p, Gb& q, float power, float alpha, fl
                                       float dep, const float4& ga, const float4& gb, float geb) {
   const float newT = p.T * (1.0f - alpha);
   const bool valid = (power > -4.0f) && (alpha > (1.0f / 255.0f)) && (p.T >= 0.001f);
-  const float w = (alpha * p.T) * (valid ? 1.0f : 0.0f);
+  // (alpha * T) * (valid ? 1 : 0) as a select: alpha * T is finite and >= +0 (alpha = min(0.99, .)
+  // with a non-negative opacity), so both give the same bits
+  const float w = valid ? alpha * p.T : 0.0f;
   p.r = p.r + cr * w;
   p.g = p.g + cg * w;
   p.b = p.b + cb * w;
@@ -233,9 +235,10 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
 // kH: the f16 variant (binary16 transmittance and colour).
 //
 // Wave w owns the 8x8 quadrant (w & 1, w >> 1) of the tile.  While a batch is staged, every entry
-// tests its conservative ellipse box (reach_box) against the four quadrants and is appended, in
-// list order, to the index list of each quadrant it may reach; a wave then walks only its own
-// list, so entries that cannot touch its pixels cost it nothing.
+// tests its ellipse (reach_quadrants: the box, then the smallest power-level over each quadrant,
+// with rounding margins) against the four quadrants and is appended, in list order, to the index
+// list of each quadrant it may reach; a wave then walks only its own list, so entries that cannot
+// touch its pixels cost it nothing.
 template <bool kG, bool kD, bool kH>
 __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   __shared__ float4 s_a[kBatch];  // cx, cy, a, b
@@ -306,17 +309,9 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     const uint4 h0 = h0n, h1 = h1n, h2 = h2n;
     const uint32_t g = g_cur;
     uint32_t qm = 0;  // quadrants the entry may reach
-    if (i < count) {
-      const float4 bx = reach_box(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x),
-                                  tile_x0, tile_x0 + 15.0f, tile_y0, tile_y0 + 15.0f);
-      if (!(bx.x > bx.y)) {
-#pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
-          const float qx0 = tile_x0 + (float)((k & 1u) * 8u), qy0 = tile_y0 + (float)((k >> 1) * 8u);
-          if (!(bx.y < qx0 || bx.x > qx0 + 7.0f || bx.w < qy0 || bx.z > qy0 + 7.0f)) qm |= 1u << k;
-        }
-      }
-    }
+    if (i < count)
+      qm = reach_quadrants(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
+                           tile_y0);
     // compact the kept entries into slots, in order, and list each under its quadrants
     const bool keep = qm != 0;
     const unsigned long long km = __ballot(keep);
@@ -411,7 +406,10 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
     if (tid == 0) b.tile_pend[m] = finished ? 0u : 1u;
     if (!finished) {  // uniform: resumed in round 2
       save_px(pst, p);
-      if (tid == 0 && b.stats) atomicAdd(&b.stats->composite_refs, loaded);
+      if (tid == 0 && b.stats) {
+        atomicAdd(&b.stats->composite_refs, loaded);
+        atomicAdd(&b.stats->resumed_refs, full - HDSR_MAX_TILE_ENTRIES);  // sorted in round 2
+      }
       return;
     }
   }

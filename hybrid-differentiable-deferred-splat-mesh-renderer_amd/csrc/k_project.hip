@@ -249,8 +249,10 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   }
   b.hot[(size_t)g * 2] = h0;
   b.hot[(size_t)g * 2 + 1] = h1;
-  b.cold[(size_t)g * 2] = c0;
-  b.cold[(size_t)g * 2 + 1] = c1;
+  if (b.write_cold) {  // uniform: G-buffer or debug readback only
+    b.cold[(size_t)g * 2] = c0;
+    b.cold[(size_t)g * 2 + 1] = c1;
+  }
   b.bounds[g] = bounds;
   b.depth[g] = depth_bits;
   // ln(max(depth, 1e-6)) once per splat; the per-tile keys and buckets subtract the tile's

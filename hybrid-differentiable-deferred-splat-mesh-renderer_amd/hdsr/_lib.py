@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libhdsr.so")
+# HDSR_LIB_PATH selects another build of the library (for comparing builds); default: in-tree
+LIB_PATH = os.environ.get("HDSR_LIB_PATH") or os.path.join(_PKG, "lib", "libhdsr.so")
 
 HDSR_OK = 0
 HDSR_E_INVALID_ARG = -1
@@ -71,7 +72,8 @@ class Stats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "refs", "small_pairs", "large_pairs", "large_splats", "visible_splats", "small_tiles", "large_tiles",
         "chunks", "segments", "pair_capacity", "overflow_reruns", "cap_refs_tripped", "cap_large_splats_tripped",
-        "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs", "cell_pairs")]
+        "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs", "cell_pairs",
+        "sorted_refs")]
 
 
 class Grads(C.Structure):

@@ -629,6 +629,7 @@ static napi_value js_stats(napi_env env, napi_callback_info info) {
   set_u32(env, o, "compositeMaxRefs", s.composite_max_refs);
   set_u32(env, o, "largeTileRefs", s.large_tile_refs);
   set_u32(env, o, "cellPairs", s.cell_pairs);
+  set_u32(env, o, "sortedRefs", s.sorted_refs);
   return o;
 }
 

@@ -211,7 +211,7 @@ int hdsr_screen_space_normals(hdsr_ctx* ctx, const hdsr_normal_params* params, v
 int hdsr_bilateral_normal_filter(hdsr_ctx* ctx, const hdsr_normal_params* params, void* stream);
 
 /* Intermediates of the last frame, in the reference's layouts (ranks, Morton ids). */
-#define HDSR_DBG_PROJCACHE 0      /* N*14 u32 (gpu_project_splats.wgsl:2-16) */
+#define HDSR_DBG_PROJCACHE 0      /* N*14 u32 (gpu_project_splats.wgsl:2-16); HDSR_DEBUG or HDSR_GBUFFER */
 #define HDSR_DBG_DEPTH 1          /* N u32 depth bits */
 #define HDSR_DBG_TILE_COUNTS 2    /* mortonTileCount u32 */
 #define HDSR_DBG_TILE_OFFSETS 3   /* mortonTileCount u32 */
@@ -246,6 +246,8 @@ typedef struct hdsr_stats {
   uint32_t composite_max_refs;    /* the largest per-tile count of those */
   uint32_t large_tile_refs;       /* references in tiles with > 4096 entries (bucket cascade input) */
   uint32_t cell_pairs;            /* (8x8-tile cell, splat) pairs of the tile-list build */
+  uint32_t sorted_refs;           /* entries the per-tile sort ordered (a frame without debug readback,
+                                     G-buffer or backward sorts only the chunks its pixels reach) */
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 
