@@ -68,9 +68,10 @@ def parse():
 
 def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composite_refs: float,
                       cell_key_bits: int = 8) -> float:
-    """Bytes each stage must move at minimum (per launch, one view).  project, bucket, sort and
-    composite use the per-unit figures of the reference's path; emit, group and ranges are this
-    build's cell-level tile-list construction (P cell pairs), counted the same way."""
+    """Bytes each stage must move (per frame, one view).  project and composite use the per-unit
+    figures of the reference's path; emit, group, ranges, bucket and sort count what this build's
+    kernels must move (its cell-level tile-list construction over P cell pairs, a bucket cascade
+    that reads the keys twice, a sort of only the chunks the rasterizer reaches)."""
     R = st["refs"]
     P = st.get("cell_pairs", 0)
     if stage == "project":
@@ -84,10 +85,18 @@ def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composi
         return 20 * P * passes + 12 * P  # radix passes on the cell key; the counting walk's gathers
     if stage == "ranges":
         return 16 * P + 8 * R  # the pairs' splat, box and log depth in; entry + key per reference out
+    if stage == "scan":
+        return 2 * 32 * (n / 256)  # two uint4 counts per 256-splat block, read and written
+    if stage == "classify":
+        T = st.get("tiles", 0)
+        return 40 * T + 16 * st.get("segments", 0)  # tile ranges, class records in and out, lists, segments
     if stage == "bucket":
-        return 44 * st.get("large_tile_refs", 0)
+        # this design's cascade: the keys read twice (counts, scatter), the entries once, both written
+        return 20 * st.get("large_tile_refs", 0)
     if stage == "sort":
-        return 12 * R
+        # key + entry in, entry out, for the entries actually sorted (the deferred chunk sort orders
+        # only the chunks the rasterizer reaches)
+        return 12 * st.get("sorted_refs", R)
     if stage == "composite":
         return 60 * composite_refs + pixels * 20
     return 0.0
@@ -178,6 +187,7 @@ def main() -> int:
 
     k = (scene.sh_degree + 1) ** 2 - 1
     mean_st = {key: float(np.mean([s[key] for s in view_stats])) for key in view_stats[0]}
+    mean_st["tiles"] = plan["tileCount"]
     pixels = W * H
     # refs the rasterizer loaded: up to the batch in which each tile's last pixel saturated
     comp_refs = mean_st["composite_refs"]
@@ -194,6 +204,9 @@ def main() -> int:
         pmc = PMC_TRAFFIC_BYTES.get(SINGLE_KERNEL_STAGES.get(name, ""))
         if full_size and pmc and ms > 0:
             per_stage[name]["traffic_GBps"] = round(pmc / (ms * 1e-3) / 1e9, 1)
+    over_peak = [nm for nm, v in per_stage.items() if (v["GBps"] or 0) > HBM_PEAK_GBS]
+    if over_peak:
+        print(f"stage bytes over the HBM peak (their per-unit figures overcount): {over_peak}", file=sys.stderr)
     dom = max(SINGLE_KERNEL_STAGES, key=lambda s: stage_ms[s])
     dom_kernel = SINGLE_KERNEL_STAGES[dom]
     dom_bytes = algorithmic_bytes(dom, scene.count, k, mean_st, pixels, comp_refs)
@@ -256,9 +269,12 @@ def main() -> int:
                      "algorithmic_bytes_per_frame": dom_bytes, "ms_per_frame": round(stage_ms[dom], 4),
                      "launches_per_frame": LAUNCHES_PER_FRAME.get(dom_kernel, 1)},
         "stages_ms_per_frame": per_stage,
+        "stages_within_peak": not over_peak,
         "frame_stats": {k2: round(v2, 1) for k2, v2 in mean_st.items()},
         "cpu_baseline": cpu,
-        "vs_cpu": None if not cpu else round(fps / cpu["value"], 1),
+        "vs_cpu": None if not cpu or "value" not in cpu else round(fps / cpu["value"], 1),
+        "vs_cpu_worker_threads": (round(fps / cpu["worker_threads"]["value"], 1)
+                                  if cpu and "worker_threads" in cpu else None),
         "setup_s": round(setup_s, 1),
         "profiled_renders": renders,
         "gpu_depth_sort": None if sort_only is None else {
@@ -327,6 +343,7 @@ def stress_bench(views=16, steps=2, seed=2) -> dict:
     stage_ms, _ = res.stage_times()
     frames = steps * len(cams)
     mean_st = {key: float(np.mean([x[key] for x in view_stats])) for key in view_stats[0]}
+    mean_st["tiles"] = plan["tileCount"]
     k = (scene.sh_degree + 1) ** 2 - 1
     cells = -(-plan["tileColumns"] // 8) * -(-plan["tileRows"] // 8)
     ckb = max(1, (cells - 1).bit_length())
@@ -337,7 +354,6 @@ def stress_bench(views=16, steps=2, seed=2) -> dict:
         per_stage[name] = {"ms": round(ms, 3), "GBps": None if gbs is None else round(gbs, 1)}
     R = mean_st["refs"]
     order_ms = stage_ms["emit"] + stage_ms["group"] + stage_ms["ranges"]
-    depth_ms = stage_ms["bucket"] + stage_ms["sort"]
     res.destroy()
     return {
         "workload": f"6M-splat SH3 stress scene (half in a 0.2-unit ball before camera 0), {len(cams)} orbit views "
@@ -347,8 +363,12 @@ def stress_bench(views=16, steps=2, seed=2) -> dict:
         "refs_per_frame": round(R),
         "large_tile_refs_per_frame": round(mean_st["large_tile_refs"]),
         "chunks_per_frame": round(mean_st["chunks"]),
+        "sorted_refs_per_frame": round(mean_st["sorted_refs"]),
+        # every reference is grouped into its tile's list; the bucket cascade partitions every
+        # large-tile reference by depth; the segment sort fully orders only the entries it sorts
         "tile_key_order_keys_per_s": round(R / (order_ms * 1e-3)),
-        "depth_order_keys_per_s": round(R / (depth_ms * 1e-3)),
+        "bucket_keys_per_s": round(mean_st["large_tile_refs"] / (stage_ms["bucket"] * 1e-3)),
+        "depth_order_keys_per_s": round(mean_st["sorted_refs"] / (stage_ms["sort"] * 1e-3)),
         "scatter_GBps": per_stage["ranges"]["GBps"],
         "peak_GBps": HBM_PEAK_GBS,
         "stages_ms_per_frame": per_stage,
@@ -428,25 +448,23 @@ def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
 def serving_bench(rates=(100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 3000, 5000), duration_s=1.0,
                   max_requests=1500, seed=3) -> dict:
     """Serving workload: 32 resident scenes with sizes log-uniform in [50k, 4M] splats, SH0 or SH3
-    by coin (each a distinct slice of a 4M-splat seeded scene of that degree), 256 requests of
-    random scene / orbit pose / resolution in {640x480, 1280x720, 1920x1080}, submitted as a Poisson
-    stream at each rate.  The batching server (64 views per launch, 2 slots, graph replay, 200 GB
-    budget) is compared with the plain per-request drop-in (one frame per launch, one slot)."""
+    by coin, each generated as the 1M scene is (its own seed: 64 clusters + 6 sheets), 256
+    requests of random scene / orbit pose / resolution in {640x480, 1280x720, 1920x1080},
+    submitted as a Poisson stream at each rate.  The batching server (64 views per launch, 2
+    slots, graph replay, 200 GB budget) is compared with the plain per-request drop-in (one frame
+    per launch, one slot)."""
     import math
+    from concurrent.futures import ThreadPoolExecutor
 
     from hdsr import scenes
     from hdsr.camera import Camera, orbit_eye
     from hdsr.serving import Server, poisson_run
     rng = np.random.default_rng(seed)
-    base = {3: scenes.scene_config1(4_000_000, seed=31, sh_degree=3), 0: scenes.scene_config1(4_000_000, seed=30,
-                                                                                              sh_degree=0)}
     sizes = np.exp(rng.uniform(math.log(50_000), math.log(4_000_000), 32)).astype(np.int64)
     degs = rng.integers(0, 2, 32) * 3
-    scene_list = []
-    for i in range(32):
-        b = base[int(degs[i])]
-        off = int(rng.integers(0, b.count - sizes[i] + 1))
-        scene_list.append(b.subset(np.arange(off, off + sizes[i])))
+    with ThreadPoolExecutor(16) as ex:
+        scene_list = list(ex.map(lambda i: scenes.scene_config1(int(sizes[i]), seed=1000 + i,
+                                                                sh_degree=int(degs[i])), range(32)))
     res = [(640, 480), (1280, 720), (1920, 1080)]
     out = {"scenes": 32, "total_splats": int(sizes.sum()), "rates_hz": list(rates), "duration_s": duration_s}
     for mode, kw in (("batched", dict(max_views_per_batch=64, slots=2, use_graphs=True)),
@@ -540,9 +558,13 @@ def diff_bench(n=500_000, views=16, steps=3, seed=4) -> dict:
 
 def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:
     """The reference's CPU path (cpuDepthSort + CPU tile-ref population + JS compositing),
-    restated in plain JavaScript (oracle/js/cpu_path.js) and timed under Node on this host, one
-    thread, on one whole frame of view 0 (about 15-20 s).  Falls back to compositing a sample of
-    tile rows if a probe frame would exceed the budget."""
+    restated in plain JavaScript (oracle/js/cpu_path.js) and timed under Node on this host on one
+    whole frame of view 0: one untimed warm-up frame, then the median of three frames.  The primary
+    figure is one thread; a second run shares the projection, the per-tile sorts and the
+    compositing out to worker_threads (W = the host's CPUs, at most its OMP_NUM_THREADS share, 16
+    by default), the depth sort and the count / scan / scatter staying on the main thread.  With a
+    budget below 15 s only every fourth tile row is composited and the time is scaled to the
+    frame."""
     import json as _json
     import shutil
     import subprocess
@@ -551,6 +573,8 @@ def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:
     if node is None:
         return None
     js = os.path.join(ROOT, "oracle", "js", "cpu_path.js")
+    workers = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16)))
+    runs = {}
     with tempfile.TemporaryDirectory() as d:
         for k, v in {"positions.bin": scene.positions, "scales.bin": scene.scales, "rotations.bin": scene.rotations,
                      "opacities.bin": scene.opacities, "shdata.bin": scene.sh_data(), "uniforms.bin": u}.items():
@@ -558,19 +582,34 @@ def cpu_baseline(scene, sorted_idx, u, W, H, plan, budget_s: float) -> dict:
         with open(os.path.join(d, "meta.json"), "w") as fh:
             _json.dump({"count": scene.count, "width": W, "height": H, "shDegree": scene.sh_degree}, fh)
         step = 1 if budget_s >= 15 else max(1, plan["tileRows"] // 4)
-        r = subprocess.run([node, "--max-old-space-size=16384", js, d, str(step)], capture_output=True, text=True,
-                           timeout=600)
-        if r.returncode != 0:
-            return {"error": r.stderr[-500:]}
-        t = _json.loads(r.stdout.strip().splitlines()[-1])
+        for w in (1, workers):
+            if w in runs:
+                continue
+            r = subprocess.run([node, "--max-old-space-size=16384", js, d, str(step), "3", str(w)],
+                               capture_output=True, text=True, timeout=900)
+            if r.returncode != 0:
+                return {"error": r.stderr[-500:]}
+            runs[w] = _json.loads(r.stdout.strip().splitlines()[-1])
+    t = runs[1]
     frame_s = t["frame_s_est"]
-    return {"value": round(1.0 / frame_s, 5), "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": (f"view 0 of the same workload: cpuDepthSort + tile-ref population over all 1M splats, "
-                       f"JS compositing of {t['composite_rows']}/{t['tile_rows']} tile rows"
-                       + (" (the whole frame)" if t["composite_rows"] == t["tile_rows"] else ", scaled to the frame")
-                       + ", Node single thread"),
-            "frame_s": round(frame_s, 3),
-            "phases_s": {k: round(v, 3) for k, v in t.items() if k.endswith("_s")}}
+    whole = t["composite_rows"] == t["tile_rows"]
+    out = {"value": round(1.0 / frame_s, 5), "unit": "frames/s", "cores": 1, "kind": "port",
+           "sample": (f"view 0 of the same workload, one warm-up then the median of {t['frames']} frames: "
+                      f"cpuDepthSort + tile-ref population over all 1M splats, JS compositing of "
+                      f"{t['composite_rows']}/{t['tile_rows']} tile rows"
+                      + (" (the whole frame)" if whole else ", scaled to the frame") + ", Node single thread"),
+           "frame_s": round(frame_s, 3), "frames_s": [round(x, 3) for x in t["frames_s"]],
+           "phases_s": {k: round(v, 3) for k, v in t.items() if k.endswith("_s") and k != "frames_s"},
+           "cpu_model": t["cpu_model"], "host_cpus": t["cpus"]}
+    if workers > 1:
+        tw = runs[workers]
+        out["worker_threads"] = {
+            "value": round(1.0 / tw["frame_s_est"], 5), "unit": "frames/s", "workers": workers,
+            "frame_s": round(tw["frame_s_est"], 3), "frames_s": [round(x, 3) for x in tw["frames_s"]],
+            "phases_s": {k: round(v, 3) for k, v in tw.items() if k.endswith("_s") and k != "frames_s"},
+            "note": "projection, per-tile sorts and compositing on worker_threads; depth sort and "
+                    "count/scan/scatter on the main thread"}
+    return out
 
 
 if __name__ == "__main__":
