@@ -1,15 +1,20 @@
-"""Serving form: a request's frame is the same alone and inside a continuous batch.
+"""Serving form: a request's frame is the same alone and inside a continuous batch, and it is the
+oracle's frame.
 
 Scenes of different sizes and SH degrees, requests at mixed resolutions and poses.  Every
-request is rendered alone by the plain C-ABI path (identity sortedIndices, as the server uses) and
-then submitted together to a batching server (several slots, graph replay, eviction under a small
-byte budget); each result must equal its alone image bit for bit.
+request is rendered alone by the plain C-ABI path (identity sortedIndices, as the server uses),
+checked against the oracle (pixels and depth within 1e-5), then submitted together to a batching
+server (several slots, graph replay, eviction under a small byte budget); each result must equal
+its alone image bit for bit, so every served frame is within 1e-5 of the oracle.  A serving-scale
+case mixes a 1M-splat SH3 scene (the serving workload's largest kind) with a small SH0 scene at
+1920x1080, 1280x720 and 640x480 in one batch.
 """
 import numpy as np
 import pytest
 
 from hdsr import camera, compositor, scenes
 from hdsr.serving import Server, poisson_run
+from oracle import oracle
 from scenes_for_tests import dense_scene
 
 pytestmark = pytest.mark.gpu
@@ -48,11 +53,22 @@ def alone(scene, cam):
     return out
 
 
+def check_oracle(scene, cam, rgba, depth):
+    plan = compositor.planTileSplatCompositor({"viewportWidth": cam.width, "viewportHeight": cam.height,
+                                               "tileSizePx": 16, "splatCount": scene.count})
+    u = camera.frame_uniforms_for(cam, plan, scene.sh_degree)
+    orc = oracle.run(scene, np.arange(scene.count, dtype=np.uint32), u, cam.width, cam.height)
+    assert np.abs(rgba.reshape(-1) - orc["rgba"]).max() <= 1e-5
+    assert np.abs(depth.reshape(-1) - orc["out_depth"]).max() <= 1e-5
+
+
 @pytest.fixture(scope="module")
 def workload(hip_device):
     sc = make_scenes()
     reqs = make_requests(sc)
     ref = [alone(sc[sid], cam) for sid, cam in reqs]
+    for (sid, cam), (rgba, depth) in zip(reqs, ref):  # served == alone (below) == oracle
+        check_oracle(sc[sid], cam, rgba, depth)
     return sc, reqs, ref
 
 
@@ -120,4 +136,33 @@ def test_poisson_stream_completes(workload):
     st = poisson_run(srv, pre, rate_hz=400.0, duration_s=0.5)
     assert st["frames"] == st["submitted"] > 50
     assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
+
+
+def test_serving_scale_mixed_resolutions(hip_device):
+    """A 1M-splat SH3 clusters-and-sheets scene and a 50k SH0 scene, requests at 1920x1080,
+    1280x720 and 640x480 batched together: each equals its alone frame bit for bit; one 1080p
+    request of the big scene against the oracle at full size."""
+    big = scenes.scene_config1(1_000_000, seed=301)
+    small = scenes.scene_config0(50_000, seed=302)
+    sc = {7: big, 8: small}
+    reqs = []
+    for i, (w, h) in enumerate([(1920, 1080), (1280, 720), (640, 480), (1280, 720), (1920, 1080), (640, 480)]):
+        f = 1500.0 * h / 1080
+        cam = camera.Camera(camera.orbit_eye(3.0, 0.8 * i, 0.2), (0, 0, 0), f, f, w, h)
+        reqs.append((7 if i % 3 != 2 else 8, cam))
+    ref = [alone(sc[sid], cam) for sid, cam in reqs]
+    check_oracle(big, reqs[0][1], *ref[0])
+    srv = Server(max_views_per_batch=64, slots=2, use_graphs=True)
+    for sid, s in sc.items():
+        srv.add_scene(sid, s)
+    tickets = [srv.submit(sid, cam) for sid, cam in reqs]
+    srv.drain()
+    batched = 0
+    for t, (rgba, depth) in zip(tickets, ref):
+        batched = max(batched, srv.result(t)["batch_views"])
+        np.testing.assert_array_equal(srv.read_rgba(t), rgba)
+        np.testing.assert_array_equal(srv.read_depth(t), depth)
+        srv.release(t)
+    assert batched > 1
     srv.close()
