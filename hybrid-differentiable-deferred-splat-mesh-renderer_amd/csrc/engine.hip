@@ -81,7 +81,7 @@ void Engine::release() {
   if (gstage_) (void)hipHostFree(gstage_);
   gstage_ = nullptr;
   gstage_bytes_ = 0;
-  void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.dlog, b_.block_cnt, b_.block_pre,
+  void* ptrs[] = {b_.hot, b_.cold, b_.drec, b_.gacc, b_.bounds, b_.depth, b_.bdep, b_.bln, b_.block_cnt, b_.block_pre,
                   b_.block_cc, b_.block_ccpre, b_.scan_tmp, b_.large_list, b_.ckeys_a, b_.cvals_a, b_.ckeys_b,
                   b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
@@ -144,7 +144,9 @@ int Engine::ensure(const Layout& L, bool debug) {
     have = n_splat_;
     if ((rc = grow(&b_.large_list, &have, ns))) return rc;
     have = n_splat_;
-    if ((rc = grow(&b_.dlog, &have, ns))) return rc;
+    if ((rc = grow(&b_.bdep, &have, ns))) return rc;
+    have = n_splat_;
+    if ((rc = grow(&b_.bln, &have, ns))) return rc;
     n_splat_ = ns;
   }
   if (diff_ && (ns > n_diff_ || !b_.drec)) {

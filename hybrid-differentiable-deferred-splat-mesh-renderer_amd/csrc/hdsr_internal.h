@@ -167,7 +167,8 @@ struct Bufs {
   float* gacc;               // [splat_blocks*256*9] dL/d(px, py, k0, k1, k2, opacity, r, g, b), or null
   uint32_t* bounds;          // [splat_blocks*256]
   uint32_t* depth;           // [splat_blocks*256]
-  uint2* dlog;               // [splat_blocks*256] (depth bits, ln(max(depth, 1e-6)) bits) for the sorts
+  uint2* bdep;               // [splat_blocks*256] (bounds, depth bits): the counting walk's gather per pair
+  uint2* bln;                // [splat_blocks*256] (bounds, ln(max(depth, 1e-6)) bits): the list write's gather
   uint4* block_cnt;          // [splat_blocks] per block (small pairs, large pairs, large splats, visible)
   uint4* block_pre;          // [splat_blocks] exclusive prefix of block_cnt
   uint4* block_cc;           // [splat_blocks] per block (small-splat cell pairs, large-splat cell pairs, 0, 0)
@@ -233,6 +234,13 @@ void launch_screen_space_normals(const float* depth, const uint32_t* src, uint32
 int launch_bilateral_normals(const uint32_t* in, const float* depth, uint32_t* out, uint32_t w, uint32_t h,
                              float spatial_sigma, float depth_sigma, float normal_sigma, int radius,
                              hipStream_t stream);
+// The deferred lighting pass's Params block (src/splatRenderer.ts:505-523), 44 floats (k_lighting.hip).
+struct LightingBlock {
+  float f[44];
+};
+void launch_deferred_lighting(const LightingBlock& p, const float* rgba, const float* depth, const uint32_t* normal,
+                              const uint32_t* mat, const float* ao, const float* bloom, const float* lut,
+                              uint16_t* out, uint32_t w, uint32_t h, hipStream_t stream);
 void launch_pack_scene(const float* pos, const float* logscale, const float* rot, const float* opacity,
                        const uint32_t* material, const float* shdata, uint32_t n, uint32_t sh_stride, float4* out,
                        hipStream_t stream);

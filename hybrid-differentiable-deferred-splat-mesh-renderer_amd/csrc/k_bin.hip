@@ -574,10 +574,10 @@ __global__ __launch_bounds__(256) void k_cell_count(Bufs b, const uint32_t* cv) 
   __syncthreads();
 #pragma unroll 1
   for (uint32_t i = t; i < ci.count; i += 256) {
-    const uint32_t g = cv[ci.first + i];
+    const uint2 r = b.bdep[cv[ci.first + i]];  // bounds and depth bits in one 8-B record
     uint32_t lx0, lx1, ly0, ly1;
-    if (!cell_rect(b.bounds[g], ci.cx, ci.cy, &lx0, &lx1, &ly0, &ly1)) continue;
-    const uint32_t d = b.depth[g];
+    if (!cell_rect(r.x, ci.cx, ci.cy, &lx0, &lx1, &ly0, &ly1)) continue;
+    const uint32_t d = r.y;
     atomicAdd(&s_diff[ly0 * 9 + lx0], 1);
     atomicAdd(&s_diff[ly0 * 9 + lx1 + 1], -1);
     atomicAdd(&s_diff[(ly1 + 1) * 9 + lx0], -1);
@@ -723,8 +723,9 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
     float ln = 0.0f;
     if (i < ci.count) {
       g = cv[ci.first + i];
-      mk = cover_mask(b.bounds[g], ci.cx, ci.cy);
-      ln = hdsr_u2f(b.dlog[g].y);  // ln(max(depth, 1e-6)), computed once per splat by the projection
+      const uint2 r = b.bln[g];  // bounds and ln(max(depth, 1e-6)), computed once per splat by the projection
+      mk = cover_mask(r.x, ci.cx, ci.cy);
+      ln = hdsr_u2f(r.y);
     }
     s_mask[i] = mk;
     s_g[i] = g;

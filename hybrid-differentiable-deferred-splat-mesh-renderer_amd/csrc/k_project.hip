@@ -53,12 +53,18 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   const uint32_t id = (r < F.n) ? F.sorted[r] : 0xFFFFFFFFu;
   if (id < F.n) {
     const float4* R = kPacked ? F.packed + (size_t)id * F.pk_stride4 : nullptr;
-    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0;
+    // packed: the whole record (at most 16 float4) is loaded up front, so both of its 128-byte
+    // lines are requested together and consumed while resident (loading the SH part where it is
+    // used lets the lines be evicted and fetched a second time); every index below is a constant
+    float4 rv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) rv[j] = make_float4(0, 0, 0, 0);
     if (kPacked) {
-      r0 = R[0];
-      r1 = R[1];
-      r2 = R[2];
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j < 8 || F.pk_stride4 > 8u) rv[j] = R[j];
     }
+    const float4 r0 = rv[0], r1 = rv[1], r2 = rv[2];
     const F3 center = kPacked ? f3(r1.x, r1.y, r1.z) : f3(F.pos[id * 3], F.pos[id * 3 + 1], F.pos[id * 3 + 2]);
     float clip[4];
     mat4_point(F.vp, center, clip);
@@ -187,24 +193,29 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
 
         // spherical harmonics (gpu_project_splats.wgsl:293-351)
         const uint32_t deg = F.sh_degree;
-        const float* sh = kPacked ? static_cast<const float*>(__builtin_assume_aligned(R + 3, 16))
-                                  : F.shdata + (size_t)id * F.sh_stride;
-        float c[3] = {sh[0], sh[1], sh[2]};
+        const float* shp = kPacked ? nullptr : F.shdata + (size_t)id * F.sh_stride;
+        // SH row float k: from the registers (packed) or the caller's array
+        auto S = [&](int k) -> float {
+          if (kPacked) {
+            const float4 v = rv[3 + (k >> 2)];
+            return (k & 3) == 0 ? v.x : (k & 3) == 1 ? v.y : (k & 3) == 2 ? v.z : v.w;
+          }
+          return shp[k];
+        };
+        float c[3] = {S(0), S(1), S(2)};
         if (deg >= 1u) {
           const float x = view_dir.x, y = view_dir.y, z = view_dir.z;
-          const float* s1 = sh + 3;
 #pragma unroll
-          for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + SH_C1 * ((s1[ch] * y + s1[3 + ch] * z) + s1[6 + ch] * x);
+          for (int ch = 0; ch < 3; ++ch) c[ch] = c[ch] + SH_C1 * ((S(3 + ch) * y + S(6 + ch) * z) + S(9 + ch) * x);
           if (deg >= 2u) {
             const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
             const float b2[5] = {SH_C2_0 * xy, SH_C2_1 * yz, SH_C2_2 * ((2.0f * zz - xx) - yy), SH_C2_3 * xz,
                                  SH_C2_4 * (xx - yy)};
-            const float* s2 = s1 + 9;
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
-              float acc = s2[ch] * b2[0];
+              float acc = S(12 + ch) * b2[0];
 #pragma unroll
-              for (int j = 1; j < 5; ++j) acc = acc + s2[3 * j + ch] * b2[j];
+              for (int j = 1; j < 5; ++j) acc = acc + S(12 + 3 * j + ch) * b2[j];
               c[ch] = c[ch] + acc;
             }
             if (deg >= 3u) {
@@ -215,12 +226,11 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
                                    (SH_C3_4 * x) * ((4.0f * zz - xx) - yy),
                                    (SH_C3_5 * z) * (xx - yy),
                                    (SH_C3_6 * x) * (xx - 3.0f * yy)};
-              const float* s3 = s2 + 15;
 #pragma unroll
               for (int ch = 0; ch < 3; ++ch) {
-                float acc = s3[ch] * b3[0];
+                float acc = S(27 + ch) * b3[0];
 #pragma unroll
-                for (int j = 1; j < 7; ++j) acc = acc + s3[3 * j + ch] * b3[j];
+                for (int j = 1; j < 7; ++j) acc = acc + S(27 + 3 * j + ch) * b3[j];
                 c[ch] = c[ch] + acc;
               }
             }
@@ -229,7 +239,13 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
         const float cr = fminf(fmaxf(c[0], 0.0f), 1.0f);
         const float cg = fminf(fmaxf(c[1], 0.0f), 1.0f);
         const float cb = fminf(fmaxf(c[2], 0.0f), 1.0f);
-        const float* em = sh + (F.sh_stride - 3u);
+        // emissive: the last three floats of the SH row (stride 3 (deg + 1)^2 + 3: offset 3, 12, 27 or 48)
+        const uint32_t eo = F.sh_stride - 3u;
+        float em[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          em[i] = kPacked ? (eo == 3u ? S(3 + i) : eo == 12u ? S(12 + i) : eo == 27u ? S(27 + i) : S(48 + i))
+                          : shp[eo + i];
 
         const uint32_t op16 = hdsr_f16_bits(opac);
         h0 = make_uint4(hdsr_f2u(px), hdsr_f2u(py), hdsr_f2u(k0), hdsr_f2u(k1));
@@ -257,7 +273,10 @@ __global__ __launch_bounds__(256) void k_project(Bufs b) {
   b.depth[g] = depth_bits;
   // ln(max(depth, 1e-6)) once per splat; the per-tile keys and buckets subtract the tile's
   // logMin from it (hdsr_log_pos evaluates the same expression)
-  b.dlog[g] = make_uint2(depth_bits, hdsr_f2u(hdsr_logf(fmaxf(hdsr_u2f(depth_bits), 1e-6f))));
+  // 8-byte records the two tile-list walks gather per (cell, splat) pair: one line per pair, 16
+  // splat ranks per 128-byte line
+  b.bdep[g] = make_uint2(bounds, depth_bits);
+  b.bln[g] = make_uint2(bounds, hdsr_f2u(hdsr_logf(fmaxf(hdsr_u2f(depth_bits), 1e-6f))));
 
   // workgroup reduction of the six counts
   __shared__ uint32_t red[4][6];

@@ -25,7 +25,7 @@
 //
 // Every entry's q20 arrives with it (qkey[]): the tile-list expansion evaluates it from the tile's
 // depth range (min / max folded during the counting walk) and ln(max(depth, 1e-6)) (computed once
-// per splat by the projection, dlog[]), so these kernels read one 4-byte key stream and do no log
+// per splat by the projection, bln[].y), so these kernels read one 4-byte key stream and do no log
 // arithmetic.
 #include "hdsr_internal.h"
 
