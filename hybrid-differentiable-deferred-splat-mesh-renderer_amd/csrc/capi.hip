@@ -1,8 +1,10 @@
 // capi.hip - the C ABI of include/hdsr.h over one Engine (a batch of one frame).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "hdsr.h"
 #include "engine.h"
@@ -39,6 +41,9 @@ struct hdsr_ctx {
   float* t_final = nullptr;       // differentiable mode
   uint32_t* last_idx = nullptr;
   uint2* ds_work = nullptr;       // depth-sort scratch (hdsr_sort_back_to_front), on first use
+  uint16_t* lit = nullptr;        // deferred lighting output (hdsr_deferred_lighting), on first use
+  float* lut_dev = nullptr;       // the three material-curve tables on the device (3 x 256)
+  std::vector<float> lut_host;    // their last uploaded values
   hipStream_t last_stream = nullptr;
   bool rendered = false;
   bool settled = false;  // the last render's capacity check has run (hdsr_sync since the render)
@@ -198,7 +203,8 @@ void hdsr_destroy(hdsr_ctx* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   free_scene(c);
-  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx, c->ds_work, c->gbn_filtered};
+  void* p[] = {c->rgba, c->depth, c->gbn, c->gbm, c->rgba16, c->t_final, c->last_idx, c->ds_work, c->gbn_filtered,
+               c->lit, c->lut_dev};
   for (void* q : p)
     if (q) (void)hipFree(q);
   delete c;
@@ -438,6 +444,7 @@ int hdsr_get_outputs(hdsr_ctx* c, hdsr_outputs* o) {
   o->rgba16 = c->rgba16;
   o->width = c->plan.viewport_width;
   o->height = c->plan.viewport_height;
+  o->lit = c->lit;
   return HDSR_OK;
 }
 
@@ -457,6 +464,7 @@ int hdsr_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
     case HDSR_OUT_T_FINAL: src = c->t_final; need = px * 4; break;
     case HDSR_OUT_LAST_INDEX: src = c->last_idx; need = px * 4; break;
     case HDSR_OUT_FILTERED_NORMAL: src = c->gbn_filtered; need = px * 4; break;
+    case HDSR_OUT_LIT: src = c->lit; need = px * 8; break;
     default: return fail(c, HDSR_E_INVALID_ARG, "unknown output");
   }
   if (!src) return fail(c, HDSR_E_STATE, "output not enabled by the create flags");
@@ -561,6 +569,116 @@ int hdsr_bilateral_normal_filter(hdsr_ctx* c, const hdsr_normal_params* params, 
     return fail(c, HDSR_E_INVALID_ARG, "kernel_radius must be at most 8");
   HIPX(c, hipGetLastError());
   c->last_stream = static_cast<hipStream_t>(stream);
+  return HDSR_OK;
+}
+
+void hdsr_lighting_params_default(hdsr_lighting_params* p) {
+  if (!p) return;
+  *p = hdsr_lighting_params{};
+  // the reference viewer's fixed light (src/main.ts:145-149, 273-279): azimuth 0.8, elevation 0.6
+  const double ce = std::cos(0.6);
+  p->light_direction[0] = std::cos(0.8) * ce;
+  p->light_direction[1] = -std::sin(0.6);
+  p->light_direction[2] = std::sin(0.8) * ce;
+  p->light_intensity = 3.0;
+  p->ambient_intensity = 0.12;
+  p->specular_only = 0;
+  p->emissive_intensity = 3.0;
+  p->emissive_threshold = 0.05;
+  p->env_intensity = 1.0;
+  p->env_rotation = 0.0;
+  p->exposure = 1.0;
+  p->roughness_curve = p->metalness_curve = p->albedo_curve = hdsr_material_curve{1.0, 0.0, 1.0};
+}
+
+void hdsr_write_lighting_params(const hdsr_lighting_params* p, const float* inv, const float* cam, uint32_t w,
+                                uint32_t h, float* o) {
+  if (!o) return;
+  hdsr_lighting_params d;
+  hdsr_lighting_params_default(&d);
+  if (!p) p = &d;
+  std::memset(o, 0, 44 * sizeof(float));
+  o[0] = (float)w;
+  o[1] = (float)h;
+  o[2] = 0.65f;  // fallback roughness / metallic (the shader reads the G-buffer's)
+  o[3] = 0.0f;
+  for (int i = 0; i < 16; ++i) o[4 + i] = inv ? inv[i] : 0.0f;
+  for (int i = 0; i < 3; ++i) o[20 + i] = cam ? cam[i] : 0.0f;
+  for (int i = 0; i < 3; ++i) o[24 + i] = (float)p->light_direction[i];
+  o[28] = 1.0f;  // warm white
+  o[29] = (float)0.95;
+  o[30] = (float)0.9;
+  o[31] = (float)p->light_intensity;
+  o[32] = (float)p->ambient_intensity;
+  o[33] = (float)p->ambient_intensity;
+  o[34] = (float)(p->ambient_intensity * 1.1);
+  o[35] = p->specular_only ? 1.0f : 0.0f;
+  o[36] = (float)p->emissive_intensity;
+  o[37] = (float)p->emissive_threshold;
+  o[38] = (float)p->env_intensity;
+  o[39] = (float)p->env_rotation;
+  o[40] = (float)p->exposure;
+}
+
+void hdsr_material_curve_lut(const hdsr_material_curve* curve, float* out) {
+  if (!out) return;
+  const hdsr_material_curve id{1.0, 0.0, 1.0};
+  const hdsr_material_curve& c = curve ? *curve : id;
+  // JavaScript's Math.min / Math.max propagate NaN
+  auto clamp01 = [](double v) { return v != v ? v : (v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v)); };
+  for (int i = 0; i < 256; ++i) {
+    double v = i / 255.0;
+    v = (v - 0.5) * c.contrast + 0.5;
+    v += c.brightness;
+    v = clamp01(v);
+    v = std::pow(v, c.gamma);
+    const float f = (float)clamp01(v);  // Float32Array store
+    uint32_t bits;
+    std::memcpy(&bits, &f, 4);
+    // f32ToF16 of uploadLUT: mantissa truncated, subnormal results flushed to signed zero
+    const uint32_t sign = (bits >> 16) & 0x8000u;
+    const int e = (int)((bits >> 23) & 0xFFu) - 127 + 15;
+    const uint32_t frac = bits & 0x7FFFFFu;
+    uint32_t h16;
+    if (e <= 0) h16 = sign;
+    else if (e >= 31) h16 = sign | 0x7C00u;
+    else h16 = sign | ((uint32_t)e << 10) | (frac >> 13);
+    out[i] = hdsr_f16_to_f32(h16);
+  }
+}
+
+int hdsr_deferred_lighting(hdsr_ctx* c, const hdsr_lighting_params* params, void* stream) {
+  if (!c) return HDSR_E_INVALID_ARG;
+  int rc = deferred_ready(c);
+  if (rc) return rc;
+  hdsr_lighting_params p;
+  hdsr_lighting_params_default(&p);
+  if (params) p = *params;
+  float inv[16];
+  const float* vp = p.lighting_view_proj ? p.lighting_view_proj : c->frame.vp;
+  if (!mat4_inverse_ref(vp, inv)) return fail(c, HDSR_E_INVALID_ARG, "lighting viewProj is singular");
+  const float* cam = p.lighting_camera_position ? p.lighting_camera_position : c->frame.cam;
+  LightingBlock blk;
+  hdsr_write_lighting_params(&p, inv, cam, c->plan.viewport_width, c->plan.viewport_height, blk.f);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t px = (size_t)c->plan.viewport_width * c->plan.viewport_height;
+  if (!c->lit) HIPX(c, hipMalloc((void**)&c->lit, std::max<size_t>(px, 1) * 8));
+  if (!c->lut_dev) HIPX(c, hipMalloc((void**)&c->lut_dev, 768 * sizeof(float)));
+  std::vector<float> lut(768);
+  hdsr_material_curve_lut(&p.roughness_curve, lut.data());
+  hdsr_material_curve_lut(&p.metalness_curve, lut.data() + 256);
+  hdsr_material_curve_lut(&p.albedo_curve, lut.data() + 512);
+  if (lut != c->lut_host) {  // materialCurves.update: upload only what changed
+    HIPX(c, hipMemcpyAsync(c->lut_dev, lut.data(), 768 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPX(c, hipStreamSynchronize(s));  // the host table is a temporary
+    c->lut_host = lut;
+  }
+  const float* rgba = c->user_rgba ? c->user_rgba : c->rgba;
+  const float* depth = c->user_depth ? c->user_depth : c->depth;
+  launch_deferred_lighting(blk, rgba, depth, c->gbn, c->gbm, p.ao, p.bloom, c->lut_dev, c->lit,
+                           c->plan.viewport_width, c->plan.viewport_height, s);
+  HIPX(c, hipGetLastError());
+  c->last_stream = s;
   return HDSR_OK;
 }
 

@@ -34,7 +34,8 @@ HDSR_MEM_DEVICE = 1
 HDSR_SH_DEGREE_AUTO = 0xFFFFFFFF  # take the SH degree from the shData length
 HDSR_NOALLOC_MAX_SPLATS = 4194304  # scratch capacity of hdsr_sort_back_to_front_noalloc
 
-OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX, OUT_FILTERED_NORMAL = range(8)
+(OUT_RGBA, OUT_DEPTH, OUT_GB_NORMAL, OUT_GB_MATERIAL, OUT_RGBA16, OUT_T_FINAL, OUT_LAST_INDEX, OUT_FILTERED_NORMAL,
+ OUT_LIT) = range(9)
 
 DBG = {
     "projcache": 0, "depth": 1, "tile_counts": 2, "tile_offsets": 3, "entries": 4, "entries_canon": 5,
@@ -65,7 +66,7 @@ class Splats(C.Structure):
 class Outputs(C.Structure):
     _fields_ = [("rgba", C.c_void_p), ("depth", C.c_void_p), ("gbuffer_normal", C.c_void_p),
                 ("gbuffer_material", C.c_void_p), ("rgba16", C.c_void_p), ("width", C.c_uint32),
-                ("height", C.c_uint32)]
+                ("height", C.c_uint32), ("lit", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -85,6 +86,20 @@ class NormalParams(C.Structure):
     _fields_ = [("recovery_mode", C.c_uint32), ("grazing_start", C.c_float), ("grazing_end", C.c_float),
                 ("spatial_sigma", C.c_float), ("depth_sigma", C.c_float), ("normal_sigma", C.c_float),
                 ("kernel_radius", C.c_uint32)]
+
+
+class MaterialCurve(C.Structure):
+    _fields_ = [("contrast", C.c_double), ("brightness", C.c_double), ("gamma", C.c_double)]
+
+
+class LightingParams(C.Structure):
+    _fields_ = [("light_direction", C.c_double * 3), ("light_intensity", C.c_double),
+                ("ambient_intensity", C.c_double), ("specular_only", C.c_uint32),
+                ("emissive_intensity", C.c_double), ("emissive_threshold", C.c_double),
+                ("env_intensity", C.c_double), ("env_rotation", C.c_double), ("exposure", C.c_double),
+                ("roughness_curve", MaterialCurve), ("metalness_curve", MaterialCurve),
+                ("albedo_curve", MaterialCurve), ("lighting_view_proj", C.c_void_p),
+                ("lighting_camera_position", C.c_void_p), ("ao", C.c_void_p), ("bloom", C.c_void_p)]
 
 
 class PlyInfo(C.Structure):
@@ -146,6 +161,11 @@ SIGNATURES = [
     ("hdsr_stage_times", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
     ("hdsr_backward", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Grads), C.c_void_p]),
     ("hdsr_normal_params_default", None, [C.POINTER(NormalParams)]),
+    ("hdsr_lighting_params_default", None, [C.POINTER(LightingParams)]),
+    ("hdsr_deferred_lighting", C.c_int, [C.c_void_p, C.POINTER(LightingParams), C.c_void_p]),
+    ("hdsr_write_lighting_params", None, [C.POINTER(LightingParams), C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                          C.c_void_p]),
+    ("hdsr_material_curve_lut", None, [C.POINTER(MaterialCurve), C.c_void_p]),
     ("hdsr_screen_space_normals", C.c_int, [C.c_void_p, C.POINTER(NormalParams), C.c_void_p]),
     ("hdsr_bilateral_normal_filter", C.c_int, [C.c_void_p, C.POINTER(NormalParams), C.c_void_p]),
     ("hdsr_ply_inspect", C.c_int, [C.c_char_p, C.c_void_p, C.c_uint64, C.POINTER(PlyInfo), C.c_char_p, C.c_uint32]),

@@ -122,6 +122,34 @@ class Resources:
         p = params if params is not None else self.normal_params()
         check(self._lib.hdsr_bilateral_normal_filter(self.handle, C.byref(p), C.c_void_p(stream)), self.handle)
 
+    def lighting_params(self, **overrides) -> "_lib.LightingParams":
+        """The deferred-lighting parameters (hdsr_lighting_params_default: the reference viewer's
+        defaults), with overrides; curves as (contrast, brightness, gamma) tuples, ao / bloom as
+        device pointers (int) or None, light_direction / lighting_view_proj /
+        lighting_camera_position as sequences."""
+        p = _lib.LightingParams()
+        self._lib.hdsr_lighting_params_default(C.byref(p))
+        keep = []
+        for k, v in overrides.items():
+            if k == "light_direction":
+                p.light_direction[:] = [float(x) for x in v]
+            elif k.endswith("_curve"):
+                setattr(p, k, _lib.MaterialCurve(*[float(x) for x in v]))
+            elif k in ("lighting_view_proj", "lighting_camera_position"):
+                a = None if v is None else np.ascontiguousarray(v, np.float32)
+                keep.append(a)
+                setattr(p, k, None if a is None else a.ctypes.data)
+            else:
+                setattr(p, k, v)
+        p._keep = keep  # host arrays referenced by the struct
+        return p
+
+    def encodeDeferredLighting(self, stream=None, params=None) -> None:
+        """createDeferredLightingPass().encode (src/splatRenderer.ts:446-548, :1115-1139): colour,
+        depth, G-buffer normals and materials -> the lit rgba16float buffer (OUT_LIT; gbuffer=True)."""
+        p = params if params is not None else self.lighting_params()
+        check(self._lib.hdsr_deferred_lighting(self.handle, C.byref(p), C.c_void_p(stream)), self.handle)
+
     def debug(self, name: str) -> np.ndarray:
         which = _lib.DBG[name]
         n = C.c_uint64()

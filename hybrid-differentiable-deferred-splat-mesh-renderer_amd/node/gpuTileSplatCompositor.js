@@ -38,7 +38,7 @@ var TEXTURE_FORMATS = {
   r32uint: { out: 2, ctor: Uint32Array, per: 1, flag: 4 },
   rgba32uint: { out: 3, ctor: Uint32Array, per: 4, flag: 4 },
 };
-var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4, filteredNormal: 7 };
+var OUT = { rgba: 0, depth: 1, gbufferNormal: 2, gbufferMaterial: 3, rgba16: 4, filteredNormal: 7, lit: 8 };
 var DBG = {
   projCache: 0, depth: 1, tileCounts: 2, tileOffsets: 3, entries: 4, entriesCanon: 5, smallTileList: 6,
   largeTileList: 7, largeTileOverflowBases: 8, chunkRanges: 9, keys20: 10, largeSplatList: 11, sortedIndices: 12,
@@ -250,6 +250,19 @@ function encodeBilateralNormalFilter(encoder, resources, options) {
   addon.bilateralNormalFilter(resources.handle, options || {});
 }
 
+// Deferred lighting (createDeferredLightingPass().encode, src/splatRenderer.ts:446-548, as encoded
+// at :1115-1139): colour, depth, G-buffer normals (after the two passes above) and materials ->
+// the lit rgba16float buffer (readback output 'lit').  options take encode's arguments by name:
+// lightDirection [x, y, z], lightIntensity, ambientIntensity, specularOnly, emissiveIntensity,
+// emissiveThreshold, envIntensity, envRotation, exposure; the MaterialCurveParams roughnessCurve /
+// metalnessCurve / albedoCurve ({contrast, brightness, gamma}); lightingViewProj (16 numbers) and
+// lightingCameraPosition (3) for a separate lighting camera; ao and bloom as device addresses
+// (BigInt) of the AO texture (H*W*4 f32) and the half-resolution bloom texture, or null.  Unset
+// fields take the reference viewer's values; the environment-map path is not built.
+function encodeDeferredLighting(encoder, resources, options) {
+  addon.deferredLighting(resources.handle, options || {});
+}
+
 function encodeCompositeOnly(encoder, resources, bindGroups) {
   addon.compositeOnly(resources.handle);
 }
@@ -303,6 +316,7 @@ module.exports = {
   encodeViewDepthSort: encodeViewDepthSort,
   encodeScreenSpaceNormals: encodeScreenSpaceNormals,
   encodeBilateralNormalFilter: encodeBilateralNormalFilter,
+  encodeDeferredLighting: encodeDeferredLighting,
   createDevice: createDevice,
   version: addon.version,
 };

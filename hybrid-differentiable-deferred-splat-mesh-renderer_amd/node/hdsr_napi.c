@@ -367,7 +367,7 @@ static napi_value js_outputs(napi_env env, napi_callback_info info) {
     const char* key;
     const void* p;
   } f[] = {{"rgba", o.rgba}, {"depth", o.depth}, {"gbufferNormal", o.gbuffer_normal},
-           {"gbufferMaterial", o.gbuffer_material}, {"rgba16", o.rgba16}};
+           {"gbufferMaterial", o.gbuffer_material}, {"rgba16", o.rgba16}, {"lit", o.lit}};
   for (size_t i = 0; i < sizeof f / sizeof f[0]; ++i) {
     CALL(env, napi_create_bigint_uint64(env, (uint64_t)(uintptr_t)f[i].p, &v));
     CALL(env, napi_set_named_property(env, out, f[i].key, v));
@@ -785,6 +785,138 @@ static napi_value js_screen_space_normals(napi_env env, napi_callback_info info)
   return NULL;
 }
 
+/* A property holding a number array (plain or typed) of exactly n elements, into out; false when
+ * absent.  A present property of another shape throws a TypeError (returns -1). */
+static int get_vec(napi_env env, napi_value obj, const char* key, double* out, uint32_t n) {
+  bool has = false;
+  napi_value v;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return 0;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return 0;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_undefined || t == napi_null) return 0;
+  bool is_arr = false, is_ta = false;
+  napi_is_array(env, v, &is_arr);
+  napi_is_typedarray(env, v, &is_ta);
+  uint32_t len = 0;
+  if (is_arr) {
+    napi_get_array_length(env, v, &len);
+  } else if (is_ta) {
+    napi_typedarray_type tt;
+    size_t l, off;
+    void* d;
+    napi_value ab;
+    napi_get_typedarray_info(env, v, &tt, &l, &d, &ab, &off);
+    len = (uint32_t)l;
+  }
+  if ((!is_arr && !is_ta) || len != n) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "%s must be an array of %u numbers", key, n);
+    napi_throw_type_error(env, NULL, msg);
+    return -1;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value e;
+    napi_get_element(env, v, i, &e);
+    if (napi_get_value_double(env, e, &out[i]) != napi_ok) {
+      char msg[128];
+      snprintf(msg, sizeof msg, "%s must be an array of %u numbers", key, n);
+      napi_throw_type_error(env, NULL, msg);
+      return -1;
+    }
+  }
+  return 1;
+}
+
+static void curve_from(napi_env env, napi_value obj, const char* key, hdsr_material_curve* c) {
+  bool has = false;
+  napi_value v;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_object) return;
+  c->contrast = get_f64(env, v, "contrast", c->contrast);
+  c->brightness = get_f64(env, v, "brightness", c->brightness);
+  c->gamma = get_f64(env, v, "gamma", c->gamma);
+}
+
+/* A BigInt device address property (0 when absent / null); -1 for another type (TypeError). */
+static int get_devptr(napi_env env, napi_value obj, const char* key, const float** out) {
+  bool has = false;
+  napi_value v;
+  *out = NULL;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return 0;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return 0;
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_undefined || t == napi_null) return 0;
+  if (t != napi_bigint) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "%s must be a device address (BigInt) or null", key);
+    napi_throw_type_error(env, NULL, msg);
+    return -1;
+  }
+  uint64_t a = 0;
+  bool lossless = false;
+  napi_get_value_bigint_uint64(env, v, &a, &lossless);
+  *out = (const float*)(uintptr_t)a;
+  return 1;
+}
+
+/* deferredLighting(handle, options): the encode arguments of createDeferredLightingPass
+ * (src/splatRenderer.ts:484-506) by name; unset fields take the reference viewer's values. */
+static napi_value js_deferred_lighting(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  CALL(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+  hdsr_ctx* ctx = ctx_arg(env, argv[0]);
+  if (!ctx) return NULL;
+  hdsr_lighting_params p;
+  hdsr_lighting_params_default(&p);
+  napi_valuetype t = napi_undefined;
+  if (argc > 1) napi_typeof(env, argv[1], &t);
+  float vp[16], cam[3];
+  if (t == napi_object) {
+    const napi_value o = argv[1];
+    if (get_vec(env, o, "lightDirection", p.light_direction, 3) < 0) return NULL;
+    p.light_intensity = get_f64(env, o, "lightIntensity", p.light_intensity);
+    p.ambient_intensity = get_f64(env, o, "ambientIntensity", p.ambient_intensity);
+    bool has = false;
+    napi_value sv;
+    if (napi_has_named_property(env, o, "specularOnly", &has) == napi_ok && has &&
+        napi_get_named_property(env, o, "specularOnly", &sv) == napi_ok) {
+      bool b = false;
+      if (napi_get_value_bool(env, sv, &b) == napi_ok) p.specular_only = b ? 1u : 0u;
+    }
+    p.emissive_intensity = get_f64(env, o, "emissiveIntensity", p.emissive_intensity);
+    p.emissive_threshold = get_f64(env, o, "emissiveThreshold", p.emissive_threshold);
+    p.env_intensity = get_f64(env, o, "envIntensity", p.env_intensity);
+    p.env_rotation = get_f64(env, o, "envRotation", p.env_rotation);
+    p.exposure = get_f64(env, o, "exposure", p.exposure);
+    curve_from(env, o, "roughnessCurve", &p.roughness_curve);
+    curve_from(env, o, "metalnessCurve", &p.metalness_curve);
+    curve_from(env, o, "albedoCurve", &p.albedo_curve);
+    double d16[16], d3[3];
+    int r = get_vec(env, o, "lightingViewProj", d16, 16);
+    if (r < 0) return NULL;
+    if (r > 0) {
+      for (int i = 0; i < 16; ++i) vp[i] = (float)d16[i];
+      p.lighting_view_proj = vp;
+    }
+    r = get_vec(env, o, "lightingCameraPosition", d3, 3);
+    if (r < 0) return NULL;
+    if (r > 0) {
+      for (int i = 0; i < 3; ++i) cam[i] = (float)d3[i];
+      p.lighting_camera_position = cam;
+    }
+    if (get_devptr(env, o, "ao", &p.ao) < 0 || get_devptr(env, o, "bloom", &p.bloom) < 0) return NULL;
+  }
+  const int rc = hdsr_deferred_lighting(ctx, &p, NULL);
+  if (rc) return throw_rc(env, rc, ctx, "hdsr_deferred_lighting");
+  return NULL;
+}
+
 static napi_value js_bilateral_normal_filter(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -828,6 +960,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"plyDecode", NULL, js_ply_decode, NULL, NULL, NULL, napi_default, NULL},
       {"screenSpaceNormals", NULL, js_screen_space_normals, NULL, NULL, NULL, napi_default, NULL},
       {"bilateralNormalFilter", NULL, js_bilateral_normal_filter, NULL, NULL, NULL, napi_default, NULL},
+      {"deferredLighting", NULL, js_deferred_lighting, NULL, NULL, NULL, napi_default, NULL},
   };
   napi_define_properties(env, exports, sizeof props / sizeof props[0], props);
   return exports;

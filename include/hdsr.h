@@ -163,6 +163,7 @@ typedef struct hdsr_outputs {
   uint16_t* rgba16;               /* H*W*4 binary16, if HDSR_OUT_RGBA16 */
   uint32_t width;
   uint32_t height;
+  uint16_t* lit;                  /* H*W*4 binary16 of hdsr_deferred_lighting, after its first call */
 } hdsr_outputs;
 int hdsr_get_outputs(hdsr_ctx* ctx, hdsr_outputs* out);
 
@@ -180,6 +181,7 @@ int hdsr_bind_outputs(hdsr_ctx* ctx, float* rgba, float* depth);
 #define HDSR_OUT_T_FINAL 5        /* H*W f32 final transmittance (HDSR_DIFFERENTIABLE) */
 #define HDSR_OUT_LAST_INDEX 6     /* H*W u32 one past the last blended entry of the pixel's tile list */
 #define HDSR_OUT_FILTERED_NORMAL 7 /* H*W u32 oct-normal words of hdsr_screen_space_normals (HDSR_GBUFFER) */
+#define HDSR_OUT_LIT 8            /* H*W*4 binary16: hdsr_deferred_lighting's output (rgba16float) */
 /* Copies an output to host memory (after hdsr_sync). */
 int hdsr_readback(hdsr_ctx* ctx, int which, void* host, uint64_t bytes);
 
@@ -209,6 +211,50 @@ int hdsr_screen_space_normals(hdsr_ctx* ctx, const hdsr_normal_params* params, v
  * filtered-normal buffer and depth -> the G-buffer normal words, which it overwrites, as the
  * reference writes its result back into the G-buffer normal texture. */
 int hdsr_bilateral_normal_filter(hdsr_ctx* ctx, const hdsr_normal_params* params, void* stream);
+
+/* Deferred lighting (createDeferredLightingPass().encode, src/splatRenderer.ts:446-548, encoded at
+ * :1115-1139; shader src/shaders/gpu_deferred_lighting.wgsl): GGX specular + Lambertian diffuse of
+ * one directional light over the G-buffer, ambient * albedo * AO, Reinhard tone mapping, sRGB
+ * encoding and additive emissive, into the context's lit buffer (HDSR_OUT_LIT, rgba16float bits).
+ * Inputs: the last render's colour (read as its rgba16float output texture holds it), depth,
+ * G-buffer normal words (after hdsr_bilateral_normal_filter when the caller runs the normal
+ * passes first, as the reference does) and material words.  The path without an environment map
+ * is the one built (the reference's 1 x 1 placeholder: env_intensity / env_rotation have no
+ * effect).  Numbers are the reference's JavaScript numbers (double), rounded to fp32 where its
+ * Float32Array stores them. */
+typedef struct hdsr_material_curve {  /* MaterialCurveParams (src/materialCurves.ts:13-17) */
+  double contrast;                    /* 1 = identity */
+  double brightness;                  /* 0 */
+  double gamma;                       /* 1 */
+} hdsr_material_curve;
+typedef struct hdsr_lighting_params {
+  double light_direction[3];          /* direction the light travels (the shader lights along -dir) */
+  double light_intensity;
+  double ambient_intensity;           /* ambient colour = (a, a, 1.1 a) */
+  uint32_t specular_only;
+  double emissive_intensity;          /* 3.0 */
+  double emissive_threshold;          /* 0.05 */
+  double env_intensity;               /* 1.0 */
+  double env_rotation;                /* 0.0 */
+  double exposure;                    /* 1.0 */
+  hdsr_material_curve roughness_curve, metalness_curve, albedo_curve;  /* DEFAULT_CURVE {1, 0, 1} */
+  const float* lighting_view_proj;    /* host, 16 floats (params.lightingViewProj), or null: the frame's */
+  const float* lighting_camera_position; /* host, 3 floats, or null: the frame's camera position */
+  const float* ao;                    /* device, H*W*4 f32: the AO texture's texels (r = visibility), or
+                                         null: visibility 1 everywhere */
+  const float* bloom;                 /* device, ceil(H/2)*ceil(W/2)*4 f32: the half-resolution bloom
+                                         texture's texels, or null: no bloom */
+} hdsr_lighting_params;
+void hdsr_lighting_params_default(hdsr_lighting_params* out);
+int hdsr_deferred_lighting(hdsr_ctx* ctx, const hdsr_lighting_params* params, void* stream);
+/* The 176-byte Params block the pass uses (src/splatRenderer.ts:505-523), written from 'params',
+ * the inverse view-projection, the camera position and the viewport.  Host only. */
+void hdsr_write_lighting_params(const hdsr_lighting_params* params, const float* view_proj_inverse16,
+                                const float* camera_position3, uint32_t width, uint32_t height, float out44[44]);
+/* A material curve's 256-entry lookup table as the pass reads it: generateLUT
+ * (src/materialCurves.ts:31-42) in double, stored to fp32, then to binary16 by uploadLUT's
+ * truncating conversion (:45-57), read back as fp32.  Host only. */
+void hdsr_material_curve_lut(const hdsr_material_curve* curve, float out256[256]);
 
 /* Intermediates of the last frame, in the reference's layouts (ranks, Morton ids). */
 #define HDSR_DBG_PROJCACHE 0      /* N*14 u32 (gpu_project_splats.wgsl:2-16); HDSR_DEBUG or HDSR_GBUFFER */

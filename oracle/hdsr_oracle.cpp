@@ -18,6 +18,7 @@
 //   deferred normals      src/shaders/gpu_screen_space_normals.wgsl:59-120,
 //                         src/shaders/bilateral_normal_filter.wgsl:39-86, mat4Inverse
 //                         (src/splatRenderer.ts:148-168)
+//   deferred lighting     src/shaders/gpu_deferred_lighting.wgsl:80-108,130-270 (no environment map)
 // and the plan arithmetic of src/gpuTileSplatCompositor.ts:109-139, 730-774.
 //
 // Decisions where the reference is nondeterministic or under-specified:
@@ -1230,6 +1231,141 @@ void orc_logf_n(const float* x, float* y, uint64_t n) {
 }
 void orc_f16_bits_n(const float* x, uint32_t* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_f16_bits(x[i]);
+}
+
+// ---- deferred lighting (src/shaders/gpu_deferred_lighting.wgsl, path without an environment map) ----
+// The three factors of the Cook-Torrance term, exported so tests can pin each one.
+
+// distributionGGX (gpu_deferred_lighting.wgsl:80-86)
+float orc_ggx_d(float n_dot_h, float roughness) {
+  const float a = roughness * roughness;
+  const float a2 = a * a;
+  const float d = (n_dot_h * n_dot_h) * (a2 - 1.0f) + 1.0f;
+  return a2 / ((3.14159265359f * d) * d);
+}
+// geometrySmith with the direct-light k = (roughness + 1)^2 / 8 (:88-99)
+float orc_smith_g(float n_dot_v, float n_dot_l, float roughness) {
+  const float r = roughness + 1.0f;
+  const float k = (r * r) / 8.0f;
+  const float gv = n_dot_v / (n_dot_v * (1.0f - k) + k);
+  const float gl = n_dot_l / (n_dot_l * (1.0f - k) + k);
+  return gv * gl;
+}
+// fresnelSchlick, one channel (:101-104)
+float orc_fresnel(float cos_theta, float f0) {
+  const float t = std::fmin(std::fmax(1.0f - cos_theta, 0.0f), 1.0f);
+  return f0 + (1.0f - f0) * hdsr_expf(5.0f * hdsr_logf(t));
+}
+
+// main (:130-270) over a W x H G-buffer.  P: the 44-float Params block; luts: roughness,
+// metalness and albedo tables (3 x 256); ao (W*H*4) and bloom (ceil(W/2)*ceil(H/2)*4) may be null.
+void orc_deferred_lighting(const float* rgba, const float* depth, const uint32_t* normal, const uint32_t* mat,
+                           const float* ao, const float* bloom, const float* luts, const float* P, uint32_t W,
+                           uint32_t H, uint16_t* out) {
+  auto pw = [](float x, float y) { return hdsr_expf(y * hdsr_logf(x)); };
+  auto sat = [](float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); };
+  auto h16 = [](float x) { return hdsr_f16_to_f32(hdsr_f16_bits(x)); };
+  auto lut_at = [&](int which, float v) {
+    // textureLoad(lut, clamp(i32(v * 255.0), 0, 255)): i32 truncates toward zero, NaN -> 0
+    const float s = v * 255.0f;
+    long long i = 0;
+    if (s == s) i = s <= -2147483648.0f ? INT32_MIN : (s >= 2147483647.0f ? INT32_MAX : (long long)s);
+    const int k = (int)std::min<long long>(std::max<long long>(i, 0), 255);
+    return luts[which * 256 + k];
+  };
+  const uint32_t bw = (W + 1) / 2, bh = (H + 1) / 2;
+  const float lum_w[3] = {0.2126f, 0.7152f, 0.0722f};
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      const size_t o = (size_t)y * W + x;
+      float col[3] = {h16(rgba[o * 4]), h16(rgba[o * 4 + 1]), h16(rgba[o * 4 + 2])};
+      const float alpha = h16(rgba[o * 4 + 3]);
+      float res[4];
+      const float dep = depth[o];
+      if (dep >= 0.9999f || dep <= 0.0001f) {
+        res[0] = col[0];
+        res[1] = col[1];
+        res[2] = col[2];
+        res[3] = 0.0f;
+      } else {
+        const float lum = (col[0] * lum_w[0] + col[1] * lum_w[1]) + col[2] * lum_w[2];
+        const float remapped = lut_at(2, lum);
+        const float scale = lum < 0.001f ? 1.0f : remapped / std::fmax(lum, 0.001f);
+        float alb[3];
+        for (int c = 0; c < 3; ++c) alb[c] = pw(std::fmax(col[c] * scale, 0.0f), 2.2f);
+        const D3 nraw = oct_dec(normal[o]);
+        const float cx = (float)x + 0.5f, cy = (float)y + 0.5f;
+        const float ndx = cx / P[0] * 2.0f - 1.0f, ndy = 1.0f - cy / P[1] * 2.0f;
+        const float* M = P + 4;
+        float wv[4];
+        for (int r = 0; r < 4; ++r) wv[r] = ((M[r] * ndx + M[4 + r] * ndy) + M[8 + r] * dep) + M[12 + r];
+        const D3 world{wv[0] / wv[3], wv[1] / wv[3], wv[2] / wv[3]};
+        const D3 V = d3_norm(D3{P[20] - world.x, P[21] - world.y, P[22] - world.z});
+        const D3 N = face_fwd(nraw, V);
+        const D3 L = d3_norm(D3{-P[24], -P[25], -P[26]});
+        const D3 Hv = d3_norm(D3{V.x + L.x, V.y + L.y, V.z + L.z});
+        const float ndl = std::fmax(d3_dot(N, L), 0.0f);
+        const float ndv = std::fmax(d3_dot(N, V), 0.001f);
+        const float ndh = std::fmax(d3_dot(N, Hv), 0.0f);
+        const float hdv = std::fmax(d3_dot(Hv, V), 0.0f);
+        const uint32_t* m4 = mat + o * 4;
+        const float rough = std::fmax(lut_at(0, hdsr_f16_to_f32(m4[0] & 0xFFFFu)), 0.04f);
+        const float metal = lut_at(1, hdsr_f16_to_f32(m4[0] >> 16));
+        const float Dv = orc_ggx_d(ndh, rough);
+        const float Gv = orc_smith_g(ndv, ndl, rough);
+        const float denom = (4.0f * ndv) * ndl + 0.0001f;
+        float spec[3], kd[3];
+        for (int c = 0; c < 3; ++c) {
+          const float f0 = 0.04f * (1.0f - metal) + alb[c] * metal;  // mix(vec3(0.04), albedo, metallic)
+          const float Fc = orc_fresnel(hdv, f0);
+          spec[c] = ((Dv * Gv) * Fc) / denom;
+          kd[c] = (1.0f - Fc) * (1.0f - metal);
+        }
+        const float ao_raw = ao ? ao[o * 4] : 1.0f;
+        const float em[3] = {hdsr_f16_to_f32(m4[1] & 0xFFFFu), hdsr_f16_to_f32(m4[1] >> 16),
+                             hdsr_f16_to_f32(m4[2] & 0xFFFFu)};
+        const float em_mag = std::fmax(em[0], std::fmax(em[1], em[2]));
+        float bl[3] = {0.0f, 0.0f, 0.0f};
+        if (bloom) {
+          // textureSampleLevel with linear filtering: U repeats, V clamps to the edge
+          const float tx = (cx / P[0]) * (float)bw - 0.5f, ty = (cy / P[1]) * (float)bh - 0.5f;
+          const float fx0 = std::floor(tx), fy0 = std::floor(ty);
+          const float fx = tx - fx0, fy = ty - fy0;
+          const long long ix = (long long)fx0, iy = (long long)fy0;
+          const long long xs[2] = {((ix % bw) + bw) % bw, (((ix + 1) % bw) + bw) % bw};
+          const long long ys[2] = {std::min<long long>(std::max<long long>(iy, 0), bh - 1),
+                                   std::min<long long>(std::max<long long>(iy + 1, 0), bh - 1)};
+          const float wts[4] = {(1.0f - fx) * (1.0f - fy), fx * (1.0f - fy), (1.0f - fx) * fy, fx * fy};
+          for (int c = 0; c < 3; ++c) {
+            float acc = 0.0f;
+            for (int k = 0; k < 4; ++k) {
+              const float t = bloom[((size_t)ys[k >> 1] * bw + xs[k & 1]) * 4 + c];
+              acc = k == 0 ? t * wts[0] : acc + t * wts[k];
+            }
+            bl[c] = acc;
+          }
+        }
+        const float bloom_mag = (bl[0] * lum_w[0] + bl[1] * lum_w[1]) + bl[2] * lum_w[2];
+        const float aov = std::fmax(ao_raw, std::fmax(sat((em_mag * P[36]) * 0.5f), sat(bloom_mag * 2.0f)));
+        const float light[3] = {P[28], P[29], P[30]};
+        float color[3];
+        for (int c = 0; c < 3; ++c) {
+          if (P[35] > 0.5f) {
+            color[c] = ((spec[c] * light[c]) * P[31]) * ndl;
+          } else {
+            const float Lo = ((((kd[c] * alb[c]) / 3.14159265359f + spec[c]) * light[c]) * P[31]) * ndl;
+            color[c] = (P[32 + c] * alb[c]) * aov + Lo;
+          }
+        }
+        for (int c = 0; c < 3; ++c) {
+          const float e = color[c] * P[40];
+          res[c] = pw(std::fmax(e / (e + 1.0f), 0.0f), 0.454545454545f);
+          if (em_mag > P[37]) res[c] = res[c] + em[c] * P[36];
+        }
+        res[3] = alpha;
+      }
+      for (int c = 0; c < 4; ++c) out[o * 4 + c] = (uint16_t)hdsr_f16_bits(res[c]);
+    }
 }
 
 }  // extern "C"

@@ -77,6 +77,11 @@ def load() -> C.CDLL:
     lib.orc_bilateral_normals.restype = None
     lib.orc_bilateral_normals.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                           C.c_float, C.c_int, C.c_void_p]
+    lib.orc_deferred_lighting.restype = None
+    lib.orc_deferred_lighting.argtypes = [C.c_void_p] * 8 + [C.c_uint32, C.c_uint32, C.c_void_p]
+    for name, nargs in (("orc_ggx_d", 2), ("orc_smith_g", 3), ("orc_fresnel", 2)):
+        getattr(lib, name).restype = C.c_float
+        getattr(lib, name).argtypes = [C.c_float] * nargs
     lib.orc_u32sat.restype = C.c_uint32
     lib.orc_u32sat.argtypes = [C.c_float]
     lib.orc_morton2.restype = C.c_uint32
@@ -116,6 +121,32 @@ def bilateral_normals(words, depth, spatial_sigma=3.0, depth_sigma=0.05, normal_
     out = np.zeros((H, W), np.uint32)
     load().orc_bilateral_normals(w.ctypes.data, d.ctypes.data, W, H, spatial_sigma, depth_sigma, normal_sigma, radius,
                                  out.ctypes.data)
+    return out
+
+
+def deferred_lighting(rgba, depth, normal_words, material_words, params44, luts, ao=None, bloom=None):
+    """gpu_deferred_lighting.wgsl (no environment map) over an (H, W) G-buffer: rgba (H, W, 4)
+    fp32, depth (H, W), normal words (H, W), material words (H, W, 4), the 44-float Params block,
+    the 3 x 256 curve tables; ao (H, W, 4) and bloom (ceil(H/2), ceil(W/2), 4) optional.
+    Returns (H, W, 4) binary16 bits (uint16)."""
+    c = np.ascontiguousarray(rgba, np.float32)
+    d = np.ascontiguousarray(depth, np.float32)
+    H, W = d.shape
+    n = np.ascontiguousarray(normal_words, np.uint32)
+    m = np.ascontiguousarray(material_words, np.uint32)
+    P = np.ascontiguousarray(params44, np.float32)
+    lt = np.ascontiguousarray(luts, np.float32)
+    assert c.size == H * W * 4 and n.size == H * W and m.size == H * W * 4 and P.size == 44 and lt.size == 768
+    a = None if ao is None else np.ascontiguousarray(ao, np.float32)
+    b = None if bloom is None else np.ascontiguousarray(bloom, np.float32)
+    if a is not None:
+        assert a.size == H * W * 4
+    if b is not None:
+        assert b.size == ((H + 1) // 2) * ((W + 1) // 2) * 4
+    out = np.zeros((H, W, 4), np.uint16)
+    load().orc_deferred_lighting(c.ctypes.data, d.ctypes.data, n.ctypes.data, m.ctypes.data,
+                                 None if a is None else a.ctypes.data, None if b is None else b.ctypes.data,
+                                 lt.ctypes.data, P.ctypes.data, W, H, out.ctypes.data)
     return out
 
 

@@ -52,6 +52,12 @@ if (meta.gbuffer) {
   save('ssn_js.bin', resources.readOutput('filteredNormal', new Uint32Array(px)));
   C.encodeBilateralNormalFilter(encoder, resources, opts);
   save('bil_js.bin', resources.readOutput('gbufferNormal', new Uint32Array(px)));
+  // then the lighting pass, with encode's arguments by name (src/splatRenderer.ts:1115-1139)
+  C.encodeDeferredLighting(encoder, resources, {
+    lightDirection: [0.3, -0.8, -0.5], lightIntensity: 2.5, ambientIntensity: 0.12, exposure: 1.2,
+    emissiveIntensity: 3.0, emissiveThreshold: 0.05, roughnessCurve: { contrast: 1.2, brightness: 0.0, gamma: 0.9 },
+  });
+  save('lit_js.bin', resources.readOutput('lit', new Uint16Array(px * 4)));
 }
 console.log(JSON.stringify(resources.stats()));
 resources.destroy();

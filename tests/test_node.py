@@ -5,6 +5,7 @@ Python host's (and the reference's formulas), unsupported options raise.
 GPU: one frame rendered through the facade in the reference's call order equals the oracle
 (bit-exact entries / keys, pixels <= 1e-5).
 """
+import ctypes as C
 import json
 import math
 import os
@@ -15,7 +16,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from hdsr import camera, compositor, scenes
+from hdsr import _lib, camera, compositor, scenes
 from oracle import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -104,6 +105,23 @@ def test_facade_renders_like_oracle(hip_device, gbuffer, sh3):
             np.testing.assert_array_equal(ssn, want)
             bil = np.fromfile(os.path.join(d, "bil_js.bin"), np.uint32).reshape(320, 480)
             np.testing.assert_array_equal(bil, oracle.bilateral_normals(want, depth.reshape(320, 480)))
+            # the lighting pass through the facade, with the options render_scene.js passes
+            p = _lib.LightingParams()
+            lib = _lib.load()
+            lib.hdsr_lighting_params_default(C.byref(p))
+            p.light_direction[:] = [0.3, -0.8, -0.5]
+            p.light_intensity, p.ambient_intensity, p.exposure = 2.5, 0.12, 1.2
+            p.roughness_curve = _lib.MaterialCurve(1.2, 0.0, 0.9)
+            P = np.zeros(44, np.float32)
+            cam3 = np.ascontiguousarray(u[28:31], np.float32)
+            lib.hdsr_write_lighting_params(C.byref(p), inv.ctypes.data, cam3.ctypes.data, 480, 320, P.ctypes.data)
+            luts = np.zeros(768, np.float32)
+            for i, cv in enumerate((p.roughness_curve, p.metalness_curve, p.albedo_curve)):
+                lib.hdsr_material_curve_lut(C.byref(cv), luts[i * 256:].ctypes.data)
+            gbm = np.fromfile(os.path.join(d, "gbm_tex.bin"), np.uint32).reshape(320, 480, 4)
+            lit = oracle.deferred_lighting(rgba.reshape(320, 480, 4), depth.reshape(320, 480), bil, gbm, P, luts)
+            np.testing.assert_array_equal(np.fromfile(os.path.join(d, "lit_js.bin"), np.uint16).reshape(320, 480, 4),
+                                          lit)
 
 
 @pytest.mark.gpu
