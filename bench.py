@@ -37,8 +37,16 @@ SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_b
 # x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  k_segsort and k_composite run twice per frame
 # (the deferred chunk sort's two rounds): theirs are the sums of both launches.  Valid for the
 # kernels as profiled.
-PMC_TRAFFIC_BYTES = {"k_project": 670.5e6, "k_emit": 30.6e6, "k_bucket": 741.0e6, "k_segsort": 218.0e6,
-                     "k_composite": 197.2e6}
+PMC_TRAFFIC_BYTES = {"k_project": 317.3e6, "k_emit": 30.6e6, "k_bucket": 761.8e6, "k_segsort": 217.9e6,
+                     "k_composite": 191.8e6}
+# What bounds the rasterizer instead: its SQ counters on the same workload (first round, which carries
+# nearly all of its work; profiles/sq_counters_1M_sh3_1080p.md).  A wave64 VALU instruction occupies
+# its SIMD for 4 cycles, so 0.25 instructions per SIMD per cycle is the issue ceiling; the LDS peak is
+# 128 B per clock per CU over 256 CUs at 2.4 GHz.
+COMPOSITE_SQ = {"valu_busy": 0.917, "valu_instr_per_simd_cycle": 0.228, "valu_issue_ceiling": 0.25,
+                "valu_frac": round(0.228 / 0.25, 3), "occupancy": 0.56, "lds_GBps": 22650.0,
+                "lds_peak_GBps": 78643.0, "lds_frac": round(22650.0 / 78643.0, 3),
+                "source": "rocprofv3 SQ counters, round 1 of k_composite (profiles/sq_counters_1M_sh3_1080p.md)"}
 # launches per frame of the single-kernel stages (the deferred rounds)
 LAUNCHES_PER_FRAME = {"k_segsort": 2, "k_composite": 2}
 
@@ -59,6 +67,9 @@ def parse():
     ap.add_argument("--no-stress", action="store_true", help="skip the 6M-splat 3840x2160 stress workload")
     ap.add_argument("--stress-only", action="store_true", help="run only the stress workload")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving sweep")
+    ap.add_argument("--serving-rates", default="", help="comma-separated request rates (Hz) of the serving sweep")
+    ap.add_argument("--serving-modes", default="batched,per_request",
+                    help="serving modes to run (batched, per_request)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -128,7 +139,9 @@ def main() -> int:
         print(json.dumps({"stress": stress_bench()}))
         return 0
     if a.serving_only:
-        print(json.dumps({"serving": serving_bench()}))
+        rates = tuple(int(x) for x in a.serving_rates.split(",") if x) or SERVING_RATES
+        modes = tuple(m for m in a.serving_modes.split(",") if m)
+        print(json.dumps({"serving": serving_bench(rates=rates, modes=modes)}))
         return 0
 
     t_setup = time.time()
@@ -267,7 +280,8 @@ def main() -> int:
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": PMC_TRAFFIC_BYTES.get(dom_kernel) if full_size else None,
                      "algorithmic_bytes_per_frame": dom_bytes, "ms_per_frame": round(stage_ms[dom], 4),
-                     "launches_per_frame": LAUNCHES_PER_FRAME.get(dom_kernel, 1)},
+                     "launches_per_frame": LAUNCHES_PER_FRAME.get(dom_kernel, 1),
+                     "issue": COMPOSITE_SQ if dom_kernel == "k_composite" and full_size else None},
         "stages_ms_per_frame": per_stage,
         "stages_within_peak": not over_peak,
         "frame_stats": {k2: round(v2, 1) for k2, v2 in mean_st.items()},
@@ -412,9 +426,12 @@ def f16_bench(a, scene, plan, sorted_host, sorted_dev, uniforms, handle, frames)
 
 
 def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
-    """The first deferred consumers of the G-buffer on view 0: screen-space normal reconstruction
-    and the bilateral normal filter (src/splatRenderer.ts:1042-1069), each timed by a HIP event pair
-    around its launch.  Algorithmic bytes per pixel: 12 (depth + normal word in, word out) each."""
+    """The deferred consumers of the G-buffer on view 0: screen-space normal reconstruction, the
+    bilateral normal filter and the lighting pass (src/splatRenderer.ts:1042-1069, 1115-1139), each
+    timed by a HIP event pair around its launch.  Algorithmic bytes per pixel: 12 (depth + normal
+    word in, word out) for each normal pass; 64 for the lighting pass with an AO texture and bloom
+    (colour 16, depth 4, normal 4, material 16, AO 16 in; rgba16float 8 out; the quarter-size
+    bloom texture's 4 B per pixel is left out as it is read through the cache)."""
     import torch
 
     from hdsr import compositor
@@ -425,8 +442,14 @@ def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
     res.sync()
     stream = torch.cuda.current_stream()
     out = {}
+    px = plan["viewportWidth"] * plan["viewportHeight"]
+    ao = torch.rand(px * 4, device="cuda")
+    bloom = torch.rand(((plan["viewportWidth"] + 1) // 2) * ((plan["viewportHeight"] + 1) // 2) * 4, device="cuda")
+    lp = res.lighting_params(ao=ao.data_ptr(), bloom=bloom.data_ptr())
+    per_px = {"screen_space_normals": 12, "bilateral_normal_filter": 12, "deferred_lighting": 64}
     for name, fn in (("screen_space_normals", res.encodeScreenSpaceNormals),
-                     ("bilateral_normal_filter", res.encodeBilateralNormalFilter)):
+                     ("bilateral_normal_filter", res.encodeBilateralNormalFilter),
+                     ("deferred_lighting", lambda h: res.encodeDeferredLighting(h, lp))):
         fn(handle)  # warm
         total = 0.0
         for _ in range(reps):
@@ -438,21 +461,24 @@ def deferred_bench(scene, plan, sorted_host, uniforms, handle, reps=20) -> dict:
             e1.synchronize()
             total += e0.elapsed_time(e1)
         ms = total / reps
-        px = plan["viewportWidth"] * plan["viewportHeight"]
-        out[name] = {"ms": round(ms, 4), "GBps": round(12 * px / (ms * 1e-3) / 1e9, 1)}
+        out[name] = {"ms": round(ms, 4), "GBps": round(per_px[name] * px / (ms * 1e-3) / 1e9, 1)}
     res.destroy()
-    out["note"] = "view 0 at 1920x1080 with the G-buffer; 12 algorithmic bytes per pixel per pass"
+    out["note"] = ("view 0 at 1920x1080 with the G-buffer; algorithmic bytes per pixel: 12 per normal pass, "
+                   "64 for lighting (with AO and bloom inputs)")
     return out
 
 
-def serving_bench(rates=(100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 3000, 5000), duration_s=1.0,
-                  max_requests=1500, seed=3) -> dict:
+SERVING_RATES = (100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 3000, 5000)
+
+
+def serving_bench(rates=SERVING_RATES, duration_s=1.0, max_requests=1500, seed=3,
+                  modes=("batched", "per_request")) -> dict:
     """Serving workload: 32 resident scenes with sizes log-uniform in [50k, 4M] splats, SH0 or SH3
     by coin, each generated as the 1M scene is (its own seed: 64 clusters + 6 sheets), 256
     requests of random scene / orbit pose / resolution in {640x480, 1280x720, 1920x1080},
     submitted as a Poisson stream at each rate.  The batching server (64 views per launch, 2
     slots, graph replay, 200 GB budget) is compared with the plain per-request drop-in (one frame
-    per launch, one slot)."""
+    per launch, one slot).  Both modes get the same 256 requests."""
     import math
     from concurrent.futures import ThreadPoolExecutor
 
@@ -466,19 +492,21 @@ def serving_bench(rates=(100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 
         scene_list = list(ex.map(lambda i: scenes.scene_config1(int(sizes[i]), seed=1000 + i,
                                                                 sh_degree=int(degs[i])), range(32)))
     res = [(640, 480), (1280, 720), (1920, 1080)]
+    requests = []
+    for k in range(256):
+        sid = int(rng.integers(0, 32))
+        w, h = res[int(rng.integers(0, 3))]
+        requests.append((sid, Camera(orbit_eye(float(rng.uniform(2.5, 3.5)), float(rng.uniform(0, 2 * math.pi)),
+                                               float(rng.uniform(-0.3, 0.6))), (0, 0, 0), 1500.0 * h / 1080,
+                                     1500.0 * h / 1080, w, h)))
     out = {"scenes": 32, "total_splats": int(sizes.sum()), "rates_hz": list(rates), "duration_s": duration_s}
-    for mode, kw in (("batched", dict(max_views_per_batch=64, slots=2, use_graphs=True)),
-                     ("per_request", dict(max_views_per_batch=1, slots=1, use_graphs=False))):
-        srv = Server(hbm_budget_bytes=200 << 30, **kw)
+    all_modes = {"batched": dict(max_views_per_batch=64, slots=2, use_graphs=True),
+                 "per_request": dict(max_views_per_batch=1, slots=1, use_graphs=False)}
+    for mode in modes:
+        srv = Server(hbm_budget_bytes=200 << 30, **all_modes[mode])
         for i, s in enumerate(scene_list):
             srv.add_scene(i, s)
-        pool = []
-        for k in range(256):
-            sid = int(rng.integers(0, 32))
-            w, h = res[int(rng.integers(0, 3))]
-            cam = Camera(orbit_eye(float(rng.uniform(2.5, 3.5)), float(rng.uniform(0, 2 * math.pi)),
-                                   float(rng.uniform(-0.3, 0.6))), (0, 0, 0), 1500.0 * h / 1080, 1500.0 * h / 1080, w, h)
-            pool.append((sid, w, h, srv.uniforms(sid, cam)))
+        pool = [(sid, cam.width, cam.height, srv.uniforms(sid, cam)) for sid, cam in requests]
         # admit every scene and settle capacities before measuring
         poisson_run(srv, [(i, 640, 480, srv.uniforms(i, Camera(orbit_eye(3.0, 0.0, 0.2), (0, 0, 0), 700, 700, 640,
                                                                      480))) for i in range(32)],
@@ -492,6 +520,8 @@ def serving_bench(rates=(100, 200, 400, 600, 800, 1000, 1200, 1600, 2000, 2400, 
                          "frames": st["frames"], "graph_replays": st["graph_replays"]})
         out[mode] = runs
         srv.close()
+    if set(modes) != set(all_modes):
+        return out
     # sustained frames/s at matched p99: the best rate of each mode whose p99 stays under the bound
     bound = max(min(r["p99_ms"] for r in out["per_request"]) * 2.0, 50.0)
     best = {m: max([r["fps"] for r in out[m] if r["p99_ms"] <= bound] or [0.0]) for m in ("batched", "per_request")}
