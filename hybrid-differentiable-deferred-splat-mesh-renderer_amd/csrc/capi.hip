@@ -165,8 +165,8 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out) {
 int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   if (!plan || !out) return HDSR_E_INVALID_ARG;
   *out = nullptr;
-  if (flags & HDSR_REF_CAPS) return HDSR_E_UNSUPPORTED;
-  if ((flags & HDSR_DIFFERENTIABLE) && (flags & (HDSR_GBUFFER | HDSR_F16_COMPOSITE))) return HDSR_E_UNSUPPORTED;
+  if ((flags & HDSR_DIFFERENTIABLE) && (flags & (HDSR_GBUFFER | HDSR_F16_COMPOSITE | HDSR_REF_CAPS)))
+    return HDSR_E_UNSUPPORTED;
   if (plan->tile_size_px != HDSR_TILE_PX) return HDSR_E_INVALID_ARG;
   if (plan->tile_columns == 0 || plan->tile_rows == 0 || plan->tile_columns > 255 || plan->tile_rows > 255)
     return HDSR_E_INVALID_ARG;  // u8 tile-box packing (gpu_project_splats.wgsl:229-231)
@@ -194,6 +194,7 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   c->eng.set_f16((flags & HDSR_F16_COMPOSITE) != 0);
   const uint64_t base_cap = std::max<uint64_t>((uint64_t)plan->splat_count * 16u, 1u << 16);
   c->eng.set_min_capacity(base_cap);
+  if (flags & HDSR_REF_CAPS) c->eng.set_ref_caps(plan->max_total_tile_refs);
   c->eng.set_profile((flags & HDSR_PROFILE) != 0);
   *out = c;
   return HDSR_OK;
@@ -702,7 +703,10 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   if (rc) return rc;
   memset(o, 0, sizeof(*o));
   const DevStats& s = c->eng.host_stats();
-  o->refs = s.refs;
+  // the frame's references (the device count is clamped to the pair capacity, which only the
+  // reference's caps leave below them once the frame is settled)
+  const uint64_t refs = (uint64_t)s.small_pairs + s.large_pairs;
+  o->refs = refs > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)refs;
   o->small_pairs = s.small_pairs;
   o->large_pairs = s.large_pairs;
   o->large_splats = s.large_splats;
@@ -713,7 +717,7 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->segments = s.segments;
   o->pair_capacity = c->eng.bufs().cap;
   o->overflow_reruns = c->reruns;
-  o->cap_refs_tripped = s.refs > c->plan.max_total_tile_refs;
+  o->cap_refs_tripped = refs > c->plan.max_total_tile_refs;
   o->cap_large_splats_tripped = s.large_splats > 65535u;
   o->cap_chunks_tripped = s.chunks > 1024u;
   o->composite_refs = s.composite_refs;
@@ -746,6 +750,8 @@ int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {
   if (!c->rendered) return fail(c, HDSR_E_STATE, "render a frame first");
   const DevStats& s = c->eng.host_stats();
   const uint64_t M = c->plan.morton_tile_count;
+  // the entry arrays hold at most the pair capacity (below the references with HDSR_REF_CAPS)
+  const uint64_t R = std::min<uint64_t>(s.refs, c->eng.bufs().cap);
   switch (which) {
     case HDSR_DBG_PROJCACHE:
       // the G-buffer-only words of the record are written by renders that need them
@@ -756,11 +762,11 @@ int hdsr_debug_size(hdsr_ctx* c, int which, uint64_t* bytes) {
     case HDSR_DBG_DEPTH: *bytes = (uint64_t)c->n * 4; break;
     case HDSR_DBG_TILE_COUNTS:
     case HDSR_DBG_TILE_OFFSETS: *bytes = M * 4; break;
-    case HDSR_DBG_ENTRIES: *bytes = (uint64_t)s.refs * 4; break;
+    case HDSR_DBG_ENTRIES: *bytes = R * 4; break;
     case HDSR_DBG_ENTRIES_CANON:
     case HDSR_DBG_KEYS20:
       if (!(c->flags & HDSR_DEBUG)) return fail(c, HDSR_E_STATE, "create the context with HDSR_DEBUG");
-      *bytes = (uint64_t)s.refs * 4;
+      *bytes = R * 4;
       break;
     case HDSR_DBG_SMALL_LIST: *bytes = (uint64_t)s.small_tiles * 4; break;
     case HDSR_DBG_LARGE_LIST:
@@ -818,7 +824,7 @@ int hdsr_debug_readback(hdsr_ctx* c, int which, void* host, uint64_t bytes) {
       for (const Seg& g : segs)
         if (g.large != 0xFFFFFFFFu) {
           o[k++] = g.dst;
-          o[k++] = g.count;
+          o[k++] = g.count & ~kSegKeepOrder;
         }
       return HDSR_OK;
     }

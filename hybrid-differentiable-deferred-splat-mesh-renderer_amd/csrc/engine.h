@@ -60,6 +60,13 @@ class Engine {
   // Grows the pair capacity to hold the last run's references (with headroom).
   int grow_for_overflow();
   void set_min_capacity(uint64_t cap) { min_cap_ = cap; }
+  // The reference's WebGPU caps (HDSR_REF_CAPS): the pair capacity is exactly maxTotalTileRefs,
+  // references past it are dropped rather than grown for, at most 65,535 large splats are stored
+  // and chunks past the first 1,024 keep their bucket order.  One frame per run.
+  void set_ref_caps(uint32_t max_total_tile_refs) {
+    ref_caps_ = true;
+    ref_cap_ = max_total_tile_refs;
+  }
   // Differentiable mode: keeps the fp32 record table and the per-splat gradient slots.
   void set_diff(bool on) { diff_ = on; }
   // Composite with binary16 transmittance / colour (the reference's f16 shader variant).
@@ -101,6 +108,8 @@ class Engine {
   Layout lay_;
   uint64_t min_cap_ = 1u << 20;
   uint64_t want_cap_ = 0;
+  bool ref_caps_ = false;
+  uint32_t ref_cap_ = 0;
   bool diff_ = false;
   bool f16_ = false;
   bool deferred_ = false;     // the last enqueued frame left chunks unsorted (deferred chunk sort)

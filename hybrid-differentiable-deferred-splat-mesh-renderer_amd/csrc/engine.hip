@@ -195,7 +195,7 @@ int Engine::ensure(const Layout& L, bool debug) {
   const size_t scan4 = scan_u4_tmp_words(L.splat_blocks) + 2 * (size_t)L.total_tiles + scan_u4_tmp_words(L.total_tiles);
   if ((rc = grow(&b_.scan_tmp, &n_scan4_, scan4))) return rc;
   // pair capacity
-  uint64_t cap = std::max<uint64_t>({min_cap_, want_cap_, 1});
+  uint64_t cap = ref_caps_ ? std::max<uint64_t>(ref_cap_, 1) : std::max<uint64_t>({min_cap_, want_cap_, 1});
   cap = std::min<uint64_t>(cap, 0xFFFFFFF0ull);
   if (cap > n_cap_ || !b_.entries) {
     have = n_cap_;
@@ -209,7 +209,7 @@ int Engine::ensure(const Layout& L, bool debug) {
     n_cap_ = cap;
     n_debug_ = 0;  // debug arrays follow the capacity
   }
-  b_.cap = (uint32_t)n_cap_;
+  b_.cap = ref_caps_ ? ref_cap_ : (uint32_t)n_cap_;  // under the caps: the drop threshold itself
   if (debug) {
     if (n_debug_ < n_cap_ || !b_.keys20) {
       have = n_debug_;
@@ -231,11 +231,13 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.tile_counts, &have, L.total_morton))) return rc;
     have = n_morton_;
     if ((rc = grow(&b_.tile_pend, &have, L.total_morton))) return rc;
-    have = n_morton_ * 2;
-    if ((rc = grow(&b_.tile_dmin, &have, (size_t)L.total_morton * 2))) return rc;
+    have = n_morton_ * 3;
+    if ((rc = grow(&b_.tile_dmin, &have, (size_t)L.total_morton * 3))) return rc;
     n_morton_ = L.total_morton;
   }
   b_.tile_dmax = b_.tile_dmin + n_morton_;
+  b_.tile_unl = ref_caps_ ? b_.tile_dmin + 2 * n_morton_ : nullptr;  // per-tile unlisted large splats
+  b_.ref_caps = ref_caps_ ? 1u : 0u;
   if (L.total_tiles > n_tiles_ || !b_.small_list) {
     have = n_tiles_;
     if ((rc = grow(&b_.small_list, &have, L.total_tiles))) return rc;
@@ -361,7 +363,7 @@ void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, boo
   if (ev) (void)hipEventRecord(ev[7], stream);
   // Deferred chunk sort when nothing reads the whole lists: no debug readback, no G-buffer (its
   // votes are not part of the saved pixel state), no backward pass.
-  deferred_ = !debug && !gbuffer && !b.drec;
+  deferred_ = !debug && !gbuffer && !b.drec && !b.ref_caps;
   last_deferred_ = deferred_;
   Bufs r1 = b, r2 = b;
   r1.round = deferred_ ? 1u : 0u;

@@ -136,9 +136,11 @@ struct DevStats {
 struct Seg {
   uint32_t src;    // first entry, in entries[] (small tile) or scratch[] (chunk)
   uint32_t dst;    // first entry in entries[]
-  uint32_t count;  // <= 4096
+  uint32_t count;  // <= 4096, | kSegKeepOrder for a chunk copied unsorted
   uint32_t large;  // index into the large-tile table, or 0xFFFFFFFF for a small tile
 };
+// Seg.count flag: with the reference's caps, a chunk past the first 1,024 keeps its bucket order.
+constexpr uint32_t kSegKeepOrder = 0x80000000u;
 
 struct LargeTile {
   uint32_t morton;   // global Morton slot
@@ -214,6 +216,8 @@ struct Bufs {
   // Round 2 sorts the other chunks of those tiles only and resumes them.  Round 3 sorts every chunk
   // after the first: it completes the lists of a deferred frame for a reader of the whole lists.
   uint32_t round;
+  uint32_t ref_caps;          // the reference's WebGPU caps enforced (HDSR_REF_CAPS; one frame)
+  uint32_t* tile_unl;         // [total_morton] with ref_caps: large splats past the first 65,535 per tile
   uint32_t* tile_pend;       // [total_morton] large tiles only: 1 = resumed in round 2
   float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)
   // The projection writes cold[] (read only by the G-buffer rasterizer and the 14-word debug

@@ -237,7 +237,7 @@ __global__ void k_totals(Bufs b) {
   s->visible = t.w;
   const uint64_t refs = (uint64_t)t.x + t.y;
   s->refs = refs > b.cap ? b.cap : (uint32_t)refs;
-  s->overflow = refs > b.cap ? 1u : 0u;
+  s->overflow = (!b.ref_caps && refs > b.cap) ? 1u : 0u;  // under the caps, dropping is the result
   const uint64_t cells = (uint64_t)c.x + c.y;  // <= kMaxCellsPerSplat per splat by construction
   s->cell_pairs = cells > b.cell_cap ? b.cell_cap : (uint32_t)cells;
 }
@@ -443,6 +443,7 @@ __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
     b.tile_counts[i] = 0u;
     b.tile_dmin[i] = 0xFFFFFFFFu;
     b.tile_dmax[i] = 0u;
+    if (b.tile_unl) b.tile_unl[i] = 0u;
   }
   if (!keys || c >= b.total_cells) return;
   const uint32_t n = b.stats->cell_pairs;
@@ -664,10 +665,14 @@ __global__ __launch_bounds__(64) void k_cell_prefix(Bufs b) {
 // Keeps every tile's range inside the pair capacity.  Only a frame whose references exceed the
 // capacity is affected (its lists are cut; the host grows the capacity and renders it again),
 // but every later stage then reads and writes inside the entry buffers.
-__global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, uint32_t cap) {
+//
+// With the reference's caps, the large splats past the first 65,535 (unl: per tile) are counted
+// for the offsets but never stored; they are the last entries of every tile's canonical list, so
+// the tile keeps the first count - unl entries, and those below the cap.
+__global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, uint32_t cap, const uint32_t* unl) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const uint64_t s = offsets[i], e = s + counts[i];
+  const uint64_t s = offsets[i], e = s + counts[i] - (unl ? unl[i] : 0u);
   const uint32_t cs = (uint32_t)(s < cap ? s : cap), ce = (uint32_t)(e < cap ? e : cap);
   offsets[i] = cs;
   counts[i] = ce - cs;
@@ -675,10 +680,10 @@ __global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, 
 
 // The tile-count scan and the clamp above in one workgroup (n <= kOneGroupScan).
 __global__ __launch_bounds__(1024) void k_tile_ranges_one(uint32_t* counts, uint32_t* offsets, uint32_t n,
-                                                          uint32_t cap) {
+                                                          uint32_t cap, const uint32_t* unl) {
   (void)one_group_scan<uint32_t, 16>(counts, offsets, n);  // ends with a barrier: offsets are written
   for (uint32_t i = threadIdx.x; i < n; i += 1024) {
-    const uint64_t st = offsets[i], e = st + counts[i];
+    const uint64_t st = offsets[i], e = st + counts[i] - (unl ? unl[i] : 0u);
     const uint32_t cs = (uint32_t)(st < cap ? st : cap), ce = (uint32_t)(e < cap ? e : cap);
     offsets[i] = cs;
     counts[i] = ce - cs;
@@ -756,6 +761,54 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   }
 }
 
+// ---- the reference's WebGPU caps (HDSR_REF_CAPS, one frame) ---------------------------------
+
+// Every large splat past the first 65,535 in rank order (the reference's list holds only those,
+// gpu_classify_large_splats.wgsl:52,62) adds one to each tile of its box: those references are
+// counted for the offsets but never stored.
+__global__ __launch_bounds__(256) void k_unlisted(Bufs b) {
+  const uint32_t nl = b.stats->large_splats;
+  const uint32_t mb = b.frames[0].morton_base;
+  for (uint32_t i = 65535u + blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += gridDim.x * blockDim.x) {
+    uint32_t x0, y0, x1, y1;
+    box_of(b.bounds[b.large_list[i]], &x0, &y0, &x1, &y1);
+    for (uint32_t ty = y0; ty <= y1; ++ty)
+      for (uint32_t tx = x0; tx <= x1; ++tx) atomicAdd(&b.tile_unl[mb + hdsr_morton2(tx, ty)], 1u);
+  }
+}
+
+// A tile's stored entries are a prefix of its canonical list: its depth range (the log frame of
+// its keys) is taken over them alone, and their keys are written again in that frame.
+__global__ __launch_bounds__(256) void k_refcaps_frames(Bufs b) {
+  __shared__ uint32_t s_mn[4], s_mx[4];
+  const uint32_t m = blockIdx.x, t = threadIdx.x;
+  const uint32_t s = b.tile_offsets[m], c = b.tile_counts[m];
+  if (!c) return;  // uniform
+  uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+  for (uint32_t i = t; i < c; i += 256) {
+    const uint32_t d = b.bdep[b.entries[s + i]].y;
+    mn = min(mn, d);
+    mx = max(mx, d);
+  }
+  mn = wave_min(mn);
+  mx = wave_max(mx);
+  if ((t & 63u) == 0) {
+    s_mn[t >> 6] = mn;
+    s_mx[t >> 6] = mx;
+  }
+  __syncthreads();
+  mn = min(min(s_mn[0], s_mn[1]), min(s_mn[2], s_mn[3]));
+  mx = max(max(s_mx[0], s_mx[1]), max(s_mx[2], s_mx[3]));
+  if (t == 0) {
+    b.tile_dmin[m] = mn;
+    b.tile_dmax[m] = mx;
+  }
+  float lm, sc;
+  hdsr_log_frame(mn, mx, &lm, &sc);
+  for (uint32_t i = t; i < c; i += 256)
+    b.qkey[s + i] = hdsr_q20_of((hdsr_u2f(b.bln[b.entries[s + i]].y) - lm) * sc);
+}
+
 // ---- classify: small / large tile lists and the sort segments, in row-major tile order -----
 
 __device__ __forceinline__ void tile_info(const Bufs& b, uint32_t t, uint32_t* morton, uint32_t* start,
@@ -791,12 +844,15 @@ __device__ __forceinline__ void write_class(const Bufs& b, uint32_t m, uint32_t 
     b.segs[p.z] = Seg{s, s, c, 0xFFFFFFFFu};
   } else if (k.y) {
     b.large_list_t[p.y] = m;
-    b.large_bases[p.y] = b.stats->refs + p.w;
+    b.large_bases[p.y] = (b.ref_caps ? b.cap : b.stats->refs) + p.w;  // capacity + earlier large counts
     b.large_tiles[p.y] = LargeTile{m, s, c, p.w, 0u, 0u, 0u, 0u};
     for (uint32_t q = 0; q < k.z; ++q) {
       const uint32_t off = q * HDSR_MAX_TILE_ENTRIES;
       const uint32_t cnt = min(HDSR_MAX_TILE_ENTRIES, c - off);
-      b.segs[p.z + q] = Seg{p.w + off, s + off, cnt, p.y};
+      // with the caps, a chunk past the first 1,024 (p.z - p.x: the chunks of earlier large tiles)
+      // keeps its bucket order (gpu_tile_bucket_sort.wgsl:126,138)
+      const uint32_t keep = (b.ref_caps && p.z - p.x + q >= 1024u) ? kSegKeepOrder : 0u;
+      b.segs[p.z + q] = Seg{p.w + off, s + off, cnt | keep, p.y};
     }
   }
 }
@@ -870,20 +926,24 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
   scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
   hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
+  if (b.ref_caps && b.tile_unl)
+    hipLaunchKernelGGL(k_unlisted, dim3(1024), dim3(256), 0, stream, b);
 }
 
 void launch_tile_ranges(const Bufs& b, hipStream_t stream) {
   if (!b.total_morton) return;
   if (b.total_morton <= kOneGroupScan) {
     hipLaunchKernelGGL(k_tile_ranges_one, dim3(1), dim3(1024), 0, stream, b.tile_counts, b.tile_offsets,
-                       b.total_morton, b.cap);
+                       b.total_morton, b.cap, (const uint32_t*)b.tile_unl);
   } else {
     scan_u32_exclusive(b.tile_counts, b.tile_offsets, b.total_morton, b.radix_tmp, stream);
     hipLaunchKernelGGL(k_clamp_ranges, dim3((b.total_morton + 255) / 256), dim3(256), 0, stream, b.tile_offsets,
-                       b.tile_counts, b.total_morton, b.cap);
+                       b.tile_counts, b.total_morton, b.cap, (const uint32_t*)b.tile_unl);
   }
   if (has_pairs(b))
     hipLaunchKernelGGL(k_cell_write, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
+  if (b.ref_caps && has_pairs(b))  // the stored lists' own depth ranges and keys
+    hipLaunchKernelGGL(k_refcaps_frames, dim3(b.total_morton), dim3(256), 0, stream, b);
 }
 
 size_t radix_hist_words(uint32_t cap) {

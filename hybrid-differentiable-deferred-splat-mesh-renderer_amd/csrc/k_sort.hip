@@ -277,6 +277,7 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     }
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
+    if (S.count & kSegKeepOrder) continue;  // uniform: copied unsorted by k_segcopy
     const uint32_t count = S.count;
     uint32_t vv[kSortPer], qq[kSortPer];
     uint32_t qmin = 0xFFFFFFFFu, qmax = 0;
@@ -354,6 +355,21 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
   }
 }
 
+// With the reference's caps: the chunks past the first 1,024 keep their bucket order; they are
+// copied from the scratch to their place in the entry list, each key holding its position.
+__global__ __launch_bounds__(256) void k_segcopy(Bufs b) {
+  const uint32_t nseg = b.stats->segments;
+  for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
+    const Seg S = b.segs[s];
+    if (!(S.count & kSegKeepOrder)) continue;  // uniform
+    const uint32_t count = S.count & ~kSegKeepOrder;
+    for (uint32_t i = threadIdx.x; i < count; i += 256) {
+      b.entries[S.dst + i] = b.scratch[S.src + i];
+      if (b.keys20) b.keys20[S.dst + i] = (b.qkey_s[S.src + i] << 12) | i;
+    }
+  }
+}
+
 }  // namespace
 
 void launch_bucket(const Bufs& b, hipStream_t stream) {
@@ -372,6 +388,7 @@ void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   const uint32_t cap = 2048u * 1024u / kSortThreads;
   const uint32_t grid = b.seg_cap < cap ? b.seg_cap : cap;
   hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
+  if (b.ref_caps) hipLaunchKernelGGL(k_segcopy, dim3(grid), dim3(256), 0, stream, b);
 }
 
 }  // namespace hdsr

@@ -59,7 +59,15 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
 #define HDSR_F16_COMPOSITE 1u   /* options.f16: composite with binary16 transmittance / colour
                                    (gpu_tile_splat_composite_f16.wgsl); not combinable with
                                    HDSR_DIFFERENTIABLE */
-#define HDSR_REF_CAPS 2u        /* emulate WebGPU caps: not provided (returns HDSR_E_UNSUPPORTED) */
+#define HDSR_REF_CAPS 2u        /* the reference's WebGPU caps instead of lifted capacities:
+                                   references past plan.max_total_tile_refs are dropped
+                                   (gpu_tile_splat_composite.wgsl:114), only the first 65,535
+                                   large splats are scattered (gpu_classify_large_splats.wgsl:52,62;
+                                   the rest keep their slots unwritten) and chunks past the first
+                                   1,024 keep their bucket order (gpu_tile_bucket_sort.wgsl:126,138).
+                                   Each tile keeps the entries the reference stores; where the
+                                   reference would read slots it never wrote, nothing is read.
+                                   No capacity re-run; not combinable with HDSR_DIFFERENTIABLE */
 #define HDSR_GBUFFER 4u         /* also write the oct-normal and material/emissive/confidence words */
 #define HDSR_DEBUG 8u           /* keep the pre-sort entry list and the 20-bit keys for readback */
 #define HDSR_OUT_RGBA16 16u     /* also write binary16 RGBA (the reference's rgba16float texture) */

@@ -485,6 +485,7 @@ struct Frame {
   std::vector<double> rgba64, depth64;
   std::vector<float> diffrec;  // differentiable mode: (opacity, r, g, b) per rank, no f16 rounding
   Splats S;
+  bool ref_caps = false;  // the reference's WebGPU caps enforced (see bin_and_sort)
 };
 
 enum {
@@ -512,7 +513,7 @@ uint32_t span_of(uint32_t b, uint32_t* x0, uint32_t* y0, uint32_t* x1, uint32_t*
 // Per-tile sort with 20-bit log-depth keys over list[0..count) (in place).  log_min/scale are
 // the frame of the whole tile.  Writes the keys at their final positions.
 void sort_segment(uint32_t* list, uint32_t* keys_out, uint32_t count, const uint32_t* depth, float log_min,
-                  float scale) {
+                  float scale, bool sort = true) {
   std::vector<uint32_t> key(count), val(list, list + count);
   for (uint32_t i = 0; i < count; ++i) {
     const float y = (hdsr_logf(std::fmax(u2f(depth[val[i]]), 1e-6f)) - log_min) * scale;
@@ -522,7 +523,7 @@ void sort_segment(uint32_t* list, uint32_t* keys_out, uint32_t count, const uint
   }
   std::vector<uint32_t> idx(count);
   for (uint32_t i = 0; i < count; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+  if (sort) std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
   for (uint32_t i = 0; i < count; ++i) {
     list[i] = val[idx[i]];
     keys_out[i] = key[idx[i]];
@@ -589,6 +590,30 @@ void bin_and_sort(Frame& F) {
         F.entries_canon[F.tile_offsets[t] + cursor[t]++] = r;
       }
   }
+  // With the reference's caps (ref_caps): the large-splat list holds the first 65,535 large splats
+  // (gpu_classify_large_splats.wgsl:52,62; the rest are counted but never scattered, and they are
+  // the last entries of every tile's canonical list), and an entry whose linear index reaches
+  // maxTotalTileRefs (uniform word 23) is not stored (gpu_tile_splat_composite.wgsl:114,
+  // gpu_large_splat_scatter.wgsl:85).  A tile keeps its stored entries: its range becomes
+  // [min(offset, cap), +min(count - unlisted, max(cap - offset, 0))).  (The reference's later
+  // passes would read the slots it never wrote; their contents are undefined, and none are read
+  // here.)  Everything below works on the stored ranges.
+  if (F.ref_caps) {
+    const uint32_t cap = F.U.total_refs;
+    std::vector<uint32_t> unl(M, 0);
+    for (size_t i = 65535; i < F.large_list.size(); ++i) {
+      uint32_t x0, y0, x1, y1;
+      span_of(F.cache[F.large_list[i] * 14 + 7], &x0, &y0, &x1, &y1);
+      for (uint32_t ty = y0; ty <= y1; ++ty)
+        for (uint32_t tx = x0; tx <= x1; ++tx) unl[hdsr_morton2(tx, ty)]++;
+    }
+    for (uint32_t m = 0; m < M; ++m) {
+      const uint64_t st = F.tile_offsets[m], e = st + (F.tile_counts[m] - unl[m]);
+      const uint32_t cs = (uint32_t)std::min<uint64_t>(st, cap), ce = (uint32_t)std::min<uint64_t>(e, cap);
+      F.tile_offsets[m] = cs;
+      F.tile_counts[m] = ce - cs;
+    }
+  }
   // classify tiles in row-major order; overflow base = capacity + earlier large counts
   F.small_tiles.clear();
   F.large_tiles.clear();
@@ -602,7 +627,7 @@ void bin_and_sort(Frame& F) {
       F.small_tiles.push_back(m);
     } else {
       F.large_tiles.push_back(m);
-      F.large_bases.push_back((uint32_t)(R + overflow));
+      F.large_bases.push_back((uint32_t)((F.ref_caps ? F.U.total_refs : R) + overflow));
       overflow += c;
     }
   }
@@ -619,11 +644,15 @@ void bin_and_sort(Frame& F) {
   });
   // large tiles: 128 log-depth buckets (stable), then consecutive 4096-entry chunks, each sorted
   F.chunks.clear();
-  for (uint32_t m : F.large_tiles)
+  std::vector<uint32_t> chunk0(F.large_tiles.size(), 0);  // global index of each large tile's first chunk
+  for (size_t li = 0; li < F.large_tiles.size(); ++li) {
+    chunk0[li] = (uint32_t)(F.chunks.size() / 2);
+    const uint32_t m = F.large_tiles[li];
     for (uint32_t cs = 0; cs < F.tile_counts[m]; cs += HDSR_MAX_TILE_ENTRIES) {
       F.chunks.push_back(F.tile_offsets[m] + cs);
       F.chunks.push_back(std::min(HDSR_MAX_TILE_ENTRIES, F.tile_counts[m] - cs));
     }
+  }
   par_for((uint32_t)F.large_tiles.size(), [&](uint32_t li) {
     const uint32_t m = F.large_tiles[li];
     const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
@@ -643,7 +672,11 @@ void bin_and_sort(Frame& F) {
     std::copy(ordered.begin(), ordered.end(), F.entries.begin() + s);
     for (uint32_t cs = 0; cs < c; cs += HDSR_MAX_TILE_ENTRIES) {
       const uint32_t cc = std::min(HDSR_MAX_TILE_ENTRIES, c - cs);
-      sort_segment(&F.entries[s + cs], &F.keys[s + cs], cc, F.depth.data(), lmin, scale);
+      // with the reference's caps a chunk past the first 1,024 is never sorted: it keeps its bucket
+      // order (gpu_tile_bucket_sort.wgsl:126,138; gpu_tile_chunk_sort.wgsl:42); its keys are
+      // written at the positions it keeps
+      const bool sorted = !F.ref_caps || chunk0[li] + cs / HDSR_MAX_TILE_ENTRIES < 1024u;
+      sort_segment(&F.entries[s + cs], &F.keys[s + cs], cc, F.depth.data(), lmin, scale, sorted);
     }
   });
   // diagnostic: inversions of the final order against the live reference's raw depth-bit key
@@ -661,8 +694,9 @@ void bin_and_sort(Frame& F) {
   F.stats[ST_CHUNKS] = (uint32_t)(F.chunks.size() / 2);
   F.stats[ST_LARGE_SPLATS] = (uint32_t)F.large_list.size();
   F.stats[ST_VISIBLE] = visible;
-  // plan: maxTotalTileRefs = min(ceil(16 N), 65535*256) (gpuTileSplatCompositor.ts:127-137)
-  const uint64_t cap = std::min<uint64_t>((uint64_t)n * 16u, 65535ull * 256ull);
+  // plan: maxTotalTileRefs = min(ceil(16 N), 65535*256) (gpuTileSplatCompositor.ts:127-137); with the
+  // caps enforced, the frame's own (uniform word 23)
+  const uint64_t cap = F.ref_caps ? F.U.total_refs : std::min<uint64_t>((uint64_t)n * 16u, 65535ull * 256ull);
   F.stats[ST_REF_CAP] = (uint32_t)cap;
   F.stats[ST_CAP_REFS_TRIPPED] = R > cap;
   F.stats[ST_CAP_LARGE_SPLATS_TRIPPED] = F.large_list.size() > 65535u;
@@ -921,6 +955,7 @@ enum {
 #define ORC_FLAG_DIFF 16u  // differentiable mode: logit opacities, unquantized rasterizer inputs
 #define ORC_FLAG_NO_COMPOSITE 4u
 #define ORC_FLAG_F16 32u   // the f16 composite variant
+#define ORC_FLAG_REF_CAPS 64u  // the reference's WebGPU caps enforced (bin_and_sort)
 
 static double now_s() {
   timespec ts;
@@ -976,6 +1011,7 @@ void* orc_run(const orc_splats* s, const void* uniforms256, uint32_t width, uint
   F->tiles_y = F->U.gy;
   F->morton_count = (F->tiles_x && F->tiles_y) ? hdsr_morton2(F->tiles_x - 1, F->tiles_y - 1) + 1 : 0;
   F->S.logits = (flags & ORC_FLAG_DIFF) != 0;
+  F->ref_caps = (flags & ORC_FLAG_REF_CAPS) != 0;
   F->cache.assign((size_t)F->n * 14, 0);
   F->depth.assign(F->n, 0);
   if (flags & ORC_FLAG_DIFF) F->diffrec.assign((size_t)F->n * 4, 0.0f);

@@ -24,6 +24,7 @@ STATS = ["refs", "small_tiles", "large_tiles", "chunks", "large_splats", "visibl
          "cap_large_splats_tripped", "cap_chunks_tripped", "ref_cap"]
 FLAG_GBUFFER, FLAG_FP64, FLAG_NO_COMPOSITE, FLAG_FP64_NOQUANT, FLAG_DIFF = 1, 2, 4, 8, 16
 FLAG_F16 = 32  # composite with binary16 transmittance / colour (the f16 shader variant)
+FLAG_REF_CAPS = 64  # the reference's WebGPU caps enforced (65,535 large splats, maxTotalTileRefs, 1,024 chunks)
 
 
 class OrcSplats(C.Structure):
@@ -88,6 +89,11 @@ def load() -> C.CDLL:
     lib.orc_morton2.argtypes = [C.c_uint32, C.c_uint32]
     _lib = lib
     return lib
+
+
+def morton2(x: int, y: int) -> int:
+    """mortonEncode2D (the Morton slot of tile (x, y))."""
+    return int(load().orc_morton2(int(x), int(y)))
 
 
 def mat4_inverse(m16):

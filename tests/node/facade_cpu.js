@@ -16,6 +16,7 @@ var proj = new Float32Array(16); proj[0] = 1.640625; proj[5] = 2.1875;
 C.writeTileSplatFrameUniforms(t, vp, plan, 1.0, 3, [1, 2, 3], view, proj);
 out.uniforms = Array.prototype.slice.call(new Uint32Array(t.buffer));
 out.uniformBytes = C.TILE_SPLAT_FRAME_UNIFORM_BYTES;
-try { C.createTileSplatCompositor(null, plan, { refCaps: true }); out.refCaps = 'accepted'; } catch (e) { out.refCaps = String(e.message); }
+var bad = Object.assign({}, plan, { tileSizePx: 8 });  // rejected before any GPU work
+try { C.createTileSplatCompositor(null, bad, {}); out.badTile = 'accepted'; } catch (e) { out.badTile = String(e.message); }
 out.version = C.version();
 console.log(JSON.stringify(out));

@@ -74,7 +74,7 @@ def test_create_rejects_unsupported_options_without_gpu_work():
     p = compositor.planTileSplatCompositor({"viewportWidth": 64, "viewportHeight": 64, "tileSizePx": 16,
                                             "splatCount": 1})
     h = C.c_void_p()
-    for flags in (_lib.HDSR_REF_CAPS, _lib.HDSR_DIFFERENTIABLE | _lib.HDSR_F16_COMPOSITE,
+    for flags in (_lib.HDSR_DIFFERENTIABLE | _lib.HDSR_REF_CAPS, _lib.HDSR_DIFFERENTIABLE | _lib.HDSR_F16_COMPOSITE,
                   _lib.HDSR_DIFFERENTIABLE | _lib.HDSR_GBUFFER):
         rc = _lib.load().hdsr_create(C.byref(compositor._plan_struct(p)), flags, C.byref(h))
         assert rc == _lib.HDSR_E_UNSUPPORTED and not h.value

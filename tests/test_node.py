@@ -47,7 +47,7 @@ def test_facade_cpu():
     expect = camera.write_frame_uniforms(vp, plan, 1.0, 3, [1, 2, 3], view, proj).view(np.uint32)
     assert out["uniforms"] == expect.tolist()
     assert out["uniformBytes"] == 256
-    assert "(-6)" in out["refCaps"]
+    assert "(-1)" in out["badTile"]
     assert out["version"].startswith("hdsr")
 
 
