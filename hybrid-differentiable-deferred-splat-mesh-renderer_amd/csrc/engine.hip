@@ -86,7 +86,8 @@ void Engine::release() {
                   b_.cvals_b, b_.cell_offsets, b_.chunk_cnt, b_.entries, b_.keys20, b_.entries_canon,
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
-                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, b_.tile_pend, b_.px_state, d_frames_};
+                  b_.large_bases, b_.large_tiles, b_.segs, b_.stats, b_.tile_large, b_.pend_tiles, b_.pend_segs, b_.px_state,
+                  d_frames_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b_ = Bufs{};
@@ -102,6 +103,7 @@ void Engine::release() {
   hstats_ = nullptr;
   if (done_ev_) (void)hipEventDestroy(done_ev_);
   done_ev_ = nullptr;
+
   for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
   prof_ev_.clear();
   prof_head_ = prof_count_ = 0;
@@ -230,7 +232,7 @@ int Engine::ensure(const Layout& L, bool debug) {
     have = n_morton_;
     if ((rc = grow(&b_.tile_counts, &have, L.total_morton))) return rc;
     have = n_morton_;
-    if ((rc = grow(&b_.tile_pend, &have, L.total_morton))) return rc;
+    if ((rc = grow(&b_.tile_large, &have, L.total_morton))) return rc;
     have = n_morton_ * 3;
     if ((rc = grow(&b_.tile_dmin, &have, (size_t)L.total_morton * 3))) return rc;
     n_morton_ = L.total_morton;
@@ -247,13 +249,17 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.large_bases, &have, L.total_tiles))) return rc;
     have = n_tiles_;
     if ((rc = grow(&b_.large_tiles, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.pend_tiles, &have, L.total_tiles))) return rc;
     have = n_tiles_ * 256 * 6;
     if ((rc = grow(&b_.px_state, &have, (size_t)L.total_tiles * 256 * 6))) return rc;
     n_tiles_ = L.total_tiles;
   }
   // segments: every small tile, plus at most one 4096-entry chunk per 4096 pairs and per large tile
   const size_t segs = 2 * (size_t)L.total_tiles + (n_cap_ + HDSR_MAX_TILE_ENTRIES - 1) / HDSR_MAX_TILE_ENTRIES + 1;
+  size_t have_pend = n_seg_;  // pend_segs[] has the capacity of segs[]
   if ((rc = grow(&b_.segs, &n_seg_, segs))) return rc;
+  if ((rc = grow(&b_.pend_segs, &have_pend, segs))) return rc;
   b_.seg_cap = (uint32_t)n_seg_;
   if (!b_.stats) {
     size_t h = 0;
@@ -280,6 +286,7 @@ int Engine::ensure(const Layout& L, bool debug) {
     err = "hipEventCreate failed";
     return HDSR_E_HIP;
   }
+
   const size_t sb = sizeof(FrameDesc) * L.num_frames + 12 * (L.num_frames + 1);
   if (sb > stage_bytes_) {
     for (int i = 0; i < kRing; ++i) {
@@ -556,10 +563,12 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
     return HDSR_E_STATE;
   }
   if (deferred_) {
-    // the lists as the last render left them: the rounds meet the same saturation again
+    // the lists as the last render left them: the rounds meet the same saturation again, and
+    // round 1 lists the same tiles for round 2 (whose chunks that render sorted)
     Bufs r1 = b_, r2 = b_;
     r1.round = 1;
     r2.round = 2;
+    launch_reset_pending(b_, stream);
     launch_composite(r1, stream, gbuffer, f16_);
     launch_composite(r2, stream, gbuffer, f16_);
   } else {

@@ -131,6 +131,8 @@ struct DevStats {
   uint32_t cell_pairs;          // (cell, rank) pairs
   uint32_t cell_chunks;         // expansion workgroups needed (sum over cells of ceil(count / kCellChunk))
   uint32_t resumed_refs;        // deferred frames: entries after the first chunk of the resumed tiles
+  uint32_t pend_tiles;          // deferred frames: tiles the first round left unsaturated (pend_tiles[])
+  uint32_t pend_segs;           // deferred frames: their chunks after the first (pend_segs[])
 };
 
 struct Seg {
@@ -147,7 +149,8 @@ struct LargeTile {
   uint32_t start;    // first entry in entries[]
   uint32_t count;
   uint32_t scratch;  // first slot in scratch[]
-  uint32_t pad0, pad1, pad2, pad3;
+  uint32_t seg;      // the segment of its first chunk (the others follow it)
+  uint32_t pad1, pad2, pad3;
 };
 
 // Device buffers of one batch (all owned by the host context).
@@ -212,13 +215,16 @@ struct Bufs {
   uint32_t bucket_wide;      // large tiles average > 16k entries: one 1024-thread workgroup per tile (host-set)
   // Deferred chunk sort (host-set round; 0 = the whole frame in one pass).  Round 1 sorts the small
   // tiles and the first chunk of every large tile, and rasterizes each large tile up to the end of
-  // that chunk; a tile left unsaturated there gets tile_pend[morton] = 1 and its pixel state saved.
-  // Round 2 sorts the other chunks of those tiles only and resumes them.  Round 3 sorts every chunk
+  // that chunk; a tile left unsaturated there saves its pixel state and appends itself to
+  // pend_tiles[] and its other chunks to pend_segs[].  Round 2 sorts those chunks and resumes
+  // those tiles, walking the two lists only.  Round 3 sorts every chunk
   // after the first: it completes the lists of a deferred frame for a reader of the whole lists.
   uint32_t round;
   uint32_t ref_caps;          // the reference's WebGPU caps enforced (HDSR_REF_CAPS; one frame)
   uint32_t* tile_unl;         // [total_morton] with ref_caps: large splats past the first 65,535 per tile
-  uint32_t* tile_pend;       // [total_morton] large tiles only: 1 = resumed in round 2
+  uint32_t* tile_large;      // [total_morton] large tiles only: index into large_tiles[]
+  uint32_t* pend_tiles;      // [total_tiles] deferred frames: global tile indices resumed in round 2
+  uint32_t* pend_segs;       // [seg capacity] deferred frames: the segments round 2 sorts
   float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)
   // The projection writes cold[] (read only by the G-buffer rasterizer and the 14-word debug
   // readback) only when set.
@@ -260,6 +266,7 @@ void launch_classify(const Bufs& b, hipStream_t stream);
 void launch_bucket(const Bufs& b, hipStream_t stream);
 void launch_segment_sort(const Bufs& b, hipStream_t stream);
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16);
+void launch_reset_pending(const Bufs& b, hipStream_t stream);  // empties the round-2 lists (re-run round 1)
 // Differentiable mode (one frame, the batch's frame 0, with n splats): backward of the
 // rasterizer into gacc, then of the projection into the parameter gradients, which are
 // accumulated (added to) and indexed by splat id.

@@ -845,7 +845,8 @@ __device__ __forceinline__ void write_class(const Bufs& b, uint32_t m, uint32_t 
   } else if (k.y) {
     b.large_list_t[p.y] = m;
     b.large_bases[p.y] = (b.ref_caps ? b.cap : b.stats->refs) + p.w;  // capacity + earlier large counts
-    b.large_tiles[p.y] = LargeTile{m, s, c, p.w, 0u, 0u, 0u, 0u};
+    b.large_tiles[p.y] = LargeTile{m, s, c, p.w, p.z, 0u, 0u, 0u};
+    b.tile_large[m] = p.y;
     for (uint32_t q = 0; q < k.z; ++q) {
       const uint32_t off = q * HDSR_MAX_TILE_ENTRIES;
       const uint32_t cnt = min(HDSR_MAX_TILE_ENTRIES, c - off);

@@ -75,6 +75,17 @@ constexpr float kTCut16 = 1049.0f / 1048576.0f;     // f16(0.001)
 constexpr float kBgRG16 = 1311.0f / 65536.0f;       // f16(0.02)
 constexpr float kBgB16 = 1311.0f / 32768.0f;        // f16(0.04)
 
+// Loads through global-address-space pointers (global_load, counted by vmcnt only; a flat load
+// would also hold every lgkmcnt wait that precedes a barrier).
+typedef unsigned int gu32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint4 gload4(const uint4* p) {
+  const gu32x4 v = *(const __attribute__((address_space(1))) gu32x4*)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gload(const uint32_t* p) {
+  return *(const __attribute__((address_space(1))) uint32_t*)(p);
+}
+
 // The float4 record at byte offset off of a staged array.
 __device__ __forceinline__ float4 rec16(const float4* a, uint32_t off) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a) + off);
@@ -239,20 +250,34 @@ __device__ __forceinline__ void store_px(const FrameDesc& F, uint32_t X, uint32_
 // with rounding margins) against the four quadrants and is appended, in list order, to the index
 // list of each quadrant it may reach; a wave then walks only its own list, so entries that cannot
 // touch its pixels cost it nothing.
-template <bool kG, bool kD, bool kH>
-__global__ __launch_bounds__(256) void k_composite(Bufs b) {
-  __shared__ float4 s_a[kBatch];  // cx, cy, a, b
-  __shared__ float4 s_b[kBatch];  // opacity, r, g, c (the blend's three fields 16-byte aligned)
-  __shared__ float4 s_c[kBatch];  // b, depth (float4: the three records share one slot index)
-  __shared__ float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
-  __shared__ float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
-  __shared__ float s_gc[kG ? kBatch : 1];   // emissive b
-  __shared__ uint32_t s_idx[kD ? kBatch : 1];  // entry index of each compacted slot
-  __shared__ uint16_t s_list[4][kBatch];       // per quadrant: 16 x the slots that may reach it, in order
-  __shared__ uint32_t s_cnt[5][4];             // per staging wave: kept entries, then per quadrant
-  __shared__ int s_done[4];
+template <bool kG, bool kD>
+struct TileLds {
+  float4 s_a[kBatch];  // cx, cy, a, b
+  float4 s_b[kBatch];  // opacity, r, g, c (the blend's three fields 16-byte aligned)
+  float4 s_c[kBatch];  // b, depth (float4: the three records share one slot index)
+  float4 s_ga[kG ? kBatch : 1];  // roughness, metalness, confidence, emissive r
+  float4 s_gb[kG ? kBatch : 1];  // normal xyz, emissive g
+  float s_gc[kG ? kBatch : 1];   // emissive b
+  uint32_t s_idx[kD ? kBatch : 1];  // entry index of each compacted slot
+  uint16_t s_list[4][kBatch];       // per quadrant: 16 x the slots that may reach it, in order
+  uint32_t s_cnt[5][4];             // per staging wave: kept entries, then per quadrant
+  int s_done[4];
+};
 
-  const uint32_t t = blockIdx.x;
+// One tile (global index t, uniform) with the workgroup's shared arrays L; the records of kDepth
+// batches ahead are in flight (1 where many tiles share a CU, more where a tile has it alone).
+template <bool kG, bool kD, bool kH, uint32_t kDepth>
+__device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, TileLds<kG, kD>& L) {
+  float4* const s_a = L.s_a;
+  float4* const s_b = L.s_b;
+  float4* const s_c = L.s_c;
+  float4* const s_ga = L.s_ga;
+  float4* const s_gb = L.s_gb;
+  float* const s_gc = L.s_gc;
+  uint32_t* const s_idx = L.s_idx;
+  uint16_t(*const s_list)[kBatch] = L.s_list;
+  uint32_t(*const s_cnt)[4] = L.s_cnt;
+  int* const s_done = L.s_done;
   const uint32_t f = frame_of(b.tile_frame_start, b.num_frames, t);
   const FrameDesc& F = b.frames[f];
   const uint32_t lt = t - F.tile_base;
@@ -262,12 +287,14 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   const uint32_t full = b.tile_counts[m];
   const uint32_t tid = threadIdx.x;
   // Deferred chunk sort: round 1 stops a large tile at the end of its first chunk (the only one
-  // sorted yet); round 2 resumes the tiles left unsaturated there from their saved state.  The
-  // batches are those of one pass (4096 is a multiple of kBatch), so the pixels are the same.
-  const bool deferrable = b.round != 0 && full > HDSR_MAX_TILE_ENTRIES;  // uniform
+  // sorted yet); round 2 resumes the tiles left unsaturated there (the tiles round 1 listed)
+  // from their saved state.  The batches are those of one pass (4096 is a multiple of kBatch),
+  // so the pixels are the same.
+  // (never with the G-buffer or the differentiable records: those frames run in one pass)
+  const bool deferrable = !kG && !kD && b.round != 0 && full > HDSR_MAX_TILE_ENTRIES;  // uniform
   uint32_t count = full, base0 = 0;
   if (b.round == 2) {
-    if (!deferrable || !b.tile_pend[m]) return;
+    if (!deferrable) return;
     base0 = HDSR_MAX_TILE_ENTRIES;
   } else if (deferrable) {
     count = HDSR_MAX_TILE_ENTRIES;
@@ -292,126 +319,156 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   uint32_t loaded = 0;
 
   const float tile_x0 = (float)(tx * 16) + 0.5f, tile_y0 = (float)(ty * 16) + 0.5f;
-  // Two-stage prefetch: while batch k is blended, the records of batch k+1 and the entry
-  // indices of batch k+2 are in flight.
-  const uint4 z4 = make_uint4(0, 0, 0, 0);
-  const uint4* rec = kD ? reinterpret_cast<const uint4*>(b.drec) : b.hot;
+  // Prefetch: while batch k is blended, the records of batches k+1 .. k+kDepth and the entry
+  // indices of batches up to k+2*kDepth are in flight.  The batch loop is unrolled kDepth times:
+  // slot s holds the batches k = s (mod kDepth), so no register with a load in flight is moved.
+  // Past the end of the list a lane reads the list's last entry and gathers its record, which the
+  // staging ignores: every load stays unpredicated, so none waits for a select.
+  const uint4* const rec = kD ? reinterpret_cast<const uint4*>(b.drec) : b.hot;
+  const uint32_t* const ent = b.entries + start;
   constexpr uint32_t kRec = kD ? 3 : 2;  // 16-byte words per record
-  const uint32_t i0 = base0 + tid;
-  uint32_t g_cur = i0 < count ? b.entries[start + i0] : 0u;
-  uint4 h0n = i0 < count ? rec[(size_t)g_cur * kRec] : z4;
-  uint4 h1n = i0 < count ? rec[(size_t)g_cur * kRec + 1] : z4;
-  uint4 h2n = (kD && i0 < count) ? rec[(size_t)g_cur * kRec + 2] : z4;
-  uint32_t g_next = i0 + kBatch < count ? b.entries[start + i0 + kBatch] : 0u;
-  bool finished = false;  // every pixel saturated
-  for (uint32_t base = base0; base < count; base += kBatch) {
-    const uint32_t i = base + tid;
-    const uint4 h0 = h0n, h1 = h1n, h2 = h2n;
-    const uint32_t g = g_cur;
-    uint32_t qm = 0;  // quadrants the entry may reach
-    if (i < count)
-      qm = reach_quadrants(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
-                           tile_y0);
-    // compact the kept entries into slots, in order, and list each under its quadrants
-    const bool keep = qm != 0;
-    const unsigned long long km = __ballot(keep);
-    unsigned long long qb[4];
+  const uint4 z4 = make_uint4(0, 0, 0, 0);
+  const uint32_t last_e = count - 1u;  // read only when count > base0 >= 0
+  uint32_t gq[kDepth], ge[kDepth];  // entry index of slot s's batch, and of the batch kDepth later
+  uint4 h0q[kDepth], h1q[kDepth], h2q[kDepth];
 #pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) qb[k] = __ballot((qm >> k) & 1u);
-    if (lane == 0) {
-      s_cnt[0][wave] = (uint32_t)__popcll(km);
+  for (uint32_t s = 0; s < kDepth; ++s) {
+    gq[s] = ge[s] = 0u;
+    h0q[s] = h1q[s] = h2q[s] = z4;
+  }
+  if (count > base0) {  // uniform: an empty walk reads nothing
 #pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) s_cnt[1 + k][wave] = (uint32_t)__popcll(qb[k]);
+    for (uint32_t s = 0; s < kDepth; ++s) {
+      const uint32_t id = base0 + s * kBatch + tid;
+      gq[s] = gload(ent + min(id, last_e));
+      ge[s] = gload(ent + min(id + kDepth * kBatch, last_e));
     }
-    __syncthreads();
-    if (keep) {
-      uint32_t slot = (uint32_t)__popcll(km & below);
-      for (uint32_t w2 = 0; w2 < wave; ++w2) slot += s_cnt[0][w2];
-      s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
-      if (kD) {
-        s_b[slot] = make_float4(hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w), hdsr_u2f(h1.x) * -0.5f);
-        s_c[slot] = make_float4(hdsr_u2f(h2.x), hdsr_u2f(h2.y), 0.0f, 0.0f);
-        s_idx[slot] = i;
-      } else {
-        s_b[slot] = make_float4(h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16), hdsr_u2f(h1.x) * -0.5f);
-        s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), 0.0f, 0.0f);
-      }
-      if (kG) {
-        const uint4 c0 = b.cold[(size_t)g * 2];
-        const uint4 c1 = b.cold[(size_t)g * 2 + 1];
-        // octDecode (gpu_tile_splat_composite.wgsl:143-149)
-        const float ex = h16(c0.z), ey = h16(c0.z >> 16);
-        float dnx = ex, dny = ey, dnz = (1.0f - fabsf(ex)) - fabsf(ey);
-        if (dnz < 0.0f) {
-          const float sx = dnx >= 0.0f ? 1.0f : -1.0f, sy = dny >= 0.0f ? 1.0f : -1.0f;
-          const float ax = (1.0f - fabsf(dny)) * sx, ay = (1.0f - fabsf(dnx)) * sy;
-          dnx = ax;
-          dny = ay;
-        }
-        const float dl = sqrtf((dnx * dnx + dny * dny) + dnz * dnz);
-        s_ga[slot] = make_float4(h16(c0.x), h16(c0.x >> 16), h16(c0.w), h16(c1.x));
-        s_gb[slot] = make_float4(dnx / dl, dny / dl, dnz / dl, h16(c1.x >> 16));
-        s_gc[slot] = h16(c1.y);
-      }
 #pragma unroll
-      for (uint32_t k = 0; k < 4; ++k)
-        if ((qm >> k) & 1u) {
-          uint32_t pos = (uint32_t)__popcll(qb[k] & below);
-          for (uint32_t w2 = 0; w2 < wave; ++w2) pos += s_cnt[1 + k][w2];
-          s_list[k][pos] = (uint16_t)(slot * 16u);  // byte offset of the slot's float4 records
-        }
-    }
-    // uniform per wave: the walk below is a scalar loop
-    const uint32_t nq = __builtin_amdgcn_readfirstlane(s_cnt[1 + wave][0] + s_cnt[1 + wave][1] +
-                                                       s_cnt[1 + wave][2] + s_cnt[1 + wave][3]);
-    __syncthreads();
-    {
-      const uint32_t i1 = base + kBatch + tid, i2 = base + 2 * kBatch + tid;
-      g_cur = g_next;
-      h0n = i1 < count ? rec[(size_t)g_cur * kRec] : z4;
-      h1n = i1 < count ? rec[(size_t)g_cur * kRec + 1] : z4;
-      if (kD) h2n = i1 < count ? rec[(size_t)g_cur * kRec + 2] : z4;
-      g_next = i2 < count ? b.entries[start + i2] : 0u;
-    }
-    loaded += min(kBatch, count - base);
-    if (!__all(saturated(p))) {
-      // two entries per step: their exponentials are independent chains, blended in list order
-      const uint16_t* lst = s_list[wave];
-      for (uint32_t j = 0; j < nq; j += 2) {
-        if ((j & 30u) == 30u && __all(saturated(p))) break;  // wave saturated
-        const bool hb = j + 1u < nq;  // uniform
-        const uint32_t oa = lst[j], ob = hb ? lst[j + 1u] : oa;
-        const float4 Aa = rec16(s_a, oa), Ba = rec16(s_b, oa);
-        const float4 Ab = rec16(s_a, ob), Bb = rec16(s_b, ob);
-        const float dxa = (p00x - Aa.x) + add_x, dya = (p00y - Aa.y) + add_y;
-        const float dxb = (p00x - Ab.x) + add_x, dyb = (p00y - Ab.y) + add_y;
-        const float pwa = (Aa.z * dxa * dxa + Aa.w * dxa * dya) + Ba.w * dya * dya;
-        const float pwb = (Ab.z * dxb * dxb + Ab.w * dxb * dyb) + Bb.w * dyb * dyb;
-        const bool ra = __any(pwa > -4.0f), rb = hb && __any(pwb > -4.0f);  // uniform
-        if (!ra && !rb) continue;  // no pixel of the wave is reached by either
-        const float ala = alpha_of(pwa, Ba.x), alb = alpha_of(pwb, Bb.x);
-        if (ra) blend_slot<kG, kD>(p, q, pwa, ala, Ba, oa, s_c, s_ga, s_gb, s_gc, s_idx, &last);
-        if (rb) blend_slot<kG, kD>(p, q, pwb, alb, Bb, ob, s_c, s_ga, s_gb, s_gc, s_idx, &last);
-      }
-    }
-    const int wave_done = __all(saturated(p));
-    if (lane == 0) s_done[wave] = wave_done;
-    __syncthreads();
-    if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) {
-      finished = true;
-      break;
+    for (uint32_t s = 0; s < kDepth; ++s) {
+      h0q[s] = gload4(rec + (size_t)gq[s] * kRec);
+      h1q[s] = gload4(rec + (size_t)gq[s] * kRec + 1);
+      if (kD) h2q[s] = gload4(rec + (size_t)gq[s] * kRec + 2);
     }
   }
-  if (b.round == 1 && deferrable) {
-    if (tid == 0) b.tile_pend[m] = finished ? 0u : 1u;
-    if (!finished) {  // uniform: resumed in round 2
-      save_px(pst, p);
-      if (tid == 0 && b.stats) {
-        atomicAdd(&b.stats->composite_refs, loaded);
-        atomicAdd(&b.stats->resumed_refs, full - HDSR_MAX_TILE_ENTRIES);  // sorted in round 2
+  bool finished = false;  // every pixel saturated
+  uint32_t base = base0;
+  while (base < count) {  // uniform
+#pragma unroll
+    for (uint32_t s = 0; s < kDepth; ++s) {
+      if (base >= count) break;  // uniform
+      const uint32_t i = base + tid;
+      const uint4 h0 = h0q[s], h1 = h1q[s], h2 = h2q[s];
+      const uint32_t g = gq[s];
+      uint32_t qm = 0;  // quadrants the entry may reach
+      if (i < count)
+        qm = reach_quadrants(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0,
+                             tile_y0);
+      // compact the kept entries into slots, in order, and list each under its quadrants
+      const bool keep = qm != 0;
+      const unsigned long long km = __ballot(keep);
+      unsigned long long qb[4];
+    #pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) qb[k] = __ballot((qm >> k) & 1u);
+      if (lane == 0) {
+        s_cnt[0][wave] = (uint32_t)__popcll(km);
+    #pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) s_cnt[1 + k][wave] = (uint32_t)__popcll(qb[k]);
       }
-      return;
+      __syncthreads();
+      if (keep) {
+        uint32_t slot = (uint32_t)__popcll(km & below);
+        for (uint32_t w2 = 0; w2 < wave; ++w2) slot += s_cnt[0][w2];
+        s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
+        if (kD) {
+          s_b[slot] = make_float4(hdsr_u2f(h1.y), hdsr_u2f(h1.z), hdsr_u2f(h1.w), hdsr_u2f(h1.x) * -0.5f);
+          s_c[slot] = make_float4(hdsr_u2f(h2.x), hdsr_u2f(h2.y), 0.0f, 0.0f);
+          s_idx[slot] = i;
+        } else {
+          s_b[slot] = make_float4(h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16), hdsr_u2f(h1.x) * -0.5f);
+          s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), 0.0f, 0.0f);
+        }
+        if (kG) {
+          const uint4 c0 = b.cold[(size_t)g * 2];
+          const uint4 c1 = b.cold[(size_t)g * 2 + 1];
+          // octDecode (gpu_tile_splat_composite.wgsl:143-149)
+          const float ex = h16(c0.z), ey = h16(c0.z >> 16);
+          float dnx = ex, dny = ey, dnz = (1.0f - fabsf(ex)) - fabsf(ey);
+          if (dnz < 0.0f) {
+            const float sx = dnx >= 0.0f ? 1.0f : -1.0f, sy = dny >= 0.0f ? 1.0f : -1.0f;
+            const float ax = (1.0f - fabsf(dny)) * sx, ay = (1.0f - fabsf(dnx)) * sy;
+            dnx = ax;
+            dny = ay;
+          }
+          const float dl = sqrtf((dnx * dnx + dny * dny) + dnz * dnz);
+          s_ga[slot] = make_float4(h16(c0.x), h16(c0.x >> 16), h16(c0.w), h16(c1.x));
+          s_gb[slot] = make_float4(dnx / dl, dny / dl, dnz / dl, h16(c1.x >> 16));
+          s_gc[slot] = h16(c1.y);
+        }
+    #pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+          if ((qm >> k) & 1u) {
+            uint32_t pos = (uint32_t)__popcll(qb[k] & below);
+            for (uint32_t w2 = 0; w2 < wave; ++w2) pos += s_cnt[1 + k][w2];
+            s_list[k][pos] = (uint16_t)(slot * 16u);  // byte offset of the slot's float4 records
+          }
+      }
+      // uniform per wave: the walk below is a scalar loop
+      const uint32_t nq = __builtin_amdgcn_readfirstlane(s_cnt[1 + wave][0] + s_cnt[1 + wave][1] +
+                                                         s_cnt[1 + wave][2] + s_cnt[1 + wave][3]);
+      __syncthreads();
+      {  // slot s takes the batch kDepth later
+        const uint32_t i2 = base + 2 * kDepth * kBatch + tid;
+        gq[s] = ge[s];
+        h0q[s] = gload4(rec + (size_t)ge[s] * kRec);
+        h1q[s] = gload4(rec + (size_t)ge[s] * kRec + 1);
+        if (kD) h2q[s] = gload4(rec + (size_t)ge[s] * kRec + 2);
+        ge[s] = gload(ent + min(i2, last_e));
+      }
+      loaded += min(kBatch, count - base);
+      if (!__all(saturated(p))) {
+        // two entries per step: their exponentials are independent chains, blended in list order
+        const uint16_t* lst = s_list[wave];
+        for (uint32_t j = 0; j < nq; j += 2) {
+          if ((j & 30u) == 30u && __all(saturated(p))) break;  // wave saturated
+          const bool hb = j + 1u < nq;  // uniform
+          const uint32_t oa = lst[j], ob = hb ? lst[j + 1u] : oa;
+          const float4 Aa = rec16(s_a, oa), Ba = rec16(s_b, oa);
+          const float4 Ab = rec16(s_a, ob), Bb = rec16(s_b, ob);
+          const float dxa = (p00x - Aa.x) + add_x, dya = (p00y - Aa.y) + add_y;
+          const float dxb = (p00x - Ab.x) + add_x, dyb = (p00y - Ab.y) + add_y;
+          const float pwa = (Aa.z * dxa * dxa + Aa.w * dxa * dya) + Ba.w * dya * dya;
+          const float pwb = (Ab.z * dxb * dxb + Ab.w * dxb * dyb) + Bb.w * dyb * dyb;
+          const bool ra = __any(pwa > -4.0f), rb = hb && __any(pwb > -4.0f);  // uniform
+          if (!ra && !rb) continue;  // no pixel of the wave is reached by either
+          const float ala = alpha_of(pwa, Ba.x), alb = alpha_of(pwb, Bb.x);
+          if (ra) blend_slot<kG, kD>(p, q, pwa, ala, Ba, oa, s_c, s_ga, s_gb, s_gc, s_idx, &last);
+          if (rb) blend_slot<kG, kD>(p, q, pwb, alb, Bb, ob, s_c, s_ga, s_gb, s_gc, s_idx, &last);
+        }
+      }
+      const int wave_done = __all(saturated(p));
+      if (lane == 0) s_done[wave] = wave_done;
+      __syncthreads();
+      if (s_done[0] && s_done[1] && s_done[2] && s_done[3]) {
+        finished = true;
+        break;
+      }
+      base += kBatch;
     }
+    if (finished) break;
+  }
+  if (b.round == 1 && deferrable && !finished) {  // uniform: resumed in round 2
+    save_px(pst, p);
+    if (tid == 0) {
+      // list the tile and its chunks after the first (they follow the first chunk's segment)
+      b.pend_tiles[atomicAdd(&b.stats->pend_tiles, 1u)] = t;
+      const uint32_t seg0 = b.large_tiles[b.tile_large[m]].seg;
+      const uint32_t later = (full - 1u) / HDSR_MAX_TILE_ENTRIES;
+      const uint32_t at = atomicAdd(&b.stats->pend_segs, later);
+      for (uint32_t q = 0; q < later; ++q) b.pend_segs[at + q] = seg0 + 1u + q;
+      atomicAdd(&b.stats->composite_refs, loaded);
+      atomicAdd(&b.stats->resumed_refs, full - HDSR_MAX_TILE_ENTRIES);  // sorted in round 2
+    }
+    return;
   }
   store_px<kG>(F, X, Y, p, q);
   if (kD && X < F.width && Y < F.height) {
@@ -425,11 +482,51 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   }
 }
 
+// Rounds 0 and 1: one workgroup per tile.
+template <bool kG, bool kD, bool kH>
+__global__ __launch_bounds__(256) void k_composite(Bufs b) {
+  __shared__ TileLds<kG, kD> L;
+  composite_tile<kG, kD, kH, 1>(b, blockIdx.x, L);
+}
+
+// Round 2 of a deferred frame (no G-buffer, no differentiable records): the tiles round 1 listed,
+// grid-stride.  A few tiles run alone on their CUs, so each keeps several batches of record
+// gathers in flight instead of one.
+constexpr uint32_t kResumeDepth = 4;
+template <bool kH>
+__global__ __launch_bounds__(256) void k_composite_resume(Bufs b) {
+  __shared__ TileLds<false, false> L;
+  const uint32_t n = b.stats->pend_tiles;
+  for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+    composite_tile<false, false, kH, kResumeDepth>(b, __builtin_amdgcn_readfirstlane(b.pend_tiles[k]), L);
+    __syncthreads();  // the next tile restages the shared arrays
+  }
+}
+
+__global__ void k_reset_pending(DevStats* s) {
+  s->pend_tiles = 0;
+  s->pend_segs = 0;
+}
+
 }  // namespace
+
+void launch_reset_pending(const Bufs& b, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reset_pending, dim3(1), dim3(1), 0, stream, b.stats);
+}
 
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16) {
   if (!b.total_tiles) return;
-  const dim3 grid(b.total_tiles), block(256);
+  const dim3 block(256);
+  if (b.round == 2) {  // deferred frames only: no G-buffer, no differentiable records
+    // the listed tiles only: a grid that holds them at once, without a workgroup per tile
+    const dim3 grid(b.total_tiles > 2048u ? 2048u : b.total_tiles);
+    if (f16)
+      hipLaunchKernelGGL(k_composite_resume<true>, grid, block, 0, stream, b);
+    else
+      hipLaunchKernelGGL(k_composite_resume<false>, grid, block, 0, stream, b);
+    return;
+  }
+  const dim3 grid(b.total_tiles);
   if (b.drec)
     hipLaunchKernelGGL((k_composite<false, true, false>), grid, block, 0, stream, b);
   else if (f16 && gbuffer)

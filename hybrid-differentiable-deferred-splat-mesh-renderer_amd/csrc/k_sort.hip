@@ -260,18 +260,20 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t red[2][kSortWaves];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
-  const uint32_t nseg = b.stats->segments;
-  for (uint32_t s = blockIdx.x; s < nseg; s += gridDim.x) {
-    const Seg S = b.segs[s];
-    if (b.round) {  // uniform: deferred chunk sort
+  // round 2 walks the chunks the first round listed (the later chunks of the tiles it left
+  // unsaturated); the other rounds walk every segment
+  const bool listed = b.round == 2;  // uniform
+  const uint32_t nseg = listed ? b.stats->pend_segs : b.stats->segments;
+  for (uint32_t k = blockIdx.x; k < nseg; k += gridDim.x) {
+    const Seg S = b.segs[listed ? b.pend_segs[k] : k];
+    if (b.round == 1 || b.round == 3) {  // uniform: deferred chunk sort
       bool take;
       if (S.large == kNone) {
         take = b.round == 1;
-      } else {  // round 1: the first chunk; round 2: the other chunks of the tiles resumed;
-                // round 3: every other chunk (completes the lists of a deferred frame)
-        const LargeTile& L = b.large_tiles[S.large];
-        const bool first = S.dst == L.start;
-        take = b.round == 1 ? first : b.round == 2 ? !first && b.tile_pend[L.morton] != 0u : !first;
+      } else {  // round 1: the first chunk; round 3: every other chunk (completes the lists of a
+                // deferred frame)
+        const bool first = S.dst == b.large_tiles[S.large].start;
+        take = b.round == 1 ? first : !first;
       }
       if (!take) continue;
     }
@@ -385,7 +387,9 @@ void launch_bucket(const Bufs& b, hipStream_t stream) {
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   if (!b.seg_cap) return;
-  const uint32_t cap = 2048u * 1024u / kSortThreads;
+  // round 2 sorts the listed chunks only: a grid that holds them at once without dispatching
+  // thousands of workgroups that find nothing to do
+  const uint32_t cap = b.round == 2 ? 1024u : 2048u * 1024u / kSortThreads;
   const uint32_t grid = b.seg_cap < cap ? b.seg_cap : cap;
   hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
   if (b.ref_caps) hipLaunchKernelGGL(k_segcopy, dim3(grid), dim3(256), 0, stream, b);

@@ -484,6 +484,7 @@ struct Frame {
   std::vector<uint32_t> stats;  // see orc_stat_*
   std::vector<double> rgba64, depth64;
   std::vector<float> diffrec;  // differentiable mode: (opacity, r, g, b) per rank, no f16 rounding
+  std::vector<uint32_t> tile_loaded;  // per Morton slot: entries a 256-entry-batch walk loads before every pixel saturates
   Splats S;
   bool ref_caps = false;  // the reference's WebGPU caps enforced (see bin_and_sort)
 };
@@ -721,11 +722,13 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
     F.gb_normal.assign((size_t)W * H, 0);
     F.gb_material.assign((size_t)W * H * 4, 0);
   }
+  F.tile_loaded.assign(F.morton_count, 0);
   par_for((F.tiles_y + row_step - 1) / row_step, [&](uint32_t row) {
     const uint32_t ty = row * row_step;
     for (uint32_t tx = 0; tx < F.tiles_x; ++tx) {
       const uint32_t m = hdsr_morton2(tx, ty);
       const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
+      uint32_t tile_sat = 0;  // the largest saturation index over the tile's 256 pixels
       for (uint32_t py = 0; py < 16; ++py)
         for (uint32_t pxl = 0; pxl < 16; ++pxl) {
           const uint32_t X = tx * 16 + pxl, Y = ty * 16 + py;
@@ -734,9 +737,13 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
           const float p00y = (float)(ty * 16 + (py & ~1u)) + 0.5f;
           float T = 1.0f, cr = 0, cg = 0, cb = 0;
           float gdw = 0, gws = 0;
+          uint32_t sat = c;  // first entry index at which the pixel is saturated (c: never)
           float mr = 0, mm = 0, nx = 0, ny = 0, nz = 0, gconf = 0, er = 0, eg = 0, eb = 0;
           for (uint32_t i = 0; i < c; ++i) {
-            if (T < t_cut) break;  // no later splat passes 'T >= cutoff': nothing changes
+            if (T < t_cut) {  // no later splat passes 'T >= cutoff': nothing changes
+              sat = i;
+              break;
+            }
             const uint32_t r = F.entries[s + i];
             const uint32_t* w = &F.cache[(size_t)r * 14];
             const float cx = u2f(w[0]), cy = u2f(w[1]);
@@ -797,6 +804,7 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
               eb = eb + h2f(w[12]) * wgt;
             }
           }
+          tile_sat = std::max(tile_sat, sat);
           // The per-quad early exit (all four T < cutoff) never changes a pixel: once T < 0.001,
           // 'valid' is false for every later splat.
           if (X >= W || Y >= H) continue;
@@ -825,6 +833,9 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
             F.gb_material[p * 4 + 3] = hdsr_pack2x16(gconf / safe, 0.0f);
           }
         }
+      // the entries the rasterizer loads: whole 256-entry batches until every pixel of the tile is
+      // saturated (a pixel saturated before entry i blended its last entry in batch (i - 1) / 256)
+      F.tile_loaded[m] = (c == 0 || tile_sat >= c) ? c : std::min(c, 256u * ((tile_sat - 1u) / 256u + 1u));
     }
   });
 }
@@ -946,6 +957,7 @@ enum {
   ORC_DEPTH64,
   ORC_RAW_INVERSIONS,
   ORC_DIFFREC,  // differentiable mode: (opacity, r, g, b) fp32 per rank, as projected
+  ORC_TILE_LOADED,  // per Morton slot: entries the rasterizer loads (whole batches until saturation)
   ORC_NUM
 };
 
@@ -1048,6 +1060,7 @@ uint64_t orc_size(void* h, int which) {
     case ORC_DEPTH64: return F->depth64.size() * 8;
     case ORC_RAW_INVERSIONS: return F->raw_order_inversions.size() * 4;
     case ORC_DIFFREC: return F->diffrec.size() * 4;
+    case ORC_TILE_LOADED: return F->tile_loaded.size() * 4;
     default: return 0;
   }
 }
@@ -1079,6 +1092,7 @@ int orc_get(void* h, int which, void* dst, uint64_t bytes) {
     case ORC_DEPTH64: src = F->depth64.data(); break;
     case ORC_RAW_INVERSIONS: src = F->raw_order_inversions.data(); break;
     case ORC_DIFFREC: src = F->diffrec.data(); break;
+    case ORC_TILE_LOADED: src = F->tile_loaded.data(); break;
     default: return -2;
   }
   if (n) memcpy(dst, src, n);

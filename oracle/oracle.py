@@ -18,7 +18,7 @@ OUT = {
     "cache": 0, "depth": 1, "large_list": 2, "tile_counts": 3, "tile_offsets": 4, "entries_canon": 5,
     "small_tiles": 6, "large_tiles": 7, "large_bases": 8, "chunks": 9, "entries": 10, "keys": 11, "rgba": 12,
     "out_depth": 13, "gb_normal": 14, "gb_material": 15, "stats": 16, "rgba64": 17, "depth64": 18,
-    "raw_inversions": 19, "diffrec": 20,
+    "raw_inversions": 19, "diffrec": 20, "tile_loaded": 21,
 }
 STATS = ["refs", "small_tiles", "large_tiles", "chunks", "large_splats", "visible", "cap_refs_tripped",
          "cap_large_splats_tripped", "cap_chunks_tripped", "ref_cap"]

@@ -39,6 +39,9 @@ def test_deferred_equals_one_pass_and_oracle(hip_device, faint, f16):
         orc = oracle.run(s, srt, u, cam.width, cam.height)
         assert np.abs(rgba.reshape(-1) - orc["rgba"]).max() <= 1e-5
         assert np.abs(depth.reshape(-1) - orc["out_depth"]).max() <= 1e-5
+        # the rounds load what one pass loads: whole 256-entry batches up to each tile's saturation
+        assert sl["composite_refs"] == int(orc["tile_loaded"].sum())
+        assert sl["composite_max_refs"] == int(orc["tile_loaded"].max())
     compositor.encodeCompositeOnly(None, lazy)
     lazy.sync()
     np.testing.assert_array_equal(lazy.read_rgba(), rgba)

@@ -241,6 +241,30 @@ def test_single_isotropic_splat_closed_form():
     np.testing.assert_allclose(rgba[24, 32 - 3], rgba[24, 32 + 2], atol=1e-6)
 
 
+@pytest.mark.parametrize("opacity,loaded", [(0.9, 256), (0.034, 256), (0.02, 512), (0.01, 768), (0.003, 1000)])
+def test_tile_loaded_closed_form(opacity, loaded):
+    """1000 copies of one wide splat over a 64x48 image: every pixel sees alpha = o * exp(p) with
+    |p| < 0.01, so its transmittance (1 - alpha)^n drops below 0.001 at n = ln(0.001) / ln(1 - alpha)
+    (3, 200, 344, 688 entries; never below 1/255 opacity).  The rasterizer loads whole 256-entry
+    batches up to the one holding the last pixel's saturating entry, or the whole list."""
+    n = 1000
+    s = scenes.Scene(np.zeros((n, 3), np.float32), np.full((n, 3), math.log(2.0), np.float32),
+                     np.tile(np.asarray([[1, 0, 0, 0]], np.float32), (n, 1)), np.full(n, opacity, np.float32),
+                     np.full((n, 3), 0.5, np.float32), None, 0, scenes.pack_material(n))
+    cam = camera.Camera(camera.orbit_eye(3.0, 0.0, 0.0), (0, 0, 0), 525.0, 525.0, 64, 48)
+    plan, srt, u = frame_inputs(s, cam)
+    f = oracle.run(s, srt, u, cam.width, cam.height)
+    counts = f["tile_counts"]
+    used = counts > 0
+    assert used.sum() == 12 and np.all(counts[used] == n)
+    np.testing.assert_array_equal(f["tile_loaded"][used], loaded)
+    assert np.all(f["tile_loaded"][~used] == 0)
+    a = float(np.float16(opacity))
+    if a > 1 / 255:
+        sat = math.ceil(math.log(0.001) / math.log(1 - a))
+        assert loaded == min(n, 256 * ((sat - 1) // 256 + 1))
+
+
 def test_bruteforce_blend_matches_oracle(small_frame):
     """Per pixel: walk the oracle's per-tile order front to back in float64 with the fp32 cache
     values and compare with the oracle's fp32 image."""
