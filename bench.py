@@ -74,6 +74,8 @@ def parse():
                     help="run only a device-to-device copy of MIB MiB (5 times): the known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--cpu-budget-s", type=float, default=20.0, help="target CPU seconds for the sample")
+    ap.add_argument("--bucket-scatter", action="store_true",
+                    help="keep the whole bucket scatter of the large tiles in every deferred frame's first round")
     return ap.parse_args()
 
 
@@ -155,7 +157,7 @@ def main() -> int:
     uniforms = [frame_uniforms_for(c, plan, scene.sh_degree) for c in cams]
     dev = torch.device("cuda:0")
     sorted_dev = [torch.from_numpy(s.astype(np.int32)).to(dev) for s in sorted_host]
-    res = compositor.createTileSplatCompositor(None, plan, profile=True)
+    res = compositor.createTileSplatCompositor(None, plan, profile=True, bucket_scatter=a.bucket_scatter)
     compositor.bind_scene(res, scene, sorted_host[0])
     stream = torch.cuda.current_stream()
     handle = stream.cuda_stream

@@ -196,6 +196,7 @@ int hdsr_create(const hdsr_plan* plan, uint32_t flags, hdsr_ctx** out) {
   c->eng.set_min_capacity(base_cap);
   if (flags & HDSR_REF_CAPS) c->eng.set_ref_caps(plan->max_total_tile_refs);
   c->eng.set_profile((flags & HDSR_PROFILE) != 0);
+  c->eng.set_bucket_scatter((flags & HDSR_BUCKET_SCATTER) != 0);
   *out = c;
   return HDSR_OK;
 }

@@ -76,6 +76,10 @@ class Engine {
 
   // Per-stage HIP events (one set per run, kept in a ring until harvested).
   void set_profile(bool on) { profile_ = on; }
+  void set_bucket_scatter(bool on) {
+    bucket_scatter_ = on;
+    first_chunk_ = !on;
+  }
   int stage_times(float* ms, uint32_t* runs);
   // Sorts the chunks a deferred frame left unsorted (no-op otherwise), so that the entry lists
   // are whole: for readers of the lists (debug readback).
@@ -115,11 +119,17 @@ class Engine {
   bool deferred_ = false;     // the last enqueued frame left chunks unsorted (deferred chunk sort)
   bool last_deferred_ = false;  // as deferred_, but not cleared when the lists are completed
   bool bucket_wide_ = false;  // the last settled frame's large tiles averaged more than 16k entries
+  // the last settled frame resumed less than half of its large-tile references in round 2: the next
+  // deferred frames select first chunks without the bucket scatter
+  bool first_chunk_ = true;
+  bool bucket_scatter_ = false;  // HDSR_BUCKET_SCATTER: never select first chunks
+  bool last_first_ = false;  // the mode of the last deferred frame (its rounds 2 / 3 follow it)
   size_t n_diff_ = 0;
   // allocation sizes (elements)
   size_t n_splat_ = 0, n_blocks_ = 0, n_scan4_ = 0, n_cap_ = 0, n_hist_ = 0, n_hist_tmp_ = 0;
   size_t n_morton_ = 0, n_tiles_ = 0, n_seg_ = 0, n_frames_ = 0, n_debug_ = 0;
   size_t n_cellpairs_ = 0, n_cells_ = 0, n_chunkcnt_ = 0, n_radix_tmp_need_ = 0;
+  size_t n_first_ = 0;  // large tiles first[] holds
   FrameDesc* d_frames_ = nullptr;
   uint32_t* d_starts_ = nullptr;
   // pinned staging ring for the frame table

@@ -87,6 +87,7 @@ void Engine::release() {
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, b_.tile_large, b_.pend_tiles, b_.pend_segs, b_.px_state,
+                  b_.pend_large, b_.first, b_.large_order,
                   d_frames_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -110,6 +111,7 @@ void Engine::release() {
   n_splat_ = n_blocks_ = n_scan4_ = n_cap_ = n_hist_ = n_hist_tmp_ = 0;
   n_morton_ = n_tiles_ = n_seg_ = n_frames_ = n_debug_ = n_diff_ = 0;
   n_cellpairs_ = n_cells_ = n_chunkcnt_ = 0;
+  n_first_ = 0;
 }
 
 template <typename T>
@@ -251,9 +253,24 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.large_tiles, &have, L.total_tiles))) return rc;
     have = n_tiles_;
     if ((rc = grow(&b_.pend_tiles, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.pend_large, &have, L.total_tiles))) return rc;
+    have = n_tiles_;
+    if ((rc = grow(&b_.large_order, &have, L.total_tiles))) return rc;
     have = n_tiles_ * 256 * 6;
     if ((rc = grow(&b_.px_state, &have, (size_t)L.total_tiles * 256 * 6))) return rc;
     n_tiles_ = L.total_tiles;
+  }
+  // first chunks of the deferred frames' large tiles: a large tile holds more than 4096 of the at
+  // most 'cap' stored entries
+  {
+    const size_t ft = std::min<size_t>(L.total_tiles, (size_t)b_.cap / (HDSR_MAX_TILE_ENTRIES + 1) + 1);
+    if (ft > n_first_ || !b_.first) {
+      have = n_first_ * HDSR_MAX_TILE_ENTRIES;
+      if ((rc = grow(&b_.first, &have, ft * HDSR_MAX_TILE_ENTRIES))) return rc;
+      n_first_ = ft;
+    }
+    b_.first_tiles = (uint32_t)n_first_;
   }
   // segments: every small tile, plus at most one 4096-entry chunk per 4096 pairs and per large tile
   const size_t segs = 2 * (size_t)L.total_tiles + (n_cap_ + HDSR_MAX_TILE_ENTRIES - 1) / HDSR_MAX_TILE_ENTRIES + 1;
@@ -366,21 +383,32 @@ void Engine::enqueue_stages(const Bufs& b, hipStream_t stream, bool gbuffer, boo
   if (ev) (void)hipEventRecord(ev[6], stream);
   if (debug && b.entries_canon)
     (void)hipMemcpyAsync(b.entries_canon, b.entries, (size_t)b.cap * 4, hipMemcpyDeviceToDevice, stream);
-  launch_bucket(b, stream);
-  if (ev) (void)hipEventRecord(ev[7], stream);
   // Deferred chunk sort when nothing reads the whole lists: no debug readback, no G-buffer (its
   // votes are not part of the saved pixel state), no backward pass.
   deferred_ = !debug && !gbuffer && !b.drec && !b.ref_caps;
   last_deferred_ = deferred_;
+  const bool first = deferred_ && first_chunk_;
+  last_first_ = first;
   Bufs r1 = b, r2 = b;
   r1.round = deferred_ ? 1u : 0u;
   r2.round = 2u;
+  r1.first_chunks = r2.first_chunks = first ? 1u : 0u;
+  if (first) {
+    // the large tiles' first chunks and the small tiles' sorts in one launch (the sort stage of
+    // round 1 is in the bucket stage's time)
+    launch_first_chunks(r1, stream);
+    if (ev) (void)hipEventRecord(ev[7], stream);
+  } else {
+    launch_bucket(r1, stream);
+    if (ev) (void)hipEventRecord(ev[7], stream);
+    launch_segment_sort(r1, stream);
+  }
   // Stage events: sort = [7, 8] + [10, 11], composite = [8, 10] + [11, 9] (the rounds interleave).
-  launch_segment_sort(r1, stream);
   if (ev) (void)hipEventRecord(ev[8], stream);
   if (deferred_) {
     launch_composite(r1, stream, gbuffer, f16_);
     if (ev) (void)hipEventRecord(ev[kStages + 1], stream);
+    if (first) launch_bucket(r2, stream);  // the bucket order of the tiles left unsaturated
     launch_segment_sort(r2, stream);
     if (ev) (void)hipEventRecord(ev[kStages + 2], stream);
     launch_composite(r2, stream, gbuffer, f16_);
@@ -437,7 +465,8 @@ int Engine::run_graph(const FrameDesc* frames_in, uint32_t V, hipStream_t stream
     tab[2 * (V + 1) + v] = L.cell_start[v];
   }
   const uint64_t sig[7] = {V, L.splat_blocks, L.total_tiles, L.total_morton, L.total_cells, L.cell_key_bits,
-                           (uint64_t)gbuffer | ((uint64_t)L.packed << 1) | ((uint64_t)bucket_wide_ << 2)};
+                           (uint64_t)gbuffer | ((uint64_t)L.packed << 1) | ((uint64_t)bucket_wide_ << 2) |
+                               ((uint64_t)first_chunk_ << 3)};
   hipGraphExec_t exec = nullptr;
   for (const GraphEntry& g : graphs_)
     if (memcmp(g.sig, sig, sizeof sig) == 0) exec = g.exec;
@@ -568,6 +597,7 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
     Bufs r1 = b_, r2 = b_;
     r1.round = 1;
     r2.round = 2;
+    r1.first_chunks = r2.first_chunks = last_first_ ? 1u : 0u;
     launch_reset_pending(b_, stream);
     launch_composite(r1, stream, gbuffer, f16_);
     launch_composite(r2, stream, gbuffer, f16_);
@@ -586,6 +616,8 @@ int Engine::complete_lists(hipStream_t stream) {
   if (!deferred_) return 0;
   Bufs r3 = b_;
   r3.round = 3;
+  r3.first_chunks = last_first_ ? 1u : 0u;
+  if (last_first_) launch_bucket(r3, stream);  // the tiles round 2 did not order
   launch_segment_sort(r3, stream);
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) {
@@ -650,6 +682,11 @@ int Engine::wait() {
   // entries, 1024 when tiles of 100k entries would leave one 512-thread workgroup as the tail
   const DevStats& s = *hstats_;
   bucket_wide_ = s.large_tiles != 0 && s.scratch_used / s.large_tiles > 16384u;
+  // first chunks pay when round 2 has little to scatter: the resumed tiles hold less than half of
+  // the large tiles' references and none is so long that its scatter, one workgroup after round 1,
+  // would form the frame's tail
+  if (last_deferred_ && !bucket_scatter_)
+    first_chunk_ = (uint64_t)s.resumed_refs * 2u < s.scratch_used && s.resumed_max <= 32768u;
   return 0;
 }
 

@@ -133,6 +133,7 @@ struct DevStats {
   uint32_t resumed_refs;        // deferred frames: entries after the first chunk of the resumed tiles
   uint32_t pend_tiles;          // deferred frames: tiles the first round left unsaturated (pend_tiles[])
   uint32_t pend_segs;           // deferred frames: their chunks after the first (pend_segs[])
+  uint32_t resumed_max;         // deferred frames: entries of the largest resumed tile
 };
 
 struct Seg {
@@ -150,7 +151,8 @@ struct LargeTile {
   uint32_t count;
   uint32_t scratch;  // first slot in scratch[]
   uint32_t seg;      // the segment of its first chunk (the others follow it)
-  uint32_t pad1, pad2, pad3;
+  uint32_t pending;  // deferred frames: 1 once the first round left the tile unsaturated (resumed in round 2)
+  uint32_t pad2, pad3;
 };
 
 // Device buffers of one batch (all owned by the host context).
@@ -213,18 +215,27 @@ struct Bufs {
   DevStats* stats;
   uint32_t packed;           // every frame reads its scene from packed records (host-set, selects the kernel)
   uint32_t bucket_wide;      // large tiles average > 16k entries: one 1024-thread workgroup per tile (host-set)
+  uint32_t first_chunks;     // deferred frames: round 1 selects the large tiles' first chunks without the
+                             //   bucket scatter (k_first_chunks); rounds 2 and 3 then scatter (host-set)
   // Deferred chunk sort (host-set round; 0 = the whole frame in one pass).  Round 1 sorts the small
   // tiles and the first chunk of every large tile, and rasterizes each large tile up to the end of
   // that chunk; a tile left unsaturated there saves its pixel state and appends itself to
   // pend_tiles[] and its other chunks to pend_segs[].  Round 2 sorts those chunks and resumes
   // those tiles, walking the two lists only.  Round 3 sorts every chunk
   // after the first: it completes the lists of a deferred frame for a reader of the whole lists.
+  // In round 1 a large tile's first chunk is selected and sorted from the tile's bucket
+  // histogram into first[] (k_first_chunks), so its canonical list stays in entries[] until the
+  // bucket scatter of round 2 (the resumed tiles) or round 3 (the others) reads it.
   uint32_t round;
   uint32_t ref_caps;          // the reference's WebGPU caps enforced (HDSR_REF_CAPS; one frame)
   uint32_t* tile_unl;         // [total_morton] with ref_caps: large splats past the first 65,535 per tile
   uint32_t* tile_large;      // [total_morton] large tiles only: index into large_tiles[]
   uint32_t* pend_tiles;      // [total_tiles] deferred frames: global tile indices resumed in round 2
   uint32_t* pend_segs;       // [seg capacity] deferred frames: the segments round 2 sorts
+  uint32_t* pend_large;      // [total_tiles] deferred frames: large-tile index of each pend_tiles[] entry
+  uint32_t* first;           // [first_tiles * 4096] deferred frames: large tile l's sorted first chunk at l * 4096
+  uint32_t* large_order;     // [total_tiles] large-tile indices by decreasing size (k_first_chunks' order)
+  uint32_t first_tiles;      // large tiles first[] holds
   float* px_state;           // [total_tiles*256*6] T, r, g, b, depth weight, weight sum per pixel (planar)
   // The projection writes cold[] (read only by the G-buffer rasterizer and the 14-word debug
   // readback) only when set.
@@ -263,7 +274,8 @@ void launch_emit_pairs(const Bufs& b, hipStream_t stream);    // (cell, rank) pa
 void launch_group_pairs(const Bufs& b, hipStream_t stream);   // radix on the cell id, tile counts
 void launch_tile_ranges(const Bufs& b, hipStream_t stream);   // tile offsets, expansion into tile lists
 void launch_classify(const Bufs& b, hipStream_t stream);
-void launch_bucket(const Bufs& b, hipStream_t stream);
+void launch_bucket(const Bufs& b, hipStream_t stream);        // rounds 0, 2, 3: the whole bucket order
+void launch_first_chunks(const Bufs& b, hipStream_t stream);  // round 1: large tiles' first chunks + small tiles
 void launch_segment_sort(const Bufs& b, hipStream_t stream);
 void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16);
 void launch_reset_pending(const Bufs& b, hipStream_t stream);  // empties the round-2 lists (re-run round 1)

@@ -325,7 +325,11 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
   // Past the end of the list a lane reads the list's last entry and gathers its record, which the
   // staging ignores: every load stays unpredicated, so none waits for a select.
   const uint4* const rec = kD ? reinterpret_cast<const uint4*>(b.drec) : b.hot;
-  const uint32_t* const ent = b.entries + start;
+  // round 1 reads a large tile's first chunk from first[] when k_first_chunks sorted it there; the
+  // tile's list in entries[] then stays canonical until round 2 or 3 orders it
+  const uint32_t* const ent = (b.round == 1 && deferrable && b.first_chunks)
+                                  ? b.first + (size_t)b.tile_large[m] * HDSR_MAX_TILE_ENTRIES
+                                  : b.entries + start;
   constexpr uint32_t kRec = kD ? 3 : 2;  // 16-byte words per record
   const uint4 z4 = make_uint4(0, 0, 0, 0);
   const uint32_t last_e = count - 1u;  // read only when count > base0 >= 0
@@ -460,13 +464,18 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
     save_px(pst, p);
     if (tid == 0) {
       // list the tile and its chunks after the first (they follow the first chunk's segment)
-      b.pend_tiles[atomicAdd(&b.stats->pend_tiles, 1u)] = t;
-      const uint32_t seg0 = b.large_tiles[b.tile_large[m]].seg;
+      const uint32_t l = b.tile_large[m];
+      const uint32_t k = atomicAdd(&b.stats->pend_tiles, 1u);
+      b.pend_tiles[k] = t;
+      b.pend_large[k] = l;
+      b.large_tiles[l].pending = 1u;
+      const uint32_t seg0 = b.large_tiles[l].seg;
       const uint32_t later = (full - 1u) / HDSR_MAX_TILE_ENTRIES;
       const uint32_t at = atomicAdd(&b.stats->pend_segs, later);
       for (uint32_t q = 0; q < later; ++q) b.pend_segs[at + q] = seg0 + 1u + q;
       atomicAdd(&b.stats->composite_refs, loaded);
       atomicAdd(&b.stats->resumed_refs, full - HDSR_MAX_TILE_ENTRIES);  // sorted in round 2
+      atomicMax(&b.stats->resumed_max, full);
     }
     return;
   }

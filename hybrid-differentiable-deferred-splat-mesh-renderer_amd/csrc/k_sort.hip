@@ -61,9 +61,39 @@ __device__ __forceinline__ uint32_t group_count(uint32_t* counter, unsigned long
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(leader << 2), (int)old);
 }
 
+// Adds the bucket counts (bucket = q20 >> 13) of keys [lo, hi) into one wave's 128 counters:
+// per round of 64 keys the lanes of equal bucket are found by seven ballots and the group's highest
+// lane adds the group's size, so no two lanes of a round update one counter (a plain LDS atomic per
+// key serialises on the clustered depths of a tile).  The next 256 keys are in flight meanwhile.
+__device__ __forceinline__ void wave_bucket_counts(const uint32_t* qsrc, uint32_t lo, uint32_t hi, uint32_t* cnt) {
+  const uint32_t lane = __lane_id();
+  uint32_t kk[4], kn[4];
+#pragma unroll
+  for (uint32_t u = 0; u < 4; ++u) {
+    const uint32_t i = lo + u * 64 + lane;
+    kk[u] = i < hi ? qsrc[i] >> 13 : 0u;
+  }
+  for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t i = r0 + 256 + u * 64 + lane;
+      kn[u] = i < hi ? qsrc[i] >> 13 : 0u;
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const bool valid = r0 + u * 64 + lane < hi;
+      const uint32_t k = kk[u];
+      const unsigned long long m = match_digit<7>(k, __ballot(valid));
+      if (valid && (m >> lane) == 1ull) atomicAdd(&cnt[k], (uint32_t)__popcll(m));
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) kk[u] = kn[u];
+  }
+}
+
 // Large tiles.  Wave w of the kBucketThreads-thread workgroup owns the contiguous slice
-// [w*S, (w+1)*S) of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the entry's key;
-// ranks inside a round from seven ballots, a wave-private running count across rounds).  Then one
+// [w*S, (w+1)*S) of the tile's entries.  Pass 1: per-wave bucket counts (bucket = q20 >> 13 of the
+// entry's key; wave_bucket_counts).  Then one
 // scan over (bucket, wave) gives every (wave, bucket) its first scratch slot, and pass 2 walks the
 // slice in the same order and writes each entry (and its key) at slot + its running rank: a
 // stable bucket order with two workgroup barriers per tile.
@@ -71,6 +101,9 @@ __device__ __forceinline__ uint32_t group_count(uint32_t* counter, unsigned long
 // and lets four workgroups share a CU: 0.245 / 0.205 / 0.237 ms per 1080p frame at 1024 / 512 /
 // 256 threads.  Frames whose large tiles average more than 16k entries (a 4K view into a dense
 // cluster: tiles of 100k entries) use 1024 threads, so the biggest tiles do not form the tail.
+// Round 0 (one pass) orders every large tile; in a deferred frame, round 2 orders the tiles the
+// first round left unsaturated (pend_large[]) and round 3 the others, whose canonical lists are
+// still in entries[] (round 1 sorted their first chunks into first[] without a scatter).
 template <uint32_t kBucketThreads>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket(Bufs b) {
   constexpr uint32_t kBucketWaves = kBucketThreads / 64;
@@ -78,32 +111,20 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket(Bufs b) {
   __shared__ uint32_t wsum[2];
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
-  const uint32_t nl = b.stats->large_tiles;
-  for (uint32_t l = blockIdx.x; l < nl; l += gridDim.x) {
+  const bool listed = b.round == 2;  // uniform
+  const uint32_t nl = listed ? b.stats->pend_tiles : b.stats->large_tiles;
+  for (uint32_t k = blockIdx.x; k < nl; k += gridDim.x) {
+    const uint32_t l = listed ? b.pend_large[k] : k;
     const LargeTile L = b.large_tiles[l];
+    if (b.round == 3 && L.pending) continue;  // uniform: round 2 ordered it
     const uint32_t* src = b.entries + L.start;
     const uint32_t* qsrc = b.qkey + L.start;
     const uint32_t S = ((L.count + kBucketThreads - 1u) / kBucketThreads) * 64u;
     const uint32_t lo = w * S, hi = min(lo + S, L.count);
     for (uint32_t q = t; q < kBucketWaves * HDSR_NUM_BUCKETS; q += kBucketThreads) wc[q >> 7][q & 127] = 0;
     __syncthreads();
-    // pass 1: wave-private bucket counts, four rounds of loads in flight
-    for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
-      uint32_t kk[4];
-#pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const uint32_t i = r0 + u * 64 + lane;
-        kk[u] = i < hi ? qsrc[i] >> 13 : 0u;
-      }
-#pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const bool valid = r0 + u * 64 + lane < hi;
-        const uint32_t k = kk[u];
-        const unsigned long long m = match_digit<7>(k, __ballot(valid));
-        if (valid && (m >> lane) == 1ull) wc[w][k] += (uint32_t)__popcll(m);
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
+    // pass 1: wave-private bucket counts, the next 256 keys in flight
+    wave_bucket_counts(qsrc, lo, hi, wc[w]);
     __syncthreads();
     // scan over (bucket, wave), bucket-major: threads 0..127 own one bucket each
     uint32_t tot = 0, inc = 0;
@@ -176,24 +197,28 @@ constexpr uint32_t kSortWaves = kSortThreads / 64;
 constexpr uint32_t kSortPer = HDSR_MAX_TILE_ENTRIES / kSortThreads;
 
 // One LSD pass over keys[0..n), in place: stable by the digit (key >> shift) & (2^bits - 1).
-// Wave w owns positions [w*S, min((w+1)*S, n)) (S a multiple of 64, at most 64 * kSortPer) and
+// Wave w owns positions [w*S, min((w+1)*S, n)) (S a multiple of 64, at most 64 * 4096 / kT) and
 // walks them in rounds of 64; equal digits inside a round are ranked by ballots, across rounds by
 // a wave-private count.  Every key is read into registers before the barriers that precede the
 // scatter, so the scatter can overwrite the same array.  Every thread of the workgroup must call
 // it (it contains barriers).
-template <uint32_t kBits>
+// kT threads (kT / 64 waves, 4096 / kT keys each); wc has a row of kRow >= 2^bits digit counts per
+// wave, and kRow <= kT.
+template <uint32_t kBits, uint32_t kT, uint32_t kRow>
 __device__ __forceinline__ void lds_radix_pass(uint32_t* ks, uint32_t n, uint32_t S, uint32_t shift,
-                                               uint32_t (*wc)[256], uint32_t* wsum) {
+                                               uint32_t (*wc)[kRow], uint32_t* wsum) {
+  static_assert((1u << kBits) <= kRow && kRow <= kT && kRow % 64 == 0, "digit rows");
+  constexpr uint32_t kWaves = kT / 64, kPer = HDSR_MAX_TILE_ENTRIES / kT;
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t mask = (1u << kBits) - 1u;
-  for (uint32_t q = t; q < kSortWaves * 256; q += kSortThreads) wc[q >> 8][q & 255] = 0;
+  for (uint32_t q = t; q < kWaves * kRow; q += kT) wc[q / kRow][q % kRow] = 0;
   __syncthreads();
   const uint32_t lo = w * S, hi = min(lo + S, n);
-  const uint32_t rounds = S / 64;  // the same for every wave: a segment of n keys needs ceil(n/256)
-  uint32_t keys[kSortPer], pos[kSortPer];
+  const uint32_t rounds = S / 64;  // the same for every wave: a segment of n keys needs ceil(n/kT)
+  uint32_t keys[kPer], pos[kPer];
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPer; ++r) {
+  for (uint32_t r = 0; r < kPer; ++r) {
     pos[r] = 0xFFFFFFFFu;
     if (r >= rounds) continue;  // uniform
     const uint32_t p = lo + r * 64 + lane;
@@ -209,9 +234,9 @@ __device__ __forceinline__ void lds_radix_pass(uint32_t* ks, uint32_t n, uint32_
   __syncthreads();
   // digit-major exclusive scan over (digit, wave): digit t's base, then each wave's offset
   uint32_t tot = 0, inc = 0;
-  if (t < 256) {
+  if (t < kRow) {  // the first kRow / 64 waves: one digit per thread
 #pragma unroll
-    for (uint32_t q = 0; q < kSortWaves; ++q) tot += wc[q][t];
+    for (uint32_t q = 0; q < kWaves; ++q) tot += wc[q][t];
     inc = tot;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -221,11 +246,11 @@ __device__ __forceinline__ void lds_radix_pass(uint32_t* ks, uint32_t n, uint32_
     if (lane == 63) wsum[w] = inc;
   }
   __syncthreads();
-  if (t < 256) {
+  if (t < kRow) {
     uint32_t run = inc - tot;
     for (uint32_t q = 0; q < w; ++q) run += wsum[q];
 #pragma unroll
-    for (uint32_t q = 0; q < kSortWaves; ++q) {
+    for (uint32_t q = 0; q < kWaves; ++q) {
       const uint32_t c = wc[q][t];
       wc[q][t] = run;
       run += c;
@@ -233,33 +258,126 @@ __device__ __forceinline__ void lds_radix_pass(uint32_t* ks, uint32_t n, uint32_
   }
   __syncthreads();
 #pragma unroll
-  for (uint32_t r = 0; r < kSortPer; ++r)
+  for (uint32_t r = 0; r < kPer; ++r)
     if (r < rounds && pos[r] != 0xFFFFFFFFu) ks[wc[w][(keys[r] >> shift) & mask] + pos[r]] = keys[r];
   __syncthreads();
 }
 
-// One pass of a digit width known only at run time (uniform): 4 to 8 bits.
+// One pass of a digit width known only at run time (uniform): 4 to log2(kRow) bits.
+template <uint32_t kT, uint32_t kRow>
 __device__ __forceinline__ void radix_pass_w(uint32_t w, uint32_t* ks, uint32_t n, uint32_t S, uint32_t shift,
-                                             uint32_t (*wc)[256], uint32_t* wsum) {
+                                             uint32_t (*wc)[kRow], uint32_t* wsum) {
   switch (w) {
-    case 4: lds_radix_pass<4>(ks, n, S, shift, wc, wsum); break;
-    case 5: lds_radix_pass<5>(ks, n, S, shift, wc, wsum); break;
-    case 6: lds_radix_pass<6>(ks, n, S, shift, wc, wsum); break;
-    case 7: lds_radix_pass<7>(ks, n, S, shift, wc, wsum); break;
-    default: lds_radix_pass<8>(ks, n, S, shift, wc, wsum); break;
+    case 4: lds_radix_pass<4, kT, kRow>(ks, n, S, shift, wc, wsum); break;
+    case 5: lds_radix_pass<5, kT, kRow>(ks, n, S, shift, wc, wsum); break;
+    case 6: lds_radix_pass<6, kT, kRow>(ks, n, S, shift, wc, wsum); break;
+    case 7: lds_radix_pass<7, kT, kRow>(ks, n, S, shift, wc, wsum); break;
+    default:
+      if constexpr (kRow >= 256) lds_radix_pass<8, kT, kRow>(ks, n, S, shift, wc, wsum);
+      break;
   }
 }
 
-// Small tiles and chunks: one workgroup per segment (grid-stride).  Keys are built from the
-// streamed q20 keys; the entries wait in registers during the passes and are then staged in the
-// key array, where each sorted key's low 12 bits index them (small tiles are sorted in place in
-// the entry list, so the source range cannot be read after the first write).
+// Sorts the n (<= 4096) keys of ks in place in LDS by their bits 12 and up, which span qbits bits:
+// ceil(qbits / B) passes of near-equal digit widths (4 to B bits, 2^B = kRow) cover exactly the
+// key span.  Every thread of the kT-thread workgroup must call it; the sorted keys are visible on
+// return.
+template <uint32_t kT, uint32_t kRow>
+__device__ __forceinline__ void lds_sort_span(uint32_t* ks, uint32_t n, uint32_t qbits, uint32_t (*wc)[kRow],
+                                              uint32_t* wsum) {
+  constexpr uint32_t kB = kRow >= 256 ? 8u : 7u;
+  static_assert(kRow == 128 || kRow == 256, "digit rows");
+  const uint32_t Sw = ((n + kT - 1) / kT) * 64;
+  const uint32_t np = (qbits + kB - 1u) / kB;
+  const uint32_t w0 = np ? max((qbits + np - 1u) / np, 4u) : 0u;
+  uint32_t low = 0;
+  for (uint32_t ps = 0; ps < np; ++ps) {  // uniform
+    const uint32_t w = ps + 1u < np ? w0 : max(qbits - low, 4u);
+    radix_pass_w<kT, kRow>(w, ks, n, Sw, 12u + low, wc, wsum);
+    low += w;
+  }
+  if (np == 0) __syncthreads();  // the caller's key writes are read after the return
+}
+
+// One segment (a small tile or a chunk, count <= 4096 entries) sorted by its 20-bit keys with the
+// kT threads of the workgroup: the entries and keys are read into registers, the keys taken
+// relative to the segment's smallest q20 (only the bits of the span are ranked) and sorted in LDS
+// (ka), then the entries are staged in the same array, where each sorted key's low 12 bits index
+// them, and written to entries[dst, dst + count).  src may be that range (a small tile sorts in
+// place: it cannot be read after the first write).  Every thread of the workgroup must call it;
+// the shared arrays may be reused on return.
+template <uint32_t kT, uint32_t kRow>
+__device__ __forceinline__ void sort_segment(const Bufs& b, const uint32_t* src, const uint32_t* qsrc, uint32_t count,
+                                             uint32_t dst_off, uint32_t* ka, uint32_t (*wc)[kRow], uint32_t* wsum,
+                                             uint32_t (*red)[kT / 64]) {
+  constexpr uint32_t kW = kT / 64, kPer = HDSR_MAX_TILE_ENTRIES / kT;
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
+  uint32_t vv[kPer], qq[kPer];
+  uint32_t qmin = 0xFFFFFFFFu, qmax = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kPer; ++r) {
+    const uint32_t i = r * kT + t;
+    vv[r] = 0;
+    qq[r] = 0;
+    if (i < count) {
+      vv[r] = src[i];
+      qq[r] = qsrc[i];
+      qmin = min(qmin, qq[r]);
+      qmax = max(qmax, qq[r]);
+    }
+  }
+  qmin = wave_min(qmin);
+  qmax = wave_max(qmax);
+  if (lane == 0) {
+    red[0][w] = qmin;
+    red[1][w] = qmax;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t q = 0; q < kW; ++q) {
+    qmin = min(qmin, red[0][q]);
+    qmax = max(qmax, red[1][q]);
+  }
+  qmin = __builtin_amdgcn_readfirstlane(qmin);  // uniform: scalar registers
+  qmax = __builtin_amdgcn_readfirstlane(qmax);
+#pragma unroll
+  for (uint32_t r = 0; r < kPer; ++r) {
+    const uint32_t i = r * kT + t;
+    if (i < count) ka[i] = ((qq[r] - qmin) << 12) | i;
+  }
+  const uint32_t span = qmax - qmin;  // uniform; 0 for a segment of one entry or equal keys
+  lds_sort_span<kT, kRow>(ka, count, span ? 32u - (uint32_t)__clz(span) : 0u, wc, wsum);
+  uint32_t sk[kPer];
+#pragma unroll
+  for (uint32_t r = 0; r < kPer; ++r) {
+    const uint32_t i = r * kT + t;
+    sk[r] = i < count ? ka[i] : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t r = 0; r < kPer; ++r) {
+    const uint32_t i = r * kT + t;
+    if (i < count) ka[i] = vv[r];
+  }
+  __syncthreads();
+  uint32_t* dst = b.entries + dst_off;
+#pragma unroll
+  for (uint32_t r = 0; r < kPer; ++r) {
+    const uint32_t i = r * kT + t;
+    if (i < count) {
+      dst[i] = ka[sk[r] & 0xFFFu];
+      if (b.keys20) b.keys20[dst_off + i] = sk[r] + (qmin << 12);
+    }
+  }
+  __syncthreads();
+}
+
+// Small tiles and chunks: one workgroup per segment (grid-stride), kSortThreads threads.
 __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b) {
   __shared__ uint32_t ka[HDSR_MAX_TILE_ENTRIES];
   __shared__ uint32_t wc[kSortWaves][256];
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t red[2][kSortWaves];
-  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
   // round 2 walks the chunks the first round listed (the later chunks of the tiles it left
   // unsaturated); the other rounds walk every segment
   const bool listed = b.round == 2;  // uniform
@@ -269,91 +387,250 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     if (b.round == 1 || b.round == 3) {  // uniform: deferred chunk sort
       bool take;
       if (S.large == kNone) {
-        take = b.round == 1;
-      } else {  // round 1: the first chunk; round 3: every other chunk (completes the lists of a
-                // deferred frame)
-        const bool first = S.dst == b.large_tiles[S.large].start;
-        take = b.round == 1 ? first : !first;
+        take = b.round == 1 && !b.first_chunks;  // with first chunks, k_first_chunks sorts small tiles
+      } else {
+        const LargeTile& L = b.large_tiles[S.large];
+        const bool head = S.dst == L.start;
+        if (b.round == 1)  // the first chunk, unless k_first_chunks sorted it into first[]
+          take = head && !b.first_chunks;
+        else if (b.first_chunks)  // round 3 completes the lists: every chunk of a tile round 2 did
+          take = !L.pending || head;  // not resume, the first chunk of one it did
+        else  // round 3 after a scatter in round 1: every chunk after the first
+          take = !head;
       }
       if (!take) continue;
     }
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
     if (S.count & kSegKeepOrder) continue;  // uniform: copied unsorted by k_segcopy
-    const uint32_t count = S.count;
-    uint32_t vv[kSortPer], qq[kSortPer];
-    uint32_t qmin = 0xFFFFFFFFu, qmax = 0;
+    sort_segment<kSortThreads, 256>(b, src, qsrc, S.count, S.dst, ka, wc, wsum, red);
+  }
+}
+
+// The large tiles in order of decreasing size (by 4096-entry class), for the kernels that give a
+// workgroup to each: with one tile per workgroup the dispatcher then schedules the longest first
+// and the short ones fill in at the end.  One workgroup; the order inside a class is whatever the
+// atomics give (it changes the schedule only, never a result).
+__global__ __launch_bounds__(1024) void k_order_large(Bufs b) {
+  __shared__ uint32_t hist[64];
+  const uint32_t t = threadIdx.x, lane = __lane_id();
+  const uint32_t nl = b.stats->large_tiles;
+  if (t < 64) hist[t] = 0;
+  __syncthreads();
+  for (uint32_t l = t; l < nl; l += 1024) {
+    const uint32_t c = b.large_tiles[l].count >> 12;
+    atomicAdd(&hist[63u - min(c, 63u)], 1u);
+  }
+  __syncthreads();
+  if (t < 64) {
+    const uint32_t v = hist[t];
+    hist[t] = wave_incl_scan(v) - v;
+  }
+  __syncthreads();
+  for (uint32_t l = t; l < nl; l += 1024) {
+    const uint32_t c = b.large_tiles[l].count >> 12;
+    b.large_order[atomicAdd(&hist[63u - min(c, 63u)], 1u)] = l;
+  }
+  (void)lane;
+}
+
+// Round 1 of a deferred frame: the first chunk of large tile l, sorted, into first[l * 4096],
+// without the bucket scatter.  The chunk holds bucket-order positions [0, 4096): every entry of
+// the buckets below b*, the bucket holding position 4095, and the first 'need' entries of b* in
+// list order (the reference's greedy packing splits b* there).  With the kT threads of the
+// workgroup:
+//   pass 1  per-wave bucket counts (wave_bucket_counts), then b*, 'need' and each wave's share;
+//   pass 2  each wave walks its slice again and appends its chunk entries in list order (ranks by
+//           two ballots per round), key (q20 << 12) | j at slot j, the entry at ea[j];
+//   sort    the 4096 keys in LDS relative to the chunk's smallest q20 (as a segment, digits of at
+//           most 7 bits, so the shared arrays leave room for four workgroups per CU).
+// Within the chunk, equal q20 means one bucket, where the bucket order is the list order, so slot
+// j orders ties as the bucket-order position does: the chunk is the one-pass chunk, entry for
+// entry.  The tile's list in entries[] is left canonical for round 2 or 3.
+// Both passes keep the next 256 keys of the slice in flight, and pass 2 stores a block's entries
+// into LDS one block after gathering them.
+constexpr uint32_t kFirstRow = HDSR_NUM_BUCKETS;  // 128 counts per wave: the buckets, then 7-bit digits
+template <uint32_t kT>
+struct FirstLds {
+  uint32_t ka[HDSR_MAX_TILE_ENTRIES];
+  uint32_t ea[HDSR_MAX_TILE_ENTRIES];
+  uint32_t wc[kT / 64][kFirstRow];
+  uint32_t wsum[kT / 64];
+  uint32_t wsel[2][kT / 64];  // per wave: entries of the buckets below b*, entries of b*
+  uint32_t cut[2];            // b*, need
+  uint32_t red[2][kT / 64];
+};
+
+template <uint32_t kT>
+__device__ __forceinline__ void first_chunk_tile(const Bufs& b, uint32_t l, FirstLds<kT>& S_) {
+  constexpr uint32_t kW = kT / 64, kPer = HDSR_MAX_TILE_ENTRIES / kT, kN = HDSR_MAX_TILE_ENTRIES;
+  constexpr uint32_t kRow = kFirstRow;
+  uint32_t* const ka = S_.ka;
+  uint32_t* const ea = S_.ea;
+  uint32_t(*const wc)[kRow] = S_.wc;
+  uint32_t* const wsum = S_.wsum;
+  uint32_t(*const wsel)[kW] = S_.wsel;
+  uint32_t* const cut = S_.cut;
+  uint32_t(*const red)[kW] = S_.red;
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  {
+    const LargeTile L = b.large_tiles[l];
+    const uint32_t* src = b.entries + L.start;
+    const uint32_t* qsrc = b.qkey + L.start;
+    const uint32_t S = ((L.count + kT - 1u) / kT) * 64u;
+    const uint32_t lo = min(w * S, L.count), hi = min(lo + S, L.count);
+    for (uint32_t q = t; q < kW * kRow; q += kT) wc[q / kRow][q % kRow] = 0;
+    if (t == 0) {
+      cut[0] = HDSR_NUM_BUCKETS - 1u;
+      cut[1] = 0u;
+    }
+    __syncthreads();
+    // pass 1: wave-private bucket counts
+    wave_bucket_counts(qsrc, lo, hi, wc[w]);
+    __syncthreads();
+    // b*: the bucket whose bucket-order range holds position 4095 (threads 0..127 own a bucket each)
+    uint32_t tot = 0, inc = 0;
+    if (t < HDSR_NUM_BUCKETS) {
 #pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      vv[r] = 0;
-      qq[r] = 0;
-      if (i < count) {
-        vv[r] = src[i];
-        qq[r] = qsrc[i];
-        qmin = min(qmin, qq[r]);
-        qmax = max(qmax, qq[r]);
+      for (uint32_t q = 0; q < kW; ++q) tot += wc[q][t];
+      inc = tot;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t x = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += x;
+      }
+      if (lane == 63) wsum[w] = inc;
+    }
+    __syncthreads();
+    if (t < HDSR_NUM_BUCKETS) {
+      const uint32_t ex = inc - tot + (w == 1 ? wsum[0] : 0u);
+      if (tot != 0u && ex < kN && kN <= ex + tot) {
+        cut[0] = t;
+        cut[1] = kN - ex;
       }
     }
-    qmin = wave_min(qmin);
-    qmax = wave_max(qmax);
+    __syncthreads();
+    const uint32_t bs = __builtin_amdgcn_readfirstlane(cut[0]);
+    const uint32_t need = __builtin_amdgcn_readfirstlane(cut[1]);
+    {
+      uint32_t below_bs = 0;
+      for (uint32_t k = lane; k < HDSR_NUM_BUCKETS; k += 64)
+        if (k < bs) below_bs += wc[w][k];
+      below_bs = wave_sum(below_bs);
+      if (lane == 0) {
+        wsel[0][w] = below_bs;
+        wsel[1][w] = wc[w][bs];
+      }
+    }
+    __syncthreads();
+    // this wave's first rank among the entries of b*, and its first slot in the chunk
+    uint32_t runb = 0, runs = 0;
+    for (uint32_t q = 0; q < w; ++q) {
+      const uint32_t eq = wsel[1][q];
+      runs += wsel[0][q] + min(eq, need > runb ? need - runb : 0u);
+      runb += eq;
+    }
+    // pass 2: the chunk's entries in list order
+    uint32_t qlo = 0xFFFFFFFFu, qhi = 0;
+    {
+      uint32_t qq[4], qn[4], pj[4], pg[4];
+      bool ps[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t i = lo + u * 64 + lane;
+        qq[u] = i < hi ? qsrc[i] : 0u;
+        ps[u] = false;
+        pj[u] = pg[u] = 0u;
+      }
+      for (uint32_t r0 = lo; r0 < hi; r0 += 256) {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const uint32_t i = r0 + 256 + u * 64 + lane;
+          qn[u] = i < hi ? qsrc[i] : 0u;
+        }
+        uint32_t jj[4];
+        bool sel[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const bool valid = r0 + u * 64 + lane < hi;
+          const uint32_t k = qq[u] >> 13;
+          const bool isb = valid && k == bs;
+          const unsigned long long mb = __ballot(isb);
+          const uint32_t rb = runb + (uint32_t)__popcll(mb & below);
+          runb += (uint32_t)__popcll(mb);
+          sel[u] = valid && (k < bs || (isb && rb < need));
+          const unsigned long long ms = __ballot(sel[u]);
+          jj[u] = runs + (uint32_t)__popcll(ms & below);
+          runs += (uint32_t)__popcll(ms);
+          sel[u] = sel[u] && jj[u] < kN;  // always: pass 1 counted the same keys
+        }
+        // the previous block's entries have had this block's loads and ranks to arrive
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+          if (ps[u]) ea[pj[u]] = pg[u];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          pg[u] = sel[u] ? src[r0 + u * 64 + lane] : 0u;
+          pj[u] = jj[u];
+          ps[u] = sel[u];
+          if (sel[u]) {
+            ka[jj[u]] = (qq[u] << 12) | jj[u];
+            qlo = min(qlo, qq[u]);
+            qhi = max(qhi, qq[u]);
+          }
+          qq[u] = qn[u];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (ps[u]) ea[pj[u]] = pg[u];
+    }
+    qlo = wave_min(qlo);
+    qhi = wave_max(qhi);
     if (lane == 0) {
-      red[0][w] = qmin;
-      red[1][w] = qmax;
+      red[0][w] = qlo;
+      red[1][w] = qhi;
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t q = 0; q < kSortWaves; ++q) {
-      qmin = min(qmin, red[0][q]);
-      qmax = max(qmax, red[1][q]);
+    for (uint32_t q = 0; q < kW; ++q) {
+      qlo = min(qlo, red[0][q]);
+      qhi = max(qhi, red[1][q]);
     }
-    qmin = __builtin_amdgcn_readfirstlane(qmin);  // uniform: scalar registers
-    qmax = __builtin_amdgcn_readfirstlane(qmax);
-    // keys relative to the segment's smallest q20: only the bits of qmax - qmin are ranked
+    const uint32_t qmin = __builtin_amdgcn_readfirstlane(qlo);
+    const uint32_t qmax = __builtin_amdgcn_readfirstlane(qhi);
+    // keys relative to the chunk's smallest q20: only the bits of qmax - qmin are ranked
 #pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      if (i < count) ka[i] = ((qq[r] - qmin) << 12) | i;
-    }
-    const uint32_t span = qmax - qmin;  // uniform; 0 for a segment of one entry or equal keys
-    const uint32_t qbits = span ? 32u - (uint32_t)__clz(span) : 0u;
-    const uint32_t Sw = ((count + kSortThreads - 1) / kSortThreads) * 64;
-    // ceil(qbits / 8) passes of near-equal digit widths (4 to 8 bits) cover exactly the key span
-    const uint32_t np = (qbits + 7u) / 8u;
-    const uint32_t w0 = np ? max((qbits + np - 1u) / np, 4u) : 0u;
-    uint32_t low = 0;
-    for (uint32_t ps = 0; ps < np; ++ps) {  // uniform
-      const uint32_t w = ps + 1u < np ? w0 : max(qbits - low, 4u);
-      radix_pass_w(w, ka, count, Sw, 12u + low, wc, wsum);
-      low += w;
-    }
-    if (np == 0) __syncthreads();  // the keys written above are read below
-    // each sorted key's low 12 bits index the segment's entries: take this thread's sorted keys,
-    // stage the entries (waiting in registers since the first load) in the same array, then
-    // gather them
-    uint32_t sk[kSortPer];
+    for (uint32_t r = 0; r < kPer; ++r) ka[r * kT + t] -= qmin << 12;
+    const uint32_t span = qmax - qmin;
+    lds_sort_span<kT, kRow>(ka, kN, span ? 32u - (uint32_t)__clz(span) : 0u, wc, wsum);
+    uint32_t* dst = b.first + (size_t)l * kN;
 #pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      sk[r] = i < count ? ka[i] : 0u;
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t i = r * kT + t;
+      dst[i] = ea[ka[i] & 0xFFFu];
     }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      if (i < count) ka[i] = vv[r];
+    __syncthreads();  // the next item reuses the shared arrays
+  }
+}
+
+// Round 1 of a deferred frame with first chunks: every large tile's first chunk (largest tiles
+// first, from large_order[]) and then every small tile's sort, one workgroup per item, in one
+// launch: the small tiles fill the CUs the last large tiles leave idle.
+template <uint32_t kT>
+__global__ __launch_bounds__(kT) void k_first_chunks(Bufs b) {
+  __shared__ FirstLds<kT> S_;
+  const uint32_t nl = min(b.stats->large_tiles, b.first_tiles);
+  const uint32_t n = nl + b.stats->small_tiles;
+  for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+    if (k < nl) {
+      first_chunk_tile<kT>(b, __builtin_amdgcn_readfirstlane(b.large_order[k]), S_);
+    } else {
+      const uint32_t m = __builtin_amdgcn_readfirstlane(b.small_list[k - nl]);
+      const uint32_t off = b.tile_offsets[m], cnt = b.tile_counts[m];
+      sort_segment<kT, kFirstRow>(b, b.entries + off, b.qkey + off, cnt, off, S_.ka, S_.wc, S_.wsum, S_.red);
     }
-    __syncthreads();
-    uint32_t* dst = b.entries + S.dst;
-#pragma unroll
-    for (uint32_t r = 0; r < kSortPer; ++r) {
-      const uint32_t i = r * kSortThreads + t;
-      if (i < count) {
-        dst[i] = ka[sk[r] & 0xFFFu];
-        if (b.keys20) b.keys20[S.dst + i] = sk[r] + (qmin << 12);
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -376,6 +653,14 @@ __global__ __launch_bounds__(256) void k_segcopy(Bufs b) {
 
 void launch_bucket(const Bufs& b, hipStream_t stream) {
   if (!b.total_tiles) return;
+  if (b.round == 2) {  // the few tiles the first round left unsaturated
+    const uint32_t grid = b.total_tiles < 256u ? b.total_tiles : 256u;
+    if (b.bucket_wide)
+      hipLaunchKernelGGL(k_bucket<1024>, dim3(grid), dim3(1024), 0, stream, b);
+    else
+      hipLaunchKernelGGL(k_bucket<512>, dim3(grid), dim3(512), 0, stream, b);
+    return;
+  }
   if (b.bucket_wide) {
     const uint32_t grid = b.total_tiles < 1024u ? b.total_tiles : 1024u;
     hipLaunchKernelGGL(k_bucket<1024>, dim3(grid), dim3(1024), 0, stream, b);
@@ -383,6 +668,17 @@ void launch_bucket(const Bufs& b, hipStream_t stream) {
     const uint32_t grid = b.total_tiles < 4096u ? b.total_tiles : 4096u;
     hipLaunchKernelGGL(k_bucket<512>, dim3(grid), dim3(512), 0, stream, b);
   }
+}
+
+void launch_first_chunks(const Bufs& b, hipStream_t stream) {
+  if (!b.total_tiles) return;
+  hipLaunchKernelGGL(k_order_large, dim3(1), dim3(1024), 0, stream, b);
+  // one workgroup per item (large and small tiles together never exceed the tiles)
+  const uint32_t grid = b.total_tiles;
+  if (b.bucket_wide)
+    hipLaunchKernelGGL(k_first_chunks<1024>, dim3(grid), dim3(1024), 0, stream, b);
+  else
+    hipLaunchKernelGGL(k_first_chunks<512>, dim3(grid), dim3(512), 0, stream, b);
 }
 
 void launch_segment_sort(const Bufs& b, hipStream_t stream) {

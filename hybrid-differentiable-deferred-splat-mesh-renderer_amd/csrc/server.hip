@@ -94,6 +94,7 @@ struct hdsr_server {
   double t_first = 0, t_last = 0;
   std::thread worker;
   bool stop = false;
+  bool held = false;  // hdsr_server_hold: no new batch is formed (batches in flight complete)
   std::string err;
 
   int fail(int code, const std::string& m) {
@@ -313,7 +314,7 @@ struct hdsr_server {
       }
       bool launched = false;
       for (auto& sp : slots) {
-        if (queue.empty()) break;
+        if (queue.empty() || (held && !stop)) break;
         if (!sp->busy) {
           launch(*sp);
           launched |= sp->busy;
@@ -446,9 +447,23 @@ int hdsr_server_submit(hdsr_server* s, const hdsr_request* q, uint64_t* ticket) 
   return HDSR_OK;
 }
 
+int hdsr_server_hold(hdsr_server* s, uint32_t hold) {
+  if (!s) return HDSR_E_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->held = hold != 0;
+  }
+  s->cv_work.notify_all();
+  return HDSR_OK;
+}
+
 int hdsr_server_wait(hdsr_server* s, uint64_t t, double timeout_s) {
   if (!s) return HDSR_E_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
+  if (s->held) {  // a held server would never complete a queued ticket
+    s->held = false;
+    s->cv_work.notify_all();
+  }
   auto pred = [&] {
     auto it = s->reqs.find(t);
     return it == s->reqs.end() || it->second.done;
@@ -518,6 +533,10 @@ int hdsr_server_release(hdsr_server* s, uint64_t t) {
 int hdsr_server_drain(hdsr_server* s) {
   if (!s) return HDSR_E_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
+  if (s->held) {
+    s->held = false;
+    s->cv_work.notify_all();
+  }
   s->cv_done.wait(lk, [&] { return s->pending == 0; });
   return HDSR_OK;
 }

@@ -27,6 +27,7 @@ HDSR_DEBUG = 8
 HDSR_OUT_RGBA16 = 16
 HDSR_PROFILE = 32
 HDSR_DIFFERENTIABLE = 64
+HDSR_BUCKET_SCATTER = 128
 STAGES = ["project", "scan", "emit", "group", "ranges", "classify", "bucket", "sort", "composite"]
 
 HDSR_MEM_HOST = 0
@@ -183,6 +184,7 @@ SIGNATURES = [
     ("hdsr_server_read", C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_uint64]),
     ("hdsr_server_release", C.c_int, [C.c_void_p, C.c_uint64]),
     ("hdsr_server_drain", C.c_int, [C.c_void_p]),
+    ("hdsr_server_hold", C.c_int, [C.c_void_p, C.c_uint32]),
     ("hdsr_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     ("hdsr_server_reset_stats", C.c_int, [C.c_void_p]),
     ("hdsr_server_last_error", C.c_char_p, [C.c_void_p]),

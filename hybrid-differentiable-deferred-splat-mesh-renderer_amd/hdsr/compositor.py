@@ -196,7 +196,7 @@ class Resources:
 
 
 def createTileSplatCompositor(device, plan: dict, options: dict | None = None, *, gbuffer=False, debug=False,
-                              rgba16=False, profile=False) -> Resources:
+                              rgba16=False, profile=False, bucket_scatter=False) -> Resources:
     options = options or {}
     flags = 0
     if options.get("f16"):
@@ -211,6 +211,8 @@ def createTileSplatCompositor(device, plan: dict, options: dict | None = None, *
         flags |= _lib.HDSR_OUT_RGBA16
     if profile or options.get("profile"):
         flags |= _lib.HDSR_PROFILE
+    if bucket_scatter or options.get("bucketScatter"):
+        flags |= _lib.HDSR_BUCKET_SCATTER
     return Resources(plan, flags)
 
 

@@ -110,6 +110,11 @@ class Server:
     def release(self, ticket: int):
         self._check(self._lib.hdsr_server_release(self.handle, ticket))
 
+    def hold(self, on: bool = True):
+        """While held, no new batch is formed; a burst submitted meanwhile shares launches
+        (wait and drain release the hold)."""
+        self._check(self._lib.hdsr_server_hold(self.handle, 1 if on else 0))
+
     def drain(self):
         self._check(self._lib.hdsr_server_drain(self.handle))
 

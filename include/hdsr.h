@@ -76,6 +76,11 @@ int hdsr_plan_compute(const hdsr_plan_in* in, hdsr_plan* out);
                                    the rasterizer reads fp32 colour / opacity (no f16 cache
                                    rounding) and keeps per-pixel state for the backward; not
                                    combinable with HDSR_GBUFFER or HDSR_F16_COMPOSITE */
+#define HDSR_BUCKET_SCATTER 128u /* frames that sort only the chunks their pixels reach keep the
+                                   whole bucket scatter of the large tiles in their first round
+                                   (by default a frame after one that resumed less than half of its
+                                   large-tile references selects each large tile's first chunk
+                                   without it); lists, order and pixels are the same either way */
 
 typedef struct hdsr_ctx hdsr_ctx;
 
@@ -410,6 +415,10 @@ void hdsr_server_destroy(hdsr_server* srv);
 int hdsr_server_add_scene(hdsr_server* srv, uint32_t scene_id, const hdsr_splats* splats);
 int hdsr_server_remove_scene(hdsr_server* srv, uint32_t scene_id);
 int hdsr_server_submit(hdsr_server* srv, const hdsr_request* req, uint64_t* ticket);
+/* hold != 0: the worker forms no new batch (batches in flight complete) until hold = 0, so a
+ * burst of requests submitted meanwhile shares launches; hdsr_server_wait and hdsr_server_drain
+ * release a hold.  No counterpart in the reference. */
+int hdsr_server_hold(hdsr_server* srv, uint32_t hold);
 /* Blocks until the ticket's frame is complete (timeout_s <= 0: no limit). */
 int hdsr_server_wait(hdsr_server* srv, uint64_t ticket, double timeout_s);
 int hdsr_server_result(hdsr_server* srv, uint64_t ticket, hdsr_result_info* info);

@@ -4,7 +4,10 @@ rasterizer stops a large tile at the end of that chunk, and only the tiles left 
 get their other chunks sorted and are resumed from their saved pixel state.  The image must equal
 the one-pass render's (a debug context sorts every chunk) bit for bit, and the oracle's; so must
 encodeCompositeOnly afterwards.  Two opacity levels: tiles that saturate inside their first chunk,
-and tiles that never saturate (every large tile resumed)."""
+and tiles that never saturate (every large tile resumed).  Two ways to the first chunks: selected
+from each large tile's bucket counts without the bucket scatter (the default after a frame that
+resumed less than half of its large-tile references, so the faint scene's later frames on the
+same context switch to the other way), and the whole scatter (HDSR_BUCKET_SCATTER)."""
 import numpy as np
 import pytest
 
@@ -16,16 +19,17 @@ from test_gpu_parity import render_gpu
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("scatter", [False, True])
 @pytest.mark.parametrize("faint", [False, True])
 @pytest.mark.parametrize("f16", [False, True])
-def test_deferred_equals_one_pass_and_oracle(hip_device, faint, f16):
+def test_deferred_equals_one_pass_and_oracle(hip_device, faint, f16, scatter):
     s = dense_scene(n=40_000, seed=17, sh_degree=3, core_fraction=0.6)
     if faint:
         s.opacities *= np.float32(0.05)
     cam = camera.Camera(camera.orbit_eye(2.4, 0.4, 0.25), (0, 0, 0), 420.0, 420.0, 384, 256)
     plan, srt, u = frame_inputs(s, cam)
     full = render_gpu(s, srt, u, plan, flags_debug=True, f16=f16)
-    lazy = render_gpu(s, srt, u, plan, flags_debug=False, f16=f16)
+    lazy = render_gpu(s, srt, u, plan, flags_debug=False, f16=f16, bucket_scatter=scatter)
     sf, sl = full.stats(), lazy.stats()
     assert sf["chunks"] > sf["large_tiles"] > 0  # large tiles with more than one chunk
     if faint:

@@ -17,8 +17,9 @@ pytestmark = pytest.mark.gpu
 PIXEL_TOL = 1e-5
 
 
-def render_gpu(scene, srt, u, plan, flags_debug=True, gbuffer=False, f16=False):
-    res = compositor.createTileSplatCompositor(None, plan, {"f16": f16}, debug=flags_debug, gbuffer=gbuffer)
+def render_gpu(scene, srt, u, plan, flags_debug=True, gbuffer=False, f16=False, bucket_scatter=False):
+    res = compositor.createTileSplatCompositor(None, plan, {"f16": f16}, debug=flags_debug, gbuffer=gbuffer,
+                                               bucket_scatter=bucket_scatter)
     compositor.bind_scene(res, scene, srt)
     res.set_frame(u)
     compositor.encodeFullComputeCompositorPipeline(None, res)
@@ -68,21 +69,25 @@ def compare_all(res, orc, plan, gbuffer=False, exact=False):
 
 
 def check_deferred(scene, srt, u, plan, one_pass, orc=None, f16=False):
-    """Renders the frame on a context with no debug readback and no G-buffer, which runs the
+    """Renders the frame on contexts with no debug readback and no G-buffer, which run the
     deferred chunk sort (first chunks, rasterize, sort the resumed tiles' other chunks, resume),
-    and requires its image and depth to equal the one-pass context's bit for bit (and the oracle's
-    within 1e-5 when given); the entry lists it completes on readback equal the one-pass lists."""
-    lazy = render_gpu(scene, srt, u, plan, flags_debug=False, f16=f16)
-    rgba, depth = lazy.read_rgba(), lazy.read_depth()
-    np.testing.assert_array_equal(rgba, one_pass.read_rgba())
-    np.testing.assert_array_equal(depth, one_pass.read_depth())
-    if orc is not None:
-        assert np.abs(rgba.reshape(-1) - orc["rgba"]).max() <= PIXEL_TOL
-        assert np.abs(depth.reshape(-1) - orc["out_depth"]).max() <= PIXEL_TOL
-    sl, sf = lazy.stats(), one_pass.stats()
-    assert sl["refs"] == sf["refs"] and sl["composite_refs"] == sf["composite_refs"]
-    np.testing.assert_array_equal(lazy.debug("entries"), one_pass.debug("entries"))
-    lazy.destroy()
+    once selecting each large tile's first chunk without the bucket scatter (a fresh context's
+    default) and once with the scatter (HDSR_BUCKET_SCATTER), and requires each image and depth to
+    equal the one-pass context's bit for bit (and the oracle's within 1e-5 when given); the entry
+    lists each completes on readback equal the one-pass lists."""
+    sf = one_pass.stats()
+    for scatter in (False, True):
+        lazy = render_gpu(scene, srt, u, plan, flags_debug=False, f16=f16, bucket_scatter=scatter)
+        rgba, depth = lazy.read_rgba(), lazy.read_depth()
+        np.testing.assert_array_equal(rgba, one_pass.read_rgba())
+        np.testing.assert_array_equal(depth, one_pass.read_depth())
+        if orc is not None:
+            assert np.abs(rgba.reshape(-1) - orc["rgba"]).max() <= PIXEL_TOL
+            assert np.abs(depth.reshape(-1) - orc["out_depth"]).max() <= PIXEL_TOL
+        sl = lazy.stats()
+        assert sl["refs"] == sf["refs"] and sl["composite_refs"] == sf["composite_refs"]
+        np.testing.assert_array_equal(lazy.debug("entries"), one_pass.debug("entries"))
+        lazy.destroy()
     return sl
 
 

@@ -78,7 +78,8 @@ def test_batched_equals_alone(workload, graphs, budget):
     srv = Server(hbm_budget_bytes=budget, max_views_per_batch=64, slots=2, use_graphs=graphs)
     for sid, s in sc.items():
         srv.add_scene(sid, s)
-    for rep in range(2):  # the second round replays captured graphs
+    for rep in range(2):  # the second round meets the first round's batch shapes again
+        srv.hold()  # the burst queues whole, so batches form whatever the submit timing
         tickets = [srv.submit(sid, cam) for sid, cam in reqs]
         srv.drain()
         views = []
@@ -156,6 +157,7 @@ def test_serving_scale_mixed_resolutions(hip_device):
     srv = Server(max_views_per_batch=64, slots=2, use_graphs=True)
     for sid, s in sc.items():
         srv.add_scene(sid, s)
+    srv.hold()  # the six requests queue together, so they share launches
     tickets = [srv.submit(sid, cam) for sid, cam in reqs]
     srv.drain()
     batched = 0
