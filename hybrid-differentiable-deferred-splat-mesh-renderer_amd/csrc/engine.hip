@@ -87,7 +87,7 @@ void Engine::release() {
                   b_.scratch, b_.qkey, b_.qkey_s, b_.tile_dmin,
                   b_.radix_hist, b_.radix_tmp, b_.tile_offsets, b_.tile_counts, b_.small_list, b_.large_list_t,
                   b_.large_bases, b_.large_tiles, b_.segs, b_.stats, b_.tile_large, b_.pend_tiles, b_.pend_segs, b_.px_state,
-                  b_.pend_large, b_.first, b_.large_order,
+                  b_.pend_large, b_.first, b_.large_order, b_.pend_aux,
                   d_frames_};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -257,6 +257,8 @@ int Engine::ensure(const Layout& L, bool debug) {
     if ((rc = grow(&b_.pend_large, &have, L.total_tiles))) return rc;
     have = n_tiles_;
     if ((rc = grow(&b_.large_order, &have, L.total_tiles))) return rc;
+    have = n_tiles_ * 2;
+    if ((rc = grow(&b_.pend_aux, &have, (size_t)L.total_tiles * 2))) return rc;
     have = n_tiles_ * 256 * 6;
     if ((rc = grow(&b_.px_state, &have, (size_t)L.total_tiles * 256 * 6))) return rc;
     n_tiles_ = L.total_tiles;

@@ -36,6 +36,9 @@ constexpr uint32_t kMaxFrames = 64;
 constexpr uint32_t kCellTiles = 8;            // a cell is 8 x 8 tiles
 constexpr uint32_t kCellChunk = 1024;         // cell-list entries per expansion workgroup
 constexpr uint32_t kMaxCellsPerSplat = 9;     // a tile box spans at most 17 tiles per axis
+// A deferred frame's resumed tiles: entries before this list index get their quadrant masks from
+// k_resume_masks (a multiple of 4096 past the first chunk); the rest are tested by the rasterizer.
+#define HDSR_RESUME_MASKED (4u * HDSR_MAX_TILE_ENTRIES)
 
 // Floats per splat of a shData row at SH degree deg: dc (3) + 3K coefficients + emissive (3),
 // K = (deg+1)^2 - 1 (gpu_project_splats.wgsl:294-296).
@@ -152,7 +155,8 @@ struct LargeTile {
   uint32_t scratch;  // first slot in scratch[]
   uint32_t seg;      // the segment of its first chunk (the others follow it)
   uint32_t pending;  // deferred frames: 1 once the first round left the tile unsaturated (resumed in round 2)
-  uint32_t pad2, pad3;
+  uint32_t tile;     // global tile index
+  uint32_t pad3;
 };
 
 // Device buffers of one batch (all owned by the host context).
@@ -197,7 +201,9 @@ struct Bufs {
   uint32_t* keys20;          // [cap] (q20 << 12) | local index, or null
   uint32_t* entries_canon;   // [cap] entries before the per-tile sort, or null
   uint32_t* scratch;         // [cap] bucket-ordered copies of large tiles
-  uint32_t* qkey;            // [cap] q20 key of each entry in its tile's log-depth frame (same order as entries)
+  uint32_t* qkey;            // [cap] q20 key of each entry in its tile's log-depth frame (same order as entries);
+                             //   in a deferred frame's round 2, a resumed tile's later chunks hold each
+                             //   sorted entry's reach_quadrants mask instead (the keys are dead by then)
   uint32_t* qkey_s;          // [cap] q20 key of each scratch entry
   uint32_t* tile_dmin;       // [total_morton] min / max depth bits over each tile's entries (u32 order,
   uint32_t* tile_dmax;       //   as the reference's atomicMin / atomicMax)
@@ -233,6 +239,7 @@ struct Bufs {
   uint32_t* pend_tiles;      // [total_tiles] deferred frames: global tile indices resumed in round 2
   uint32_t* pend_segs;       // [seg capacity] deferred frames: the segments round 2 sorts
   uint32_t* pend_large;      // [total_tiles] deferred frames: large-tile index of each pend_tiles[] entry
+  uint32_t* pend_aux;        // [2 * total_tiles] deferred frames: round 2's loaded entries and finished quadrants per pend_tiles[] entry
   uint32_t* first;           // [first_tiles * 4096] deferred frames: large tile l's sorted first chunk at l * 4096
   uint32_t* large_order;     // [total_tiles] large-tile indices by decreasing size (k_first_chunks' order)
   uint32_t first_tiles;      // large tiles first[] holds

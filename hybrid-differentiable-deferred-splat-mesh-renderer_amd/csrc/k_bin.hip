@@ -838,14 +838,15 @@ __global__ void k_classify_count(Bufs b, uint4* cls) {
 }
 
 // One tile's list entries and segments, at its class prefix p.
-__device__ __forceinline__ void write_class(const Bufs& b, uint32_t m, uint32_t s, uint32_t c, uint4 k, uint4 p) {
+__device__ __forceinline__ void write_class(const Bufs& b, uint32_t t, uint32_t m, uint32_t s, uint32_t c, uint4 k,
+                                            uint4 p) {
   if (k.x) {
     b.small_list[p.x] = m;
     b.segs[p.z] = Seg{s, s, c, 0xFFFFFFFFu};
   } else if (k.y) {
     b.large_list_t[p.y] = m;
     b.large_bases[p.y] = (b.ref_caps ? b.cap : b.stats->refs) + p.w;  // capacity + earlier large counts
-    b.large_tiles[p.y] = LargeTile{m, s, c, p.w, p.z, 0u, 0u, 0u};
+    b.large_tiles[p.y] = LargeTile{m, s, c, p.w, p.z, 0u, t, 0u};
     b.tile_large[m] = p.y;
     for (uint32_t q = 0; q < k.z; ++q) {
       const uint32_t off = q * HDSR_MAX_TILE_ENTRIES;
@@ -872,7 +873,7 @@ __global__ void k_classify_write(Bufs b, const uint4* pre) {
     b.stats->chunks = p.z + k.z - (p.x + k.x);
     b.stats->scratch_used = p.w + k.w;
   }
-  write_class(b, m, s, c, k, p);
+  write_class(b, t, m, s, c, k, p);
 }
 
 }  // namespace

@@ -10,6 +10,7 @@
 // quadrant) walks only the entries whose conservative box reaches its quadrant, a wave whose 64
 // pixels are all saturated skips the batch, and the workgroup stops once all four are.  From
 // saturation on no splat can pass the 'T >= 0.001' gate, so the output equals the full walk.
+// Pixels outside the viewport (the last tile row or column) count as saturated from the start.
 //
 // Per pixel and splat (fp32, no contraction, exp from hdsr_fmath.h), with the reference's quad
 // arithmetic kept: d = p00 - centre, plus 1 for the odd column / row of the quad:
@@ -265,7 +266,7 @@ struct TileLds {
 };
 
 // One tile (global index t, uniform) with the workgroup's shared arrays L; the records of kDepth
-// batches ahead are in flight (1 where many tiles share a CU, more where a tile has it alone).
+// batches ahead are in flight (the launches use 1: many tiles share a CU).
 template <bool kG, bool kD, bool kH, uint32_t kDepth>
 __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, TileLds<kG, kD>& L) {
   float4* const s_a = L.s_a;
@@ -287,18 +288,13 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
   const uint32_t full = b.tile_counts[m];
   const uint32_t tid = threadIdx.x;
   // Deferred chunk sort: round 1 stops a large tile at the end of its first chunk (the only one
-  // sorted yet); round 2 resumes the tiles left unsaturated there (the tiles round 1 listed)
-  // from their saved state.  The batches are those of one pass (4096 is a multiple of kBatch),
-  // so the pixels are the same.
-  // (never with the G-buffer or the differentiable records: those frames run in one pass)
-  const bool deferrable = !kG && !kD && b.round != 0 && full > HDSR_MAX_TILE_ENTRIES;  // uniform
-  uint32_t count = full, base0 = 0;
-  if (b.round == 2) {
-    if (!deferrable) return;
-    base0 = HDSR_MAX_TILE_ENTRIES;
-  } else if (deferrable) {
-    count = HDSR_MAX_TILE_ENTRIES;
-  }
+  // sorted yet); k_composite_resume resumes the tiles left unsaturated there from their saved
+  // state.  The batches are those of one pass (4096 is a multiple of kBatch), so the pixels are
+  // the same.  (Never with the G-buffer or the differentiable records: those frames run in one
+  // pass.)
+  const bool deferrable = !kG && !kD && b.round == 1 && full > HDSR_MAX_TILE_ENTRIES;  // uniform
+  const uint32_t count = deferrable ? HDSR_MAX_TILE_ENTRIES : full;
+  const uint32_t base0 = 0;
   float* const pst = b.px_state + (size_t)t * (256 * 6) + tid;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -312,8 +308,9 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
 
   using PxT = typename std::conditional<kH, Ph, Px>::type;
   PxT p{};
-  p.T = 1.0f;
-  if (base0) load_px(pst, p);
+  // a pixel outside the viewport is never stored: it starts saturated, so it never keeps the
+  // tile's walk going (with T = 0 no entry passes the gate, and the stored pixels do not depend on it)
+  p.T = (X < F.width && Y < F.height) ? 1.0f : 0.0f;
   uint32_t last = 0;
   Gb q = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t loaded = 0;
@@ -468,6 +465,8 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
       const uint32_t k = atomicAdd(&b.stats->pend_tiles, 1u);
       b.pend_tiles[k] = t;
       b.pend_large[k] = l;
+      b.pend_aux[2 * k] = 0u;  // round 2's per-tile count of loaded entries and finished quadrants
+      b.pend_aux[2 * k + 1] = 0u;
       b.large_tiles[l].pending = 1u;
       const uint32_t seg0 = b.large_tiles[l].seg;
       const uint32_t later = (full - 1u) / HDSR_MAX_TILE_ENTRIES;
@@ -487,7 +486,7 @@ __device__ __forceinline__ void composite_tile(const Bufs& b, const uint32_t t, 
   }
   if (tid == 0 && b.stats) {
     atomicAdd(&b.stats->composite_refs, loaded);
-    atomicMax(&b.stats->composite_max_refs, base0 + loaded);
+    atomicMax(&b.stats->composite_max_refs, loaded);
   }
 }
 
@@ -498,18 +497,164 @@ __global__ __launch_bounds__(256) void k_composite(Bufs b) {
   composite_tile<kG, kD, kH, 1>(b, blockIdx.x, L);
 }
 
-// Round 2 of a deferred frame (no G-buffer, no differentiable records): the tiles round 1 listed,
-// grid-stride.  A few tiles run alone on their CUs, so each keeps several batches of record
-// gathers in flight instead of one.
-constexpr uint32_t kResumeDepth = 4;
+// Round 2 of a deferred frame (no G-buffer, no differentiable records): the tiles round 1 left
+// unsaturated, resumed from their saved state.  There are few (tens per 1080p frame) and each is
+// a chain of dependent batches, so a tile gets four workgroups of one wave, one per 8x8 quadrant:
+// the quadrants run on different CUs, need no workgroup barrier, and each stops when its own 64
+// pixels saturate.  k_resume_masks wrote each entry's reach_quadrants mask beside it (in qkey) up
+// to list index HDSR_RESUME_MASKED, so there a quadrant reads a batch's 256 entries and masks
+// (four per lane, coalesced) and gathers only the records of the entries that may reach it (a
+// lane with nothing to gather reads the first record, one line for all of them); past it, the
+// quadrant gathers every record and tests it (reach_quadrant).  Records are in flight kResumeDepth
+// batches ahead, entries and masks twice that.  The kept entries are staged in order and walked
+// with the arithmetic of round 1, so every pixel is the same.  The tile's count of loaded entries is the largest of its
+// quadrants' (whole batches up to the batch where its last pixel saturates), as one workgroup
+// counts it.
+constexpr uint32_t kResumeDepth = 2;
 template <bool kH>
-__global__ __launch_bounds__(256) void k_composite_resume(Bufs b) {
-  __shared__ TileLds<false, false> L;
-  const uint32_t n = b.stats->pend_tiles;
-  for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
-    composite_tile<false, false, kH, kResumeDepth>(b, __builtin_amdgcn_readfirstlane(b.pend_tiles[k]), L);
-    __syncthreads();  // the next tile restages the shared arrays
+__device__ __forceinline__ void resume_quadrant(const Bufs& b, const uint32_t k, const uint32_t t, const uint32_t qd,
+                                                float4* s_a, float4* s_b, float4* s_c) {
+  const uint32_t f = frame_of(b.tile_frame_start, b.num_frames, t);
+  const FrameDesc& F = b.frames[f];
+  const uint32_t lt = t - F.tile_base;
+  const uint32_t tx = lt % F.tiles_x, ty = lt / F.tiles_x;
+  const uint32_t m = F.morton_base + hdsr_morton2(tx, ty);
+  const uint32_t start = b.tile_offsets[m];
+  const uint32_t count = b.tile_counts[m];  // > 4096: round 1 stopped at 4096
+  const uint32_t lane = threadIdx.x;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t lx = (qd & 1u) * 8u + (lane & 7u), ly = (qd >> 1) * 8u + (lane >> 3);
+  const uint32_t X = tx * 16 + lx, Y = ty * 16 + ly;
+  const float p00x = (float)(X & ~1u) + 0.5f, p00y = (float)(Y & ~1u) + 0.5f;
+  const float add_x = (lx & 1u) ? 1.0f : 0.0f, add_y = (ly & 1u) ? 1.0f : 0.0f;
+  using PxT = typename std::conditional<kH, Ph, Px>::type;
+  PxT p{};
+  load_px(b.px_state + (size_t)t * (256 * 6) + qd * 64 + lane, p);
+  Gb gq = Gb{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const uint4* const rec = b.hot;
+  const uint32_t* const ent = b.entries + start;
+  const uint32_t* const msk = b.qkey + start;
+  const uint32_t last_e = count - 1u;
+  constexpr uint32_t kD = kResumeDepth;
+  // slot s: the kept entries (bit u: this lane's entry u) and their records, of the batches s
+  // (mod kD); the entries and masks of the batch kD later
+  uint32_t kq[kD], ga[kD][4], ma[kD][4];
+  uint4 h0q[kD][4], h1q[kD][4];
+  uint32_t base = HDSR_MAX_TILE_ENTRIES;
+#pragma unroll
+  for (uint32_t s = 0; s < kD; ++s) {
+    uint32_t gg[4], mm[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t id = base + s * kBatch + u * 64 + lane;
+      gg[u] = gload(ent + min(id, last_e));
+      mm[u] = gload(msk + min(id, last_e));
+      ga[s][u] = gload(ent + min(id + kD * kBatch, last_e));
+      ma[s][u] = gload(msk + min(id + kD * kBatch, last_e));
+    }
+    const bool masked = base + s * kBatch < HDSR_RESUME_MASKED;  // uniform
+    kq[s] = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const bool keep = base + s * kBatch + u * 64 + lane < count && (!masked || ((mm[u] >> qd) & 1u));
+      kq[s] |= keep ? 1u << u : 0u;
+      const size_t r = keep ? (size_t)gg[u] * 2 : 0;
+      h0q[s][u] = gload4(rec + r);
+      h1q[s][u] = gload4(rec + r + 1);
+    }
   }
+  const float tile_x0 = (float)(tx * 16) + 0.5f, tile_y0 = (float)(ty * 16) + 0.5f;
+  uint32_t loaded = 0;
+  bool finished = false;
+  while (base < count && !finished) {  // uniform
+#pragma unroll
+    for (uint32_t s = 0; s < kD; ++s) {
+      if (base >= count) break;  // uniform
+      // stage the kept entries in list order (past the masked range, the entries that pass the
+      // quadrant's own test)
+      const bool masked = base < HDSR_RESUME_MASKED;  // uniform
+      uint32_t nk = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint4 h0 = h0q[s][u], h1 = h1q[s][u];
+        const bool keep = ((kq[s] >> u) & 1u) &&
+                          (masked || reach_quadrant(qd, hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z),
+                                                    hdsr_u2f(h0.w), hdsr_u2f(h1.x), tile_x0, tile_y0));
+        const unsigned long long km = __ballot(keep);
+        if (keep) {
+          const uint32_t slot = nk + (uint32_t)__popcll(km & below);
+          s_a[slot] = make_float4(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z) * -0.5f, -hdsr_u2f(h0.w));
+          s_b[slot] = make_float4(h16(h1.w >> 16), h16(h1.z), h16(h1.z >> 16), hdsr_u2f(h1.x) * -0.5f);
+          s_c[slot] = make_float4(h16(h1.w), hdsr_u2f(h1.y), 0.0f, 0.0f);
+        }
+        nk += (uint32_t)__popcll(km);
+      }
+      // slot s takes the batch kD later: its kept entries' records, and the entries and masks of
+      // the batch after that
+      {
+        const uint32_t b1 = base + kD * kBatch, b2 = base + 2 * kD * kBatch;
+        const bool masked1 = b1 < HDSR_RESUME_MASKED;  // uniform
+        kq[s] = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const bool keep = b1 + u * 64 + lane < count && (!masked1 || ((ma[s][u] >> qd) & 1u));
+          kq[s] |= keep ? 1u << u : 0u;
+          const size_t r = keep ? (size_t)ga[s][u] * 2 : 0;
+          h0q[s][u] = gload4(rec + r);
+          h1q[s][u] = gload4(rec + r + 1);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          ga[s][u] = gload(ent + min(b2 + u * 64 + lane, last_e));
+          ma[s][u] = gload(msk + min(b2 + u * 64 + lane, last_e));
+        }
+      }
+      loaded += min(kBatch, count - base);
+      __syncthreads();  // one wave: orders the staging before the walk
+      for (uint32_t j = 0; j < nk; j += 2) {
+        if ((j & 30u) == 30u && __all(saturated(p))) break;  // quadrant saturated
+        const bool hb = j + 1u < nk;  // uniform
+        const uint32_t oa = j * 16u, ob = hb ? oa + 16u : oa;
+        const float4 Aa = rec16(s_a, oa), Ba = rec16(s_b, oa);
+        const float4 Ab = rec16(s_a, ob), Bb = rec16(s_b, ob);
+        const float dxa = (p00x - Aa.x) + add_x, dya = (p00y - Aa.y) + add_y;
+        const float dxb = (p00x - Ab.x) + add_x, dyb = (p00y - Ab.y) + add_y;
+        const float pwa = (Aa.z * dxa * dxa + Aa.w * dxa * dya) + Ba.w * dya * dya;
+        const float pwb = (Ab.z * dxb * dxb + Ab.w * dxb * dyb) + Bb.w * dyb * dyb;
+        const bool ra = __any(pwa > -4.0f), rb = hb && __any(pwb > -4.0f);  // uniform
+        if (!ra && !rb) continue;
+        const float ala = alpha_of(pwa, Ba.x), alb = alpha_of(pwb, Bb.x);
+        if (ra) blend_slot<false, false>(p, gq, pwa, ala, Ba, oa, s_c, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (rb) blend_slot<false, false>(p, gq, pwb, alb, Bb, ob, s_c, nullptr, nullptr, nullptr, nullptr, nullptr);
+      }
+      __syncthreads();  // the next batch restages the shared arrays
+      if (__all(saturated(p))) {
+        finished = true;
+        break;
+      }
+      base += kBatch;
+    }
+  }
+  store_px<false>(F, X, Y, p, gq);
+  if (lane == 0 && b.stats) {
+    // the tile's loaded entries: the largest of its quadrants', added by the last to finish
+    atomicMax(&b.pend_aux[2 * k], loaded);
+    __threadfence();
+    if (atomicAdd(&b.pend_aux[2 * k + 1], 1u) == 3u) {
+      __threadfence();
+      const uint32_t most = atomicMax(&b.pend_aux[2 * k], 0u);
+      atomicAdd(&b.stats->composite_refs, most);
+      atomicMax(&b.stats->composite_max_refs, HDSR_MAX_TILE_ENTRIES + most);
+    }
+  }
+}
+
+template <bool kH>
+__global__ __launch_bounds__(64) void k_composite_resume(Bufs b) {
+  __shared__ float4 s_a[kBatch], s_b[kBatch], s_c[kBatch];
+  const uint32_t n = 4u * b.stats->pend_tiles;
+  for (uint32_t k = blockIdx.x; k < n; k += gridDim.x)
+    resume_quadrant<kH>(b, k >> 2, __builtin_amdgcn_readfirstlane(b.pend_tiles[k >> 2]), k & 3u, s_a, s_b, s_c);
 }
 
 __global__ void k_reset_pending(DevStats* s) {
@@ -527,12 +672,13 @@ void launch_composite(const Bufs& b, hipStream_t stream, bool gbuffer, bool f16)
   if (!b.total_tiles) return;
   const dim3 block(256);
   if (b.round == 2) {  // deferred frames only: no G-buffer, no differentiable records
-    // the listed tiles only: a grid that holds them at once, without a workgroup per tile
-    const dim3 grid(b.total_tiles > 2048u ? 2048u : b.total_tiles);
+    // four one-wave workgroups per listed tile: a grid that holds them at once, without one per tile
+    const uint32_t q = 4u * b.total_tiles;
+    const dim3 grid(q > 4096u ? 4096u : q);
     if (f16)
-      hipLaunchKernelGGL(k_composite_resume<true>, grid, block, 0, stream, b);
+      hipLaunchKernelGGL(k_composite_resume<true>, grid, dim3(64), 0, stream, b);
     else
-      hipLaunchKernelGGL(k_composite_resume<false>, grid, block, 0, stream, b);
+      hipLaunchKernelGGL(k_composite_resume<false>, grid, dim3(64), 0, stream, b);
     return;
   }
   const dim3 grid(b.total_tiles);

@@ -28,6 +28,7 @@
 // per splat by the projection, bln[].y), so these kernels read one 4-byte key stream and do no log
 // arithmetic.
 #include "hdsr_internal.h"
+#include "raster_common.h"
 
 namespace hdsr {
 namespace {
@@ -634,6 +635,32 @@ __global__ __launch_bounds__(kT) void k_first_chunks(Bufs b) {
   }
 }
 
+// Round 2 of a deferred frame, after the sort of the resumed tiles' later chunks: each of their
+// entries' reach_quadrants mask in its tile, into qkey beside it (those keys are dead once the
+// chunks are sorted), for the entries before HDSR_RESUME_MASKED in the tile's list.  The resumed
+// rasterizer then gathers only the records of the entries that may reach its quadrant; past that
+// bound (a tile that stays unsaturated that long) it tests the records itself, so a frame with
+// very long resumed tiles does not test entries its pixels never reach.  One thread per entry:
+// item (segment k, 256-entry part j).
+__global__ __launch_bounds__(256) void k_resume_masks(Bufs b) {
+  const uint32_t n = b.stats->pend_segs * (HDSR_MAX_TILE_ENTRIES / 256u);
+  for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
+    const Seg S = b.segs[b.pend_segs[it / (HDSR_MAX_TILE_ENTRIES / 256u)]];
+    const uint32_t i = (it % (HDSR_MAX_TILE_ENTRIES / 256u)) * 256u + threadIdx.x;
+    if (i >= S.count) continue;
+    const LargeTile& L = b.large_tiles[S.large];
+    if (S.dst - L.start >= HDSR_RESUME_MASKED) continue;  // uniform: the segment's first entry
+    const uint32_t t = L.tile;
+    const FrameDesc& F = b.frames[frame_of(b.tile_frame_start, b.num_frames, t)];
+    const uint32_t lt = t - F.tile_base;
+    const float x0 = (float)((lt % F.tiles_x) * 16u) + 0.5f, y0 = (float)((lt / F.tiles_x) * 16u) + 0.5f;
+    const uint32_t g = b.entries[S.dst + i];
+    const uint4 h0 = b.hot[(size_t)g * 2];
+    const float k2 = hdsr_u2f(b.hot[(size_t)g * 2 + 1].x);
+    b.qkey[S.dst + i] = reach_quadrants(hdsr_u2f(h0.x), hdsr_u2f(h0.y), hdsr_u2f(h0.z), hdsr_u2f(h0.w), k2, x0, y0);
+  }
+}
+
 // With the reference's caps: the chunks past the first 1,024 keep their bucket order; they are
 // copied from the scratch to their place in the entry list, each key holding its position.
 __global__ __launch_bounds__(256) void k_segcopy(Bufs b) {
@@ -688,6 +715,8 @@ void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   const uint32_t cap = b.round == 2 ? 1024u : 2048u * 1024u / kSortThreads;
   const uint32_t grid = b.seg_cap < cap ? b.seg_cap : cap;
   hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
+  if (b.round == 2)  // the resumed entries' quadrant masks
+    hipLaunchKernelGGL(k_resume_masks, dim3(4096), dim3(256), 0, stream, b);
   if (b.ref_caps) hipLaunchKernelGGL(k_segcopy, dim3(grid), dim3(256), 0, stream, b);
 }
 

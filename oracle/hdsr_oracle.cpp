@@ -728,7 +728,7 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
     for (uint32_t tx = 0; tx < F.tiles_x; ++tx) {
       const uint32_t m = hdsr_morton2(tx, ty);
       const uint32_t s = F.tile_offsets[m], c = F.tile_counts[m];
-      uint32_t tile_sat = 0;  // the largest saturation index over the tile's 256 pixels
+      uint32_t tile_sat = 0;  // the largest saturation index over the tile's pixels in the viewport
       for (uint32_t py = 0; py < 16; ++py)
         for (uint32_t pxl = 0; pxl < 16; ++pxl) {
           const uint32_t X = tx * 16 + pxl, Y = ty * 16 + py;
@@ -804,10 +804,11 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
               eb = eb + h2f(w[12]) * wgt;
             }
           }
-          tile_sat = std::max(tile_sat, sat);
           // The per-quad early exit (all four T < cutoff) never changes a pixel: once T < 0.001,
-          // 'valid' is false for every later splat.
+          // 'valid' is false for every later splat.  A pixel outside the viewport is never
+          // stored, so only the viewport's pixels decide when the tile's walk may stop.
           if (X >= W || Y >= H) continue;
+          tile_sat = std::max(tile_sat, sat);
           const size_t p = (size_t)Y * W + X;
           if (f16) {
             F.rgba[p * 4 + 0] = (float)f16r((double)cr + f16r((double)T * bg_rg16));
@@ -833,8 +834,9 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
             F.gb_material[p * 4 + 3] = hdsr_pack2x16(gconf / safe, 0.0f);
           }
         }
-      // the entries the rasterizer loads: whole 256-entry batches until every pixel of the tile is
-      // saturated (a pixel saturated before entry i blended its last entry in batch (i - 1) / 256)
+      // the entries the rasterizer loads: whole 256-entry batches until every pixel of the tile
+      // inside the viewport is saturated (a pixel saturated before entry i blended its last entry
+      // in batch (i - 1) / 256)
       F.tile_loaded[m] = (c == 0 || tile_sat >= c) ? c : std::min(c, 256u * ((tile_sat - 1u) / 256u + 1u));
     }
   });

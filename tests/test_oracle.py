@@ -241,6 +241,42 @@ def test_single_isotropic_splat_closed_form():
     np.testing.assert_allclose(rgba[24, 32 - 3], rgba[24, 32 + 2], atol=1e-6)
 
 
+def test_tile_loaded_ignores_pixels_outside_the_viewport():
+    """1000 copies of a splat wide in x and about 7 px tall, centred on row 20 of a 64x40 image:
+    rows 40-47 of the last tile row lie outside the viewport, where the power is below -4 and a
+    pixel never saturates.  Only the viewport's pixels decide how many 256-entry batches a tile
+    loads: float64 closed form of each pixel's saturation count, n = ceil(ln 0.001 / ln(1 - a))."""
+    n = 1000
+    sc = np.asarray([[math.log(2.0), math.log(0.04), math.log(2.0)]], np.float32)
+    s = scenes.Scene(np.zeros((n, 3), np.float32), np.tile(sc, (n, 1)),
+                     np.tile(np.asarray([[1, 0, 0, 0]], np.float32), (n, 1)), np.full(n, 0.99, np.float32),
+                     np.full((n, 3), 0.5, np.float32), None, 0, scenes.pack_material(n))
+    cam = camera.Camera(camera.orbit_eye(3.0, 0.0, 0.0), (0, 0, 0), 525.0, 525.0, 64, 40)
+    plan, srt, u = frame_inputs(s, cam)
+    f = oracle.run(s, srt, u, cam.width, cam.height)
+    w = f["cache"][:14]
+    cx, cy = w[0:2].view(np.float32).astype(np.float64)
+    k0, k1, k2 = w[2:5].view(np.float32).astype(np.float64)
+    o = float(np.uint16(w[6] >> 16).view(np.float16))
+    assert abs(cy - 20.0) < 1e-4 and 6.5 < 1 / math.sqrt(k2) < 7.5 and k0 < 1e-4  # wide in x, ~7 px tall
+
+    def sat(X, Y):
+        dx, dy = X + 0.5 - cx, Y + 0.5 - cy
+        p = -0.5 * k0 * dx * dx - k1 * dx * dy - 0.5 * k2 * dy * dy
+        a = min(0.99, o * math.exp(p))
+        if not (p > -4 and a > 1 / 255):
+            return n + 1  # never
+        return math.ceil(math.log(0.001) / math.log(1 - a))
+
+    assert sat(0, 40) == n + 1 and sat(0, 39) <= n  # below the viewport: never; its last row: saturates
+    for ty in range(plan["tileRows"]):
+        for tx in range(plan["tileColumns"]):
+            m = oracle.morton2(tx, ty)
+            assert f["tile_counts"][m] == n
+            last = max(sat(X, Y) for X in range(16 * tx, 16 * tx + 16) for Y in range(16 * ty, min(16 * ty + 16, 40)))
+            assert f["tile_loaded"][m] == min(n, 256 * ((last - 1) // 256 + 1)) < n
+
+
 @pytest.mark.parametrize("opacity,loaded", [(0.9, 256), (0.034, 256), (0.02, 512), (0.01, 768), (0.003, 1000)])
 def test_tile_loaded_closed_form(opacity, loaded):
     """1000 copies of one wide splat over a 64x48 image: every pixel sees alpha = o * exp(p) with
