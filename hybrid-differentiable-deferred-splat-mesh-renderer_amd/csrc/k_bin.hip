@@ -436,8 +436,11 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* kin, cons
 // cell list is kept inside every tile.
 
 // Also resets the per-tile counts and depth ranges that the counting walk accumulates into; with
-// no pairs to group (keys null: no splats in the batch) it does only that.
-__global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
+// no pairs to group (keys null: no splats in the batch) it does only that.  When the cell ids
+// were sorted in one radix pass (at most 256 cells), each cell is a digit and its range comes
+// from the pass's scanned digit-major histogram (hist, nblocks per digit); otherwise from two
+// binary searches in the sorted keys.
+__global__ void k_cell_ranges(Bufs b, const uint32_t* keys, const uint32_t* hist, uint32_t nblocks) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t i = c; i < b.total_morton; i += gridDim.x * blockDim.x) {
     b.tile_counts[i] = 0u;
@@ -447,20 +450,27 @@ __global__ void k_cell_ranges(Bufs b, const uint32_t* keys) {
   }
   if (!keys || c >= b.total_cells) return;
   const uint32_t n = b.stats->cell_pairs;
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (keys[mid] < c) lo = mid + 1; else hi = mid;
-  }
-  const uint32_t s = lo;
-  hi = n;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (keys[mid] <= c) lo = mid + 1; else hi = mid;
+  uint32_t s, e;
+  if (hist) {
+    s = hist[(size_t)c * nblocks];
+    e = c + 1u < kRadixBuckets ? hist[(size_t)(c + 1u) * nblocks] : n;
+  } else {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (keys[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    s = lo;
+    hi = n;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (keys[mid] <= c) lo = mid + 1; else hi = mid;
+    }
+    e = lo;
   }
   b.cell_offsets[c] = s;
-  b.cell_counts[c] = lo - s;
-  b.cell_nchunk[c] = (lo - s + kCellChunk - 1) / kCellChunk;
+  b.cell_counts[c] = e - s;
+  b.cell_nchunk[c] = (e - s + kCellChunk - 1) / kCellChunk;
 }
 
 // Expansion workgroups of the frame batch: sum over cells of ceil(count / kCellChunk).
@@ -923,7 +933,11 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
                            b.cell_key_bits, b.radix_hist, b.radix_tmp, stream);
   // cell ranges and expansion chunks, per-chunk tile counts, per-cell prefixes -> tile counts
   const uint32_t cg = std::max((b.total_cells + 255) / 256, std::min((b.total_morton + 255) / 256, 1024u));
-  hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, pairs ? sorted_cell_keys(b) : nullptr);
+  // one radix pass: the cells are its digits, and its scanned histogram holds their ranges
+  const bool one_pass = pairs && cell_passes(b) == 1 && b.total_cells <= kRadixBuckets;
+  const uint32_t nblocks = (b.cell_cap + kRadixBlockItems - 1) / kRadixBlockItems;
+  hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, pairs ? sorted_cell_keys(b) : nullptr,
+                     one_pass ? (const uint32_t*)b.radix_hist : nullptr, nblocks);
   if (!pairs) return;  // every tile count is zero: the later stages find no entries
   scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
