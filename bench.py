@@ -29,26 +29,35 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Stages that are one kernel each: the roofline is taken on the slowest of them.
+# Stages whose time is one kernel's (or one kernel and a negligible helper's): the roofline is taken
+# on the slowest of them.  In a frame that selects first chunks the bucket stage is k_first_chunks
+# (after the one-workgroup k_order_large), and the sort stage is the second round's scatter, sort
+# and masks, not one kernel; otherwise the bucket stage is k_bucket and the sort stage k_segsort.
 SINGLE_KERNEL_STAGES = {"project": "k_project", "emit": "k_emit", "bucket": "k_bucket", "sort": "k_segsort",
                         "composite": "k_composite"}
+SINGLE_KERNEL_STAGES_FIRST = {"project": "k_project", "emit": "k_emit", "bucket": "k_first_chunks",
+                              "composite": "k_composite"}
 # HBM bytes per frame of each kernel on this workload (1M SH3, 1920x1080), from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a 1 GiB copy (FETCH x2.0, WRITE
-# x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  k_segsort and k_composite run twice per frame
-# (the deferred chunk sort's two rounds): theirs are the sums of both launches.  Valid for the
-# kernels as profiled.
-PMC_TRAFFIC_BYTES = {"k_project": 317.3e6, "k_emit": 30.6e6, "k_bucket": 761.8e6, "k_segsort": 217.9e6,
-                     "k_composite": 191.8e6}
-# What bounds the rasterizer instead: its SQ counters on the same workload (first round, which carries
-# nearly all of its work; profiles/sq_counters_1M_sh3_1080p.md).  A wave64 VALU instruction occupies
-# its SIMD for 4 cycles, so 0.25 instructions per SIMD per cycle is the issue ceiling; the LDS peak is
-# 128 B per clock per CU over 256 CUs at 2.4 GHz.
-COMPOSITE_SQ = {"valu_busy": 0.917, "valu_instr_per_simd_cycle": 0.228, "valu_issue_ceiling": 0.25,
-                "valu_frac": round(0.228 / 0.25, 3), "occupancy": 0.56, "lds_GBps": 22650.0,
-                "lds_peak_GBps": 78643.0, "lds_frac": round(22650.0 / 78643.0, 3),
+# x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  k_composite is the sum of its two launches
+# (k_composite and k_composite_resume).  Valid for the kernels as profiled (frames that select
+# first chunks, the default here).
+PMC_TRAFFIC_BYTES = {"k_project": 317.2e6, "k_emit": 30.6e6, "k_first_chunks": 365.8e6, "k_composite": 182.4e6}
+# What bounds the two heaviest kernels instead of HBM: their SQ counters on the same workload
+# (profiles/sq_counters_1M_sh3_1080p.md).  A wave64 VALU instruction occupies its SIMD for 4 cycles,
+# so 0.25 instructions per SIMD per cycle is the issue ceiling; the LDS peak is 128 B per clock per
+# CU over 256 CUs at 2.4 GHz.
+COMPOSITE_SQ = {"valu_busy": 1.0, "valu_instr_per_simd_cycle": 0.249, "valu_issue_ceiling": 0.25,
+                "valu_frac": round(0.249 / 0.25, 3), "occupancy": 0.54, "lds_GBps": 26504.0,
+                "lds_peak_GBps": 78643.0, "lds_frac": round(26504.0 / 78643.0, 3),
                 "source": "rocprofv3 SQ counters, round 1 of k_composite (profiles/sq_counters_1M_sh3_1080p.md)"}
+FIRST_CHUNKS_SQ = {"valu_busy": 0.63, "valu_instr_per_simd_cycle": 0.158, "valu_issue_ceiling": 0.25,
+                   "valu_frac": round(0.158 / 0.25, 3), "occupancy": 0.44, "lds_GBps": 4981.0,
+                   "lds_peak_GBps": 78643.0, "lds_frac": round(4981.0 / 78643.0, 3), "waiting": 0.14,
+                   "source": "rocprofv3 SQ counters of k_first_chunks (profiles/sq_counters_1M_sh3_1080p.md)"}
+KERNEL_SQ = {"k_composite": COMPOSITE_SQ, "k_first_chunks": FIRST_CHUNKS_SQ}
 # launches per frame of the single-kernel stages (the deferred rounds)
-LAUNCHES_PER_FRAME = {"k_segsort": 2, "k_composite": 2}
+LAUNCHES_PER_FRAME = {"k_composite": 2}
 
 
 def parse():
@@ -103,13 +112,20 @@ def algorithmic_bytes(stage: str, n: int, k: int, st: dict, pixels: int, composi
     if stage == "classify":
         T = st.get("tiles", 0)
         return 40 * T + 16 * st.get("segments", 0)  # tile ranges, class records in and out, lists, segments
+    L = st.get("large_tile_refs", 0)
+    f = st.get("first_chunks", 0.0)  # the views' share of frames that selected first chunks
     if stage == "bucket":
-        # this design's cascade: the keys read twice (counts, scatter), the entries once, both written
-        return 20 * st.get("large_tile_refs", 0)
+        # first chunks: the large tiles' keys read twice (counts, selection), each first chunk's 4096
+        # entries read and written, and the small tiles' key + entry in, entry out (sorted in the
+        # same launch); with the whole scatter: the keys read twice, the entries once, both written
+        first = 8 * L + 8 * 4096 * st.get("large_tiles", 0) + 12 * (R - L)
+        return f * first + (1 - f) * 20 * L
     if stage == "sort":
-        # key + entry in, entry out, for the entries actually sorted (the deferred chunk sort orders
-        # only the chunks the rasterizer reaches)
-        return 12 * st.get("sorted_refs", R)
+        # first chunks: the second round scatters the resumed tiles whole and sorts their later
+        # chunks; with the whole scatter: key + entry in, entry out for every entry sorted (the
+        # deferred chunk sort orders only the chunks the rasterizer reaches)
+        Lr, Tr = st.get("resumed_tile_refs", 0), st.get("resumed_tiles", 0)
+        return f * (20 * Lr + 12 * (Lr - 4096 * Tr)) + (1 - f) * 12 * st.get("sorted_refs", R)
     if stage == "composite":
         return 60 * composite_refs + pixels * 20
     return 0.0
@@ -216,14 +232,16 @@ def main() -> int:
         per_stage[name] = {"ms": round(ms, 4), "GBps": None if gbs is None else round(gbs, 1)}
         # the counters' bytes for single-kernel stages: the bucket cascade's per-unit figure (the
         # reference's three global passes, 44 B per large-tile reference) exceeds what this kernel moves
-        pmc = PMC_TRAFFIC_BYTES.get(SINGLE_KERNEL_STAGES.get(name, ""))
+        pmc = PMC_TRAFFIC_BYTES.get((SINGLE_KERNEL_STAGES_FIRST if mean_st.get("first_chunks", 0.0) >= 0.5
+                                     else SINGLE_KERNEL_STAGES).get(name, ""))
         if full_size and pmc and ms > 0:
             per_stage[name]["traffic_GBps"] = round(pmc / (ms * 1e-3) / 1e9, 1)
     over_peak = [nm for nm, v in per_stage.items() if (v["GBps"] or 0) > HBM_PEAK_GBS]
     if over_peak:
         print(f"stage bytes over the HBM peak (their per-unit figures overcount): {over_peak}", file=sys.stderr)
-    dom = max(SINGLE_KERNEL_STAGES, key=lambda s: stage_ms[s])
-    dom_kernel = SINGLE_KERNEL_STAGES[dom]
+    kernels = SINGLE_KERNEL_STAGES_FIRST if mean_st.get("first_chunks", 0.0) >= 0.5 else SINGLE_KERNEL_STAGES
+    dom = max(kernels, key=lambda s: stage_ms[s])
+    dom_kernel = kernels[dom]
     dom_bytes = algorithmic_bytes(dom, scene.count, k, mean_st, pixels, comp_refs)
     achieved = dom_bytes / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
 
@@ -283,7 +301,8 @@ def main() -> int:
                      "traffic": PMC_TRAFFIC_BYTES.get(dom_kernel) if full_size else None,
                      "algorithmic_bytes_per_frame": dom_bytes, "ms_per_frame": round(stage_ms[dom], 4),
                      "launches_per_frame": LAUNCHES_PER_FRAME.get(dom_kernel, 1),
-                     "issue": COMPOSITE_SQ if dom_kernel == "k_composite" and full_size else None},
+                     "issue": KERNEL_SQ.get(dom_kernel) if full_size else None},
+        "rasterizer_issue": COMPOSITE_SQ if full_size else None,
         "stages_ms_per_frame": per_stage,
         "stages_within_peak": not over_peak,
         "frame_stats": {k2: round(v2, 1) for k2, v2 in mean_st.items()},

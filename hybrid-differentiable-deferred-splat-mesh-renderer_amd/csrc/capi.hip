@@ -727,9 +727,12 @@ int hdsr_get_stats(hdsr_ctx* c, hdsr_stats* o) {
   o->cell_pairs = s.cell_pairs;
   // a deferred frame sorts the small tiles, the first 4096-entry chunk of every large tile and the
   // other chunks of the tiles resumed; a one-pass frame sorts every entry
-  o->sorted_refs = c->eng.last_frame_deferred()
-                       ? (s.refs - s.scratch_used) + HDSR_MAX_TILE_ENTRIES * s.large_tiles + s.resumed_refs
-                       : s.refs;
+  const bool deferred = c->eng.last_frame_deferred();
+  o->sorted_refs = deferred ? (s.refs - s.scratch_used) + HDSR_MAX_TILE_ENTRIES * s.large_tiles + s.resumed_refs
+                            : s.refs;
+  o->resumed_tiles = deferred ? s.pend_tiles : 0u;
+  o->resumed_tile_refs = deferred ? s.resumed_refs + HDSR_MAX_TILE_ENTRIES * s.pend_tiles : 0u;
+  o->first_chunks = deferred && c->eng.last_frame_first_chunks() ? 1u : 0u;
   return HDSR_OK;
 }
 

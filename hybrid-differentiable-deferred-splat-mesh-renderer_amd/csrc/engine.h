@@ -55,6 +55,7 @@ class Engine {
   // whether the last enqueued frame ran the deferred chunk sort (first chunks, then the resumed
   // tiles' other chunks) rather than sorting every chunk
   bool last_frame_deferred() const { return last_deferred_; }
+  bool last_frame_first_chunks() const { return last_first_; }
   int wait();
   bool overflowed() const { return hstats_ && hstats_->overflow != 0; }
   // Grows the pair capacity to hold the last run's references (with headroom).

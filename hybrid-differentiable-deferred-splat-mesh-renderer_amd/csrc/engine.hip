@@ -604,6 +604,7 @@ int Engine::composite_only(const FrameDesc* frames_in, uint32_t V, hipStream_t s
     launch_composite(r1, stream, gbuffer, f16_);
     launch_composite(r2, stream, gbuffer, f16_);
   } else {
+    launch_reset_pending(b_, stream);  // the rasterizer's counters count this run
     launch_composite(b_, stream, gbuffer, f16_);
   }
   const hipError_t le = hipGetLastError();

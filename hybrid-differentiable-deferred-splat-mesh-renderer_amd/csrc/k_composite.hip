@@ -657,9 +657,15 @@ __global__ __launch_bounds__(64) void k_composite_resume(Bufs b) {
     resume_quadrant<kH>(b, k >> 2, __builtin_amdgcn_readfirstlane(b.pend_tiles[k >> 2]), k & 3u, s_a, s_b, s_c);
 }
 
+// Before the rounds run again over the same lists (encodeCompositeOnly): the round-2 lists and the
+// rasterizer's counters start empty.
 __global__ void k_reset_pending(DevStats* s) {
   s->pend_tiles = 0;
   s->pend_segs = 0;
+  s->resumed_refs = 0;
+  s->resumed_max = 0;
+  s->composite_refs = 0;
+  s->composite_max_refs = 0;
 }
 
 }  // namespace

@@ -75,7 +75,7 @@ class Stats(C.Structure):
         "refs", "small_pairs", "large_pairs", "large_splats", "visible_splats", "small_tiles", "large_tiles",
         "chunks", "segments", "pair_capacity", "overflow_reruns", "cap_refs_tripped", "cap_large_splats_tripped",
         "cap_chunks_tripped", "composite_refs", "composite_max_refs", "large_tile_refs", "cell_pairs",
-        "sorted_refs")]
+        "sorted_refs", "resumed_tiles", "resumed_tile_refs", "first_chunks")]
 
 
 class Grads(C.Structure):

@@ -630,6 +630,9 @@ static napi_value js_stats(napi_env env, napi_callback_info info) {
   set_u32(env, o, "largeTileRefs", s.large_tile_refs);
   set_u32(env, o, "cellPairs", s.cell_pairs);
   set_u32(env, o, "sortedRefs", s.sorted_refs);
+  set_u32(env, o, "resumedTiles", s.resumed_tiles);
+  set_u32(env, o, "resumedTileRefs", s.resumed_tile_refs);
+  set_u32(env, o, "firstChunks", s.first_chunks);
   return o;
 }
 

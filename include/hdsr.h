@@ -307,6 +307,10 @@ typedef struct hdsr_stats {
   uint32_t cell_pairs;            /* (8x8-tile cell, splat) pairs of the tile-list build */
   uint32_t sorted_refs;           /* entries the per-tile sort ordered (a frame without debug readback,
                                      G-buffer or backward sorts only the chunks its pixels reach) */
+  uint32_t resumed_tiles;         /* such a frame: large tiles left unsaturated by their first chunk */
+  uint32_t resumed_tile_refs;     /* their references (the bucket order their second round builds) */
+  uint32_t first_chunks;          /* 1: such a frame selected the large tiles' first chunks without the
+                                     bucket scatter (see HDSR_BUCKET_SCATTER) */
 } hdsr_stats;
 int hdsr_get_stats(hdsr_ctx* ctx, hdsr_stats* out);
 
