@@ -29,7 +29,7 @@ var addon = require(path.join(__dirname, 'hdsr_napi.node'));
 
 var TILE_SPLAT_FRAME_UNIFORM_BYTES = 256;
 
-var FLAGS = { f16: 1, refCaps: 2, gbuffer: 4, debug: 8, rgba16: 16, profile: 32 };
+var FLAGS = { f16: 1, refCaps: 2, gbuffer: 4, debug: 8, rgba16: 16, profile: 32, bucketScatter: 128 };
 // texture format -> (output buffer, typed array constructor, elements per pixel, creation flag it needs)
 var TEXTURE_FORMATS = {
   rgba32float: { out: 0, ctor: Float32Array, per: 4, flag: 0 },
