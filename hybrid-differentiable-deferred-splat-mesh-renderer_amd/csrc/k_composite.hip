@@ -120,13 +120,15 @@ struct Gb {
 };
 
 // One entry into one pixel, without branches: an entry that does not pass the gate leaves every
-// accumulator unchanged (w = 0, T kept).  exp is only consumed where power > -4; there the
-// check-free path gives hdsr_expf's result bit for bit once the argument is capped at 88 (above,
-// both saturate alpha at 0.99).  Below -4 the value is unused: whatever the path returns there
-// (zero, a NaN or an infinity for an extreme argument), alpha = min(0.99, o * gauss) stays finite
-// (min drops a NaN), so w = alpha * T * 0 is zero and T is kept.
+// accumulator unchanged (w = 0, T kept).  exp is only consumed where power > -4, capped at 88; there
+// hdsr_expf_gauss is the oracle's exponential bit for bit.  The cap is a signed integer min of the
+// bits (one instruction, where a float min also canonicalises its input): for every float that is
+// not a NaN it is min(power, 88).  Below -4 the value is unused: whatever it is there (any bits, a
+// NaN or an infinity for an extreme argument), alpha = min(0.99, o * gauss) stays finite (min drops
+// a NaN), so w = alpha * T * 0 is zero and T is kept.
 __device__ __forceinline__ float alpha_of(float power, float o) {
-  return fminf(0.99f, o * hdsr_expf_core(fminf(power, 88.0f)));
+  const float x = hdsr_u2f((uint32_t)min((int32_t)hdsr_f2u(power), (int32_t)0x42B00000));  // 88.0f
+  return fminf(0.99f, o * hdsr_expf_gauss(x));
 }
 
 template <bool kG>

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_raster_bwd(Bufs b, const float* __restr
         const bool act = s_idx[j] < my_last && pw > -4.0f;
         if (__any(act)) {  // uniform
           // the forward's gaussian and alpha, bit for bit (exp only consumed where power > -4)
-          const float gauss = hdsr_expf_core(fminf(fmaxf(pw, -87.0f), 88.0f));
+          const float gauss = hdsr_expf_gauss(fminf(fmaxf(pw, -86.0f), 88.0f));
           const float alpha = fminf(0.99f, B.y * gauss);
           const bool valid = act && alpha > (1.0f / 255.0f);
           if (__any(valid)) {

@@ -756,7 +756,7 @@ void composite32(Frame& F, bool gbuffer, uint32_t row_step = 1, bool f16 = false
             const float power = (ca * dx * dx + cbx * dx * dy) + cc * dy * dy;
             const bool dm = !F.diffrec.empty();
             const float o = dm ? F.diffrec[(size_t)r * 4] : h2f(w[6] >> 16);
-            const float g = hdsr_expf(power);
+            const float g = hdsr_expf_gauss(std::fmin(power, 88.0f));  // read only where power > -4
             const float alpha = std::fmin(0.99f, o * g);
             const bool valid = power > -4.0f && alpha > a_th && T >= t_cut;
             const float r_ = dm ? F.diffrec[(size_t)r * 4 + 1] : h2f(w[9]);
@@ -1277,6 +1277,9 @@ uint32_t orc_morton2(uint32_t x, uint32_t y) { return hdsr_morton2(x, y); }
 
 void orc_expf_n(const float* x, float* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_expf(x[i]);
+}
+void orc_expf_gauss_n(const float* x, float* y, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_expf_gauss(x[i]);
 }
 void orc_logf_n(const float* x, float* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_logf(x[i]);

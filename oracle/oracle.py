@@ -59,7 +59,7 @@ def load() -> C.CDLL:
     for name in ("orc_expf", "orc_logf"):
         getattr(lib, name).restype = C.c_float
         getattr(lib, name).argtypes = [C.c_float]
-    for name in ("orc_expf_n", "orc_logf_n"):
+    for name in ("orc_expf_n", "orc_logf_n", "orc_expf_gauss_n"):
         getattr(lib, name).restype = None
         getattr(lib, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
     lib.orc_f16_bits_n.restype = None

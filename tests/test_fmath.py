@@ -44,6 +44,28 @@ def test_expf_within_2ulp(lib):
     assert lib.orc_expf(0.0) == 1.0
 
 
+def test_expf_gauss_within_2ulp(lib):
+    """The compositor's exponential (rounding constant for k, 2^k added to the exponent bits): within
+    2 ulp of float64 exp over the power range it is read on and its whole stated range, exact at 0,
+    and equal to hdsr_expf except where x * log2(e) lies within rounding of a half-integer."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([
+        np.linspace(-4.0, 0.0, 400_001, dtype=np.float32),      # the powers it is read on
+        -rng.exponential(1.0, 200_000).astype(np.float32),
+        np.linspace(-86.0, 88.0, 200_001, dtype=np.float32),
+        np.array([0.0, -0.0, -1e-30, -0.34657359, 0.34657359, -3.4657359], np.float32),
+    ])
+    y = run_n(lib.orc_expf_gauss_n, x)
+    e = ulp_err(y, np.exp(x.astype(np.float64)))
+    assert e.max() <= 2.0, f"max ulp {e.max()} at x={x[np.argmax(e)]}"
+    assert run_n(lib.orc_expf_gauss_n, np.zeros(1))[0] == 1.0
+    z = run_n(lib.orc_expf_n, x)
+    diff = x[y != z].astype(np.float64)
+    assert diff.size < 1e-4 * x.size
+    t = diff * np.log2(np.e)  # the differences sit where k's two roundings can part
+    assert np.all(np.abs(np.abs(t - np.floor(t)) - 0.5) < 1e-6 * np.maximum(1.0, np.abs(t)))
+
+
 def test_expf_special_values(lib):
     x = np.array([np.nan, 100.0, 88.5, -100.0, -87.5, np.inf, -np.inf], np.float32)
     y = run_n(lib.orc_expf_n, x)
