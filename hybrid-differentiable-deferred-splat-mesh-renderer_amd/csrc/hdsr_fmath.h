@@ -233,7 +233,14 @@ HDSR_HD uint32_t hdsr_bucket_of(float y) {
   return b < HDSR_NUM_BUCKETS - 1u ? b : HDSR_NUM_BUCKETS - 1u;
 }
 
-HDSR_HD uint32_t hdsr_q20_of(float y) {
-  const uint32_t q = hdsr_u32sat(y * 8192.0f);
-  return q < 0xFFFFFu ? q : 0xFFFFFu;
+// q20 of a position already multiplied by 2^13: min(u32sat(x), 2^20 - 1) without branches.  fmaxf
+// sends NaN and negatives to 0, and the conversion of a value in [0, 2^20 - 1] truncates, as
+// u32sat does.  Callers may fold the 2^13 into a finite tile scale (hdsr_log_frame's is 0 or
+// at most 1.28e12): (d * scale) * 2^13 and d * (scale * 2^13) round alike (a power of two
+// commutes with rounding), except where d * scale is subnormal or overflows, and there both
+// give 0 or 2^20 - 1.
+HDSR_HD uint32_t hdsr_q20_scaled(float x) {
+  return (uint32_t)fminf(fmaxf(x, 0.0f), 1048575.0f);
 }
+
+HDSR_HD uint32_t hdsr_q20_of(float y) { return hdsr_q20_scaled(y * 8192.0f); }

@@ -712,7 +712,6 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   if (q >= cell_chunk_total(b)) return;  // uniform
   const ChunkInfo ci = chunk_info(b, q);
   const uint32_t t = threadIdx.x, w = t >> 6, lane = __lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
   const FrameDesc& F = *ci.F;
   // lanes 0..15 of wave w fetch tile 16w + lane's write base and log-depth frame up front, so
   // the loads and the logarithms of all 16 tiles overlap the staging below
@@ -726,7 +725,7 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
       float lm, sc;
       hdsr_log_frame(b.tile_dmin[m], b.tile_dmax[m], &lm, &sc);
       my_lm = hdsr_f2u(lm);
-      my_sc = hdsr_f2u(sc);
+      my_sc = hdsr_f2u(sc * 8192.0f);  // the q20 scale
       my_ok = 1;
     }
   }
@@ -749,21 +748,25 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
   __syncthreads();
   const uint32_t cap = b.cap;
   const uint32_t ngroups = (ci.count + 63) / 64;
+  // the tile's bit is tested in the one 32-bit half of the mask that holds it
+  const uint32_t* s_mw = reinterpret_cast<const uint32_t*>(s_mask);
   for (uint32_t k = 0; k < 16; ++k) {
     const uint32_t tt = w * 16 + k;  // this wave's tile (uniform)
     if (!__builtin_amdgcn_readlane(my_ok, k)) continue;  // outside the tile grid
     uint32_t base = __builtin_amdgcn_readlane(my_base, k);
     const float lm = hdsr_u2f(__builtin_amdgcn_readlane(my_lm, k));
-    const float sc = hdsr_u2f(__builtin_amdgcn_readlane(my_sc, k));
+    const float sq = hdsr_u2f(__builtin_amdgcn_readlane(my_sc, k));
+    const uint32_t half = tt >> 5, bit = 1u << (tt & 31u);
     for (uint32_t gi = 0; gi < ngroups; ++gi) {
       const uint32_t i = gi * 64 + lane;
-      const bool in = (s_mask[i] >> tt) & 1ull;  // entries past the count have an empty mask
+      const bool in = (s_mw[2 * i + half] & bit) != 0u;  // entries past the count have an empty mask
       const unsigned long long mm = __ballot(in);
       if (in) {
-        const uint32_t dst = base + (uint32_t)__popcll(mm & below);
+        // base + covering lanes below this one
+        const uint32_t dst = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm, base));
         if (dst < cap) {
           b.entries[dst] = s_g[i];
-          b.qkey[dst] = hdsr_q20_of((s_ln[i] - lm) * sc);  // the entry's key in this tile's frame
+          b.qkey[dst] = hdsr_q20_scaled((s_ln[i] - lm) * sq);  // the entry's key in this tile's frame
         }
       }
       base += (uint32_t)__popcll(mm);

@@ -1284,6 +1284,10 @@ void orc_expf_gauss_n(const float* x, float* y, uint64_t n) {
 void orc_logf_n(const float* x, float* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_logf(x[i]);
 }
+// q20 with the 2^13 folded into the scale, as the tile-list write computes it
+void orc_q20_folded_n(const float* d, const float* sc, uint32_t* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = hdsr_q20_scaled(d[i] * (sc[i] * 8192.0f));
+}
 void orc_f16_bits_n(const float* x, uint32_t* y, uint64_t n) {
   for (uint64_t i = 0; i < n; ++i) y[i] = hdsr_f16_bits(x[i]);
 }

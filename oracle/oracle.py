@@ -64,6 +64,8 @@ def load() -> C.CDLL:
         getattr(lib, name).argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
     lib.orc_f16_bits_n.restype = None
     lib.orc_f16_bits_n.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+    lib.orc_q20_folded_n.restype = None
+    lib.orc_q20_folded_n.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]
     lib.orc_f16_bits.restype = C.c_uint32
     lib.orc_f16_bits.argtypes = [C.c_float]
     lib.orc_f16_to_f32.restype = C.c_float

@@ -140,6 +140,29 @@ def test_u32sat(lib):
         assert lib.orc_u32sat(x) == want, x
 
 
+def test_q20_with_folded_scale(lib):
+    # the tile-list write multiplies by scale * 2^13 at once; the key must equal
+    # min(u32sat((d * scale) * 2^13), 2^20 - 1), two roundings, as the sort stages define it
+    rng = np.random.default_rng(3)
+    n = 400_000
+    d = (rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-45, 39, n)).astype(np.float32)
+    d[: n // 4] = rng.uniform(-1.0, 129.0, n // 4).astype(np.float32)  # ln-depth offsets of a tile
+    sc = (10.0 ** rng.uniform(0.0, np.log10(1.28e12), n)).astype(np.float32)
+    sc[::7] = 0.0
+    sc[1::11] = np.float32(1.28e12)
+    d[2::13] = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 3.4e38, 2.0 ** -140,
+                         127.99999, 128.0, 1048575.0 / 8192.0, 1048576.0 / 8192.0], np.float32)[
+        np.arange(len(d[2::13])) % 13]
+    out = np.empty(n, np.uint32)
+    lib.orc_q20_folded_n(d.ctypes.data, sc.ctypes.data, out.ctypes.data, n)
+    with np.errstate(over="ignore", invalid="ignore", under="ignore"):
+        y = (d * sc).astype(np.float32) * np.float32(8192.0)
+        want = np.where(~(y > 0), 0.0, np.where(y >= 4294967296.0, 4294967295.0, np.trunc(np.nan_to_num(y))))
+    want = np.minimum(want, 0xFFFFF).astype(np.uint32)
+    bad = np.flatnonzero(out != want)
+    assert bad.size == 0, (d[bad[:5]], sc[bad[:5]], out[bad[:5]], want[bad[:5]])
+
+
 def test_morton_bijection(lib):
     def ref(x, y):
         out = 0
