@@ -705,9 +705,10 @@ __global__ __launch_bounds__(1024) void k_tile_ranges_one(uint32_t* counts, uint
 // chunk in order, so each tile's run of this chunk is written as one contiguous range (whole
 // cache lines instead of a few entries per store).
 __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) {
-  __shared__ unsigned long long s_mask[kCellChunk];
-  __shared__ uint32_t s_g[kCellChunk];
-  __shared__ float s_ln[kCellChunk];
+  // one group of padding: the walk reads the group after the last (and discards it)
+  __shared__ unsigned long long s_mask[kCellChunk + 64];
+  __shared__ uint32_t s_g[kCellChunk + 64];
+  __shared__ float s_ln[kCellChunk + 64];
   const uint32_t q = blockIdx.x;
   if (q >= cell_chunk_total(b)) return;  // uniform
   const ChunkInfo ci = chunk_info(b, q);
@@ -757,19 +758,33 @@ __global__ __launch_bounds__(256) void k_cell_write(Bufs b, const uint32_t* cv) 
     const float lm = hdsr_u2f(__builtin_amdgcn_readlane(my_lm, k));
     const float sq = hdsr_u2f(__builtin_amdgcn_readlane(my_sc, k));
     const uint32_t half = tt >> 5, bit = 1u << (tt & 31u);
-    for (uint32_t gi = 0; gi < ngroups; ++gi) {
-      const uint32_t i = gi * 64 + lane;
-      const bool in = (s_mw[2 * i + half] & bit) != 0u;  // entries past the count have an empty mask
+    // group gi's writes overlap the LDS reads of group gi + 1 (two register sets, no copies)
+    auto put = [&](uint32_t mw, uint32_t g, float ln) {
+      const bool in = (mw & bit) != 0u;  // entries past the count have an empty mask
       const unsigned long long mm = __ballot(in);
       if (in) {
         // base + covering lanes below this one
         const uint32_t dst = __builtin_amdgcn_mbcnt_hi((uint32_t)(mm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mm, base));
         if (dst < cap) {
-          b.entries[dst] = s_g[i];
-          b.qkey[dst] = hdsr_q20_scaled((s_ln[i] - lm) * sq);  // the entry's key in this tile's frame
+          b.entries[dst] = g;
+          b.qkey[dst] = hdsr_q20_scaled((ln - lm) * sq);  // the entry's key in this tile's frame
         }
       }
       base += (uint32_t)__popcll(mm);
+    };
+    uint32_t mwa = s_mw[2 * lane + half], ga = s_g[lane];
+    float lna = s_ln[lane];
+    for (uint32_t gi = 0; gi < ngroups; gi += 2) {
+      const uint32_t ib = (gi + 1) * 64 + lane;  // at most the padding group
+      const uint32_t mwb = s_mw[2 * ib + half], gb = s_g[ib];
+      const float lnb = s_ln[ib];
+      put(mwa, ga, lna);
+      if (gi + 1 >= ngroups) break;  // uniform
+      const uint32_t ia = (gi + 2) * 64 + lane;  // gi + 2 <= ngroups: at most the padding group
+      mwa = s_mw[2 * ia + half];
+      ga = s_g[ia];
+      lna = s_ln[ia];
+      put(mwb, gb, lnb);
     }
   }
 }
