@@ -74,65 +74,9 @@ __device__ __forceinline__ T block_excl_256(T v, T* total) {
   return Ops<T>::add(base, exc);
 }
 
-template <typename T>
-__device__ __forceinline__ T readlane63(T v);
-template <>
-__device__ __forceinline__ uint32_t readlane63(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-
 // One 1024-thread workgroup scans up to kOneGroupScan elements: the tile-count scan of a frame
 // then takes one launch instead of the hierarchical path's three.
 constexpr uint32_t kOneGroupScan = 16384;
-
-// Exclusive scan of in[0..n) into out (n <= kWaves * 1024) by a workgroup of kWaves waves;
-// returns the total.  Wave w owns the contiguous span [w*S, (w+1)*S), S a multiple of 64, read in
-// coalesced rounds of 64: a first read sums the span (all loads in flight together), the spans'
-// sums give every wave its base, and a second read (from L2) scans each round across the wave
-// with a running carry.
-template <typename T, uint32_t kWaves>
-__device__ __forceinline__ T one_group_scan(const T* in, T* out, uint32_t n) {
-  __shared__ T wsum[kWaves];
-  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-  const uint32_t S = ((n + kWaves * 64u - 1u) / (kWaves * 64u)) * 64u, rounds = S / 64u;
-  T part = Ops<T>::zero();
-#pragma unroll
-  for (uint32_t r = 0; r < 16; ++r) {
-    const uint32_t i = w * S + r * 64u + lane;
-    if (r < rounds && i < n) part = Ops<T>::add(part, in[i]);
-  }
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {  // inclusive scan across the wave: lane 63 holds the span's sum
-    const T t = Ops<T>::up(part, d);
-    if (lane >= (uint32_t)d) part = Ops<T>::add(part, t);
-  }
-  if (lane == 63) wsum[w] = part;
-  __syncthreads();
-  T carry = Ops<T>::zero(), tot = Ops<T>::zero();
-#pragma unroll
-  for (uint32_t q = 0; q < kWaves; ++q) {
-    const T x = wsum[q];
-    if (q < w) carry = Ops<T>::add(carry, x);
-    tot = Ops<T>::add(tot, x);
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < 16; ++r) {
-    if (r < rounds) {  // uniform
-      const uint32_t i = w * S + r * 64u + lane;
-      const T o = i < n ? in[i] : Ops<T>::zero();
-      T inc = o;
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const T t = Ops<T>::up(inc, d);
-        if (lane >= (uint32_t)d) inc = Ops<T>::add(inc, t);
-      }
-      if (i < n) out[i] = Ops<T>::add(carry, Ops<T>::sub(inc, o));
-      carry = Ops<T>::add(carry, readlane63(inc));
-    }
-  }
-  __syncthreads();  // wsum may be reused by the next call
-  return tot;
-}
 
 // ---- generic hierarchical exclusive scan: 4096 elements per workgroup (256 threads x 16) ----
 
@@ -688,15 +632,68 @@ __global__ void k_clamp_ranges(uint32_t* offsets, uint32_t* counts, uint32_t n, 
   counts[i] = ce - cs;
 }
 
-// The tile-count scan and the clamp above in one workgroup (n <= kOneGroupScan).
-__global__ __launch_bounds__(1024) void k_tile_ranges_one(uint32_t* counts, uint32_t* offsets, uint32_t n,
-                                                          uint32_t cap, const uint32_t* unl) {
-  (void)one_group_scan<uint32_t, 16>(counts, offsets, n);  // ends with a barrier: offsets are written
-  for (uint32_t i = threadIdx.x; i < n; i += 1024) {
-    const uint64_t st = offsets[i], e = st + counts[i] - (unl ? unl[i] : 0u);
+// The tile-count scan and the clamp above in one workgroup (n <= kOneGroupScan).  Thread t holds
+// elements 16t .. 16t+15 in registers, all loaded at once (one memory round trip), and scans them
+// in the thread, across its wave and across the 16 waves' sums; each element's range is clamped
+// as it is stored.
+__global__ __launch_bounds__(1024) void k_tile_ranges_one(uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
+                                                          uint32_t n, uint32_t cap, const uint32_t* __restrict__ unl) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+  const uint32_t i0 = t * 16u;
+  // the thread's 16 elements are all inside: vector loads and stores (counts and offsets are
+  // separate allocations, 16-byte aligned; the unlisted counts need not be)
+  const bool whole = i0 + 16u <= n;
+  uint32_t c[16], u[16];
+  if (whole) {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k += 4) {
+      const uint4 v = *reinterpret_cast<const uint4*>(counts + i0 + k);
+      c[k] = v.x; c[k + 1] = v.y; c[k + 2] = v.z; c[k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) c[k] = i0 + k < n ? counts[i0 + k] : 0u;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) u[k] = (unl && i0 + k < n) ? unl[i0 + k] : 0u;
+  uint32_t part = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) part += c[k];
+  uint32_t inc = part;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {  // inclusive scan across the wave
+    const uint32_t x = __shfl_up(inc, d, 64);
+    if (lane >= (uint32_t)d) inc += x;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t run = inc - part;  // exclusive offset of element 16t (u32, as the offsets are)
+#pragma unroll
+  for (uint32_t q = 0; q < 16; ++q)
+    if (q < w) run += wsum[q];
+  uint32_t o[16], r[16];
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) {
+    const uint64_t st = run, e = st + c[k] - u[k];
     const uint32_t cs = (uint32_t)(st < cap ? st : cap), ce = (uint32_t)(e < cap ? e : cap);
-    offsets[i] = cs;
-    counts[i] = ce - cs;
+    o[k] = cs;
+    r[k] = ce - cs;
+    run += c[k];
+  }
+  if (whole) {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k += 4) {
+      *reinterpret_cast<uint4*>(offsets + i0 + k) = make_uint4(o[k], o[k + 1], o[k + 2], o[k + 3]);
+      *reinterpret_cast<uint4*>(counts + i0 + k) = make_uint4(r[k], r[k + 1], r[k + 2], r[k + 3]);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k)
+      if (i0 + k < n) {
+        offsets[i0 + k] = o[k];
+        counts[i0 + k] = r[k];
+      }
   }
 }
 
