@@ -392,13 +392,29 @@ __global__ void k_cell_ranges(Bufs b, const uint32_t* keys, const uint32_t* hist
     b.tile_dmax[i] = 0u;
     if (b.tile_unl) b.tile_unl[i] = 0u;
   }
+  if (hist) {
+    // one radix pass: at most 256 cells, all in workgroup 0, which also scans their chunk counts
+    // into the chunk bases (no separate scan launch)
+    if (blockIdx.x != 0) return;  // uniform
+    const uint32_t n = b.stats->cell_pairs;
+    const bool in = c < b.total_cells;
+    const uint32_t s = in ? hist[(size_t)c * nblocks] : 0u;
+    const uint32_t e = !in ? 0u : c + 1u < kRadixBuckets ? hist[(size_t)(c + 1u) * nblocks] : n;
+    const uint32_t nch = (e - s + kCellChunk - 1) / kCellChunk;
+    uint32_t tot;
+    const uint32_t c0 = block_excl_256<uint32_t>(nch, &tot);
+    if (in) {
+      b.cell_offsets[c] = s;
+      b.cell_counts[c] = e - s;
+      b.cell_nchunk[c] = nch;
+      b.cell_chunk0[c] = c0;
+    }
+    return;
+  }
   if (!keys || c >= b.total_cells) return;
   const uint32_t n = b.stats->cell_pairs;
   uint32_t s, e;
-  if (hist) {
-    s = hist[(size_t)c * nblocks];
-    e = c + 1u < kRadixBuckets ? hist[(size_t)(c + 1u) * nblocks] : n;
-  } else {
+  {
     uint32_t lo = 0, hi = n;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -600,14 +616,14 @@ __global__ __launch_bounds__(64) void k_cell_prefix(Bufs b) {
   const uint32_t cx = lc % F.cells_x, cy = lc / F.cells_x;
   const uint32_t q0 = b.cell_chunk0[c], nq = b.cell_nchunk[c];
   uint32_t run = 0;
-  // eight chunks' counts are loaded before any of their prefixes is stored, so the loads of a
-  // group are in flight together instead of one round trip per chunk
-  for (uint32_t g = 0; g < nq; g += 8) {
-    uint32_t v[8];
+  // up to 32 chunks' counts are loaded before any of their prefixes is stored, so a cell of up
+  // to 32k pairs takes one memory round trip
+  for (uint32_t g = 0; g < nq; g += 32) {
+    uint32_t v[32];
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) v[k] = g + k < nq ? b.chunk_cnt[(size_t)(q0 + g + k) * 64 + t] : 0u;
+    for (uint32_t k = 0; k < 32; ++k) v[k] = g + k < nq ? b.chunk_cnt[(size_t)(q0 + g + k) * 64 + t] : 0u;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
+    for (uint32_t k = 0; k < 32; ++k) {
       if (g + k < nq) b.chunk_cnt[(size_t)(q0 + g + k) * 64 + t] = run;
       run += v[k];
     }
@@ -954,7 +970,7 @@ void launch_group_pairs(const Bufs& b, hipStream_t stream) {
   hipLaunchKernelGGL(k_cell_ranges, dim3(cg), dim3(256), 0, stream, b, pairs ? sorted_cell_keys(b) : nullptr,
                      one_pass ? (const uint32_t*)b.radix_hist : nullptr, nblocks);
   if (!pairs) return;  // every tile count is zero: the later stages find no entries
-  scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
+  if (!one_pass) scan_u32_exclusive(b.cell_nchunk, b.cell_chunk0, b.total_cells, b.radix_tmp, stream);
   hipLaunchKernelGGL(k_cell_count, dim3(b.max_chunks), dim3(256), 0, stream, b, sorted_cell_vals(b));
   hipLaunchKernelGGL(k_cell_prefix, dim3(b.total_cells), dim3(64), 0, stream, b);
   if (b.ref_caps && b.tile_unl)
