@@ -186,6 +186,68 @@ __global__ void k_totals(Bufs b) {
   s->cell_pairs = cells > b.cell_cap ? b.cell_cap : (uint32_t)cells;
 }
 
+// Both block-count scans and the totals in one 1024-thread workgroup (splat_blocks <= 4096):
+// thread t holds blocks 4t .. 4t+3 of both arrays in registers, loaded at once; the thread that
+// holds the last block writes the totals as k_totals does.
+constexpr uint32_t kOneGroupBlocks = 4096;
+__global__ __launch_bounds__(1024) void k_scan_blocks_one(Bufs b) {
+  __shared__ uint4 wsum[2][16];
+  const uint32_t n = b.splat_blocks;
+  const uint32_t t = threadIdx.x, lane = __lane_id(), w = t >> 6, i0 = t * 4u;
+  uint4 a[4], c[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    a[k] = i0 + k < n ? b.block_cnt[i0 + k] : make_uint4(0, 0, 0, 0);
+    c[k] = i0 + k < n ? b.block_cc[i0 + k] : make_uint4(0, 0, 0, 0);
+  }
+  const uint4 pa = a[0] + a[1] + a[2] + a[3], pc = c[0] + c[1] + c[2] + c[3];
+  uint4 ia = pa, ic = pc;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {  // inclusive scans across the wave
+    const uint4 xa = Ops<uint4>::up(ia, d), xc = Ops<uint4>::up(ic, d);
+    if (lane >= (uint32_t)d) {
+      ia = ia + xa;
+      ic = ic + xc;
+    }
+  }
+  if (lane == 63) {
+    wsum[0][w] = ia;
+    wsum[1][w] = ic;
+  }
+  __syncthreads();
+  uint4 ra = Ops<uint4>::sub(ia, pa), rc = Ops<uint4>::sub(ic, pc);
+#pragma unroll
+  for (uint32_t q = 0; q < 16; ++q)
+    if (q < w) {
+      ra = ra + wsum[0][q];
+      rc = rc + wsum[1][q];
+    }
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (i0 + k < n) {
+      b.block_pre[i0 + k] = ra;
+      b.block_ccpre[i0 + k] = rc;
+    }
+    if (i0 + k == n - 1u) {  // the totals (one thread)
+      const uint4 tt = ra + a[k], cc = rc + c[k];
+      uint32_t* st = reinterpret_cast<uint32_t*>(b.stats);
+      for (uint32_t j = 0; j < sizeof(DevStats) / 4; ++j) st[j] = 0;
+      DevStats* sp = b.stats;
+      sp->small_pairs = tt.x;
+      sp->large_pairs = tt.y;
+      sp->large_splats = tt.z;
+      sp->visible = tt.w;
+      const uint64_t refs = (uint64_t)tt.x + tt.y;
+      sp->refs = refs > b.cap ? b.cap : (uint32_t)refs;
+      sp->overflow = (!b.ref_caps && refs > b.cap) ? 1u : 0u;
+      const uint64_t cells = (uint64_t)cc.x + cc.y;
+      sp->cell_pairs = cells > b.cell_cap ? b.cell_cap : (uint32_t)cells;
+    }
+    ra = ra + a[k];
+    rc = rc + c[k];
+  }
+}
+
 // The cell box of a tile box: first cell, width and height in cells (0 when invisible / empty).
 __device__ __forceinline__ uint32_t cell_box(uint32_t bd, uint32_t* cx0, uint32_t* cy0, uint32_t* cw) {
   if (bd == HDSR_SENTINEL) return 0u;
@@ -929,12 +991,16 @@ void scan_u4_exclusive(const uint4* in, uint4* out, uint32_t n, uint4* tmp, hipS
 }
 
 void launch_clear_stats(const Bufs& b, hipStream_t stream) {
-  if (b.splat_blocks) return;  // k_totals clears them
+  if (b.splat_blocks) return;  // the block scan clears them with the totals
   hipLaunchKernelGGL(k_clear_stats, dim3(1), dim3(64), 0, stream, b.stats);
 }
 
 void launch_scan_blocks(const Bufs& b, hipStream_t stream) {
   if (!b.splat_blocks) return;
+  if (b.splat_blocks <= kOneGroupBlocks) {
+    hipLaunchKernelGGL(k_scan_blocks_one, dim3(1), dim3(1024), 0, stream, b);
+    return;
+  }
   scan_u4_exclusive(b.block_cnt, b.block_pre, b.splat_blocks, b.scan_tmp, stream);
   scan_u4_exclusive(b.block_cc, b.block_ccpre, b.splat_blocks, b.scan_tmp, stream);
   hipLaunchKernelGGL(k_totals, dim3(1), dim3(1), 0, stream, b);
