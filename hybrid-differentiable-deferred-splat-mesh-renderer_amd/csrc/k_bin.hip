@@ -979,6 +979,44 @@ __global__ void k_classify_write(Bufs b, const uint4* pre) {
   write_class(b, t, m, s, c, k, p);
 }
 
+// Classification in two launches (total_tiles <= kTwoPassTiles): every 256-tile workgroup sums its
+// tiles' class vectors, then each workgroup adds the sums of the workgroups before it (at most 256,
+// one per thread) to its own block scan and writes its tiles as k_classify_write does.
+constexpr uint32_t kTwoPassTiles = 256u * 256u;
+__global__ __launch_bounds__(256) void k_classify_part(Bufs b, uint4* part) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint4 k = make_uint4(0, 0, 0, 0);
+  if (t < b.total_tiles) {
+    uint32_t m, s, c;
+    tile_info(b, t, &m, &s, &c);
+    k = class_of(c);
+  }
+  uint4 tot;
+  (void)block_excl_256<uint4>(k, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_classify_write2(Bufs b, const uint4* part) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const bool in = t < b.total_tiles;
+  uint32_t m = 0, s = 0, c = 0;
+  if (in) tile_info(b, t, &m, &s, &c);
+  const uint4 before = threadIdx.x < blockIdx.x ? part[threadIdx.x] : make_uint4(0, 0, 0, 0);
+  uint4 base, tot;
+  (void)block_excl_256<uint4>(before, &base);  // the earlier workgroups' sum
+  const uint4 k = in ? class_of(c) : make_uint4(0, 0, 0, 0);
+  const uint4 p = base + block_excl_256<uint4>(k, &tot);
+  if (!in) return;
+  if (t == b.total_tiles - 1) {
+    b.stats->small_tiles = p.x + k.x;
+    b.stats->large_tiles = p.y + k.y;
+    b.stats->segments = p.z + k.z;
+    b.stats->chunks = p.z + k.z - (p.x + k.x);
+    b.stats->scratch_used = p.w + k.w;
+  }
+  write_class(b, t, m, s, c, k, p);
+}
+
 }  // namespace
 
 size_t scan_u32_tmp_words(uint32_t n) { return scan_tmp_elems<uint32_t>(n); }
@@ -1089,6 +1127,11 @@ void launch_classify(const Bufs& b, hipStream_t stream) {
   uint4* pre = cls + b.total_tiles;
   uint4* tmp = pre + b.total_tiles;
   const uint32_t grid = (b.total_tiles + 255) / 256;
+  if (b.total_tiles <= kTwoPassTiles) {  // the workgroup sums take the class vector's place
+    hipLaunchKernelGGL(k_classify_part, dim3(grid), dim3(256), 0, stream, b, cls);
+    hipLaunchKernelGGL(k_classify_write2, dim3(grid), dim3(256), 0, stream, b, (const uint4*)cls);
+    return;
+  }
   hipLaunchKernelGGL(k_classify_count, dim3(grid), dim3(256), 0, stream, b, cls);
   scan_u4_exclusive(cls, pre, b.total_tiles, tmp, stream);
   hipLaunchKernelGGL(k_classify_write, dim3(grid), dim3(256), 0, stream, b, (const uint4*)pre);
