@@ -168,3 +168,25 @@ def test_serving_scale_mixed_resolutions(hip_device):
         srv.release(t)
     assert batched > 1
     srv.close()
+
+
+def test_batch_past_two_launch_classification(hip_device):
+    """Nine 1920x1080 views of one scene in one batch hold 73,440 tiles, past the 65,536 that the
+    two-launch tile classification takes, so the batch runs the scanned classification: each view
+    equals its alone frame bit for bit."""
+    s = scenes.scene_config0(50_000, seed=303)
+    cams = [camera.Camera(camera.orbit_eye(3.0, 0.7 * i, 0.1), (0, 0, 0), 1500.0, 1500.0, 1920, 1080)
+            for i in range(9)]
+    ref = [alone(s, cam) for cam in cams]
+    srv = Server(max_views_per_batch=64, slots=1, use_graphs=False)
+    srv.add_scene(1, s)
+    srv.hold()
+    tickets = [srv.submit(1, cam) for cam in cams]
+    srv.drain()
+    views = [srv.result(t)["batch_views"] for t in tickets]
+    for t, (rgba, depth) in zip(tickets, ref):
+        np.testing.assert_array_equal(srv.read_rgba(t), rgba)
+        np.testing.assert_array_equal(srv.read_depth(t), depth)
+        srv.release(t)
+    srv.close()
+    assert max(views) == 9, views
