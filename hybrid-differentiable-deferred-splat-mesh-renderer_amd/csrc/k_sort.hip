@@ -374,11 +374,14 @@ __device__ __forceinline__ void sort_segment(const Bufs& b, const uint32_t* src,
 }
 
 // Small tiles and chunks: one workgroup per segment (grid-stride), kSortThreads threads.
-__global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b) {
+// kT threads per segment: 256 where many segments are sorted (several workgroups share a CU),
+// 1024 in round 2 after first chunks, whose few tens of chunks wait on the longest sort.
+template <uint32_t kT>
+__global__ __launch_bounds__(kT, kT == kSortThreads ? kSortMinWaves : 1) void k_segsort(Bufs b) {
   __shared__ uint32_t ka[HDSR_MAX_TILE_ENTRIES];
-  __shared__ uint32_t wc[kSortWaves][256];
-  __shared__ uint32_t wsum[4];
-  __shared__ uint32_t red[2][kSortWaves];
+  __shared__ uint32_t wc[kT / 64][256];
+  __shared__ uint32_t wsum[kT / 64];
+  __shared__ uint32_t red[2][kT / 64];
   // round 2 walks the chunks the first round listed (the later chunks of the tiles it left
   // unsaturated); the other rounds walk every segment
   const bool listed = b.round == 2;  // uniform
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(kSortThreads, kSortMinWaves) void k_segsort(Bufs b)
     const uint32_t* src = (S.large == kNone ? b.entries : b.scratch) + S.src;
     const uint32_t* qsrc = (S.large == kNone ? b.qkey : b.qkey_s) + S.src;
     if (S.count & kSegKeepOrder) continue;  // uniform: copied unsorted by k_segcopy
-    sort_segment<kSortThreads, 256>(b, src, qsrc, S.count, S.dst, ka, wc, wsum, red);
+    sort_segment<kT, 256>(b, src, qsrc, S.count, S.dst, ka, wc, wsum, red);
   }
 }
 
@@ -680,12 +683,9 @@ __global__ __launch_bounds__(256) void k_segcopy(Bufs b) {
 
 void launch_bucket(const Bufs& b, hipStream_t stream) {
   if (!b.total_tiles) return;
-  if (b.round == 2) {  // the few tiles the first round left unsaturated
+  if (b.round == 2) {  // the few tiles the first round left unsaturated, the longest ones: wide
     const uint32_t grid = b.total_tiles < 256u ? b.total_tiles : 256u;
-    if (b.bucket_wide)
-      hipLaunchKernelGGL(k_bucket<1024>, dim3(grid), dim3(1024), 0, stream, b);
-    else
-      hipLaunchKernelGGL(k_bucket<512>, dim3(grid), dim3(512), 0, stream, b);
+    hipLaunchKernelGGL(k_bucket<1024>, dim3(grid), dim3(1024), 0, stream, b);
     return;
   }
   if (b.bucket_wide) {
@@ -714,7 +714,10 @@ void launch_segment_sort(const Bufs& b, hipStream_t stream) {
   // thousands of workgroups that find nothing to do
   const uint32_t cap = b.round == 2 ? 1024u : 2048u * 1024u / kSortThreads;
   const uint32_t grid = b.seg_cap < cap ? b.seg_cap : cap;
-  hipLaunchKernelGGL(k_segsort, dim3(grid), dim3(kSortThreads), 0, stream, b);
+  if (b.round == 2 && b.first_chunks)  // the frame selected first chunks: few tiles resume
+    hipLaunchKernelGGL(k_segsort<1024>, dim3(grid), dim3(1024), 0, stream, b);
+  else
+    hipLaunchKernelGGL(k_segsort<kSortThreads>, dim3(grid), dim3(kSortThreads), 0, stream, b);
   if (b.round == 2)  // the resumed entries' quadrant masks
     hipLaunchKernelGGL(k_resume_masks, dim3(4096), dim3(256), 0, stream, b);
   if (b.ref_caps) hipLaunchKernelGGL(k_segcopy, dim3(grid), dim3(256), 0, stream, b);
