@@ -619,11 +619,93 @@ __device__ __forceinline__ void first_chunk_tile(const Bufs& b, uint32_t l, Firs
   }
 }
 
+// One wave sorts a small tile of at most kWaveSortMax entries in place (entries and keys at off)
+// by (q20, list index), as sort_segment does, with no workgroup barrier.  The keys ping-pong
+// between ka[0, 512) and ka[512, 1024), and each key's rank in its digit waits in ka[1024, 1536)
+// (few registers: the kernel's occupancy is set by its other items); each LSD pass ranks the keys
+// with ballots against the wave's own 128 digit counters, scans those across the wave and
+// scatters; ea holds the entries by list index.  A wave's LDS
+// operations execute in issue order, so a wave-scope fence between the phases (which keeps the
+// compiler from moving memory operations across it) is all the ordering they need.
+constexpr uint32_t kWaveSortMax = 512;
+__device__ __forceinline__ void wave_sort_small(const Bufs& b, uint32_t off, uint32_t cnt, uint32_t* ka,
+                                                uint32_t* ea, uint32_t* wc) {
+  const uint32_t lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t rounds = (cnt + 63u) / 64u;  // uniform, <= 8
+  uint32_t qmin = 0xFFFFFFFFu, qmax = 0u;
+  for (uint32_t i = lane; i < cnt; i += 64u) {
+    const uint32_t q = b.qkey[off + i];
+    ea[i] = b.entries[off + i];
+    ka[i] = q;
+    qmin = min(qmin, q);
+    qmax = max(qmax, q);
+  }
+  qmin = wave_min(qmin);
+  qmax = wave_max(qmax);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (uint32_t i = lane; i < cnt; i += 64u) ka[i] = ((ka[i] - qmin) << 12) | i;
+  const uint32_t span = qmax - qmin;  // uniform
+  const uint32_t qbits = span ? 32u - (uint32_t)__clz(span) : 0u;
+  const uint32_t np = (qbits + 6u) / 7u;
+  const uint32_t w0 = np ? max((qbits + np - 1u) / np, 4u) : 0u;
+  uint32_t* src = ka;
+  uint32_t* dst = ka + kWaveSortMax;
+  uint32_t* const rk = ka + 2u * kWaveSortMax;
+  uint32_t low = 0;
+  for (uint32_t ps = 0; ps < np; ++ps) {  // uniform
+    const uint32_t bits = ps + 1u < np ? w0 : max(qbits - low, 4u);
+    const uint32_t shift = 12u + low, mask = (1u << bits) - 1u;
+    wc[lane] = 0u;
+    wc[lane + 64u] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll 1
+    for (uint32_t r = 0; r < rounds; ++r) {
+      const bool valid = r * 64u + lane < cnt;
+      const uint32_t d = valid ? (src[r * 64u + lane] >> shift) & mask : 0u;  // < 128
+      const unsigned long long m = match_digit<7>(d, __ballot(valid));
+      const uint32_t old = group_count(&wc[d], m, valid, lane);
+      if (valid) rk[r * 64u + lane] = old + (uint32_t)__popcll(m & below);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // exclusive scan of the 128 digit counts: digit lane, then digit lane + 64
+    const uint32_t c0 = wc[lane], c1 = wc[lane + 64u];
+    uint32_t i0 = c0, i1 = c1;
+#pragma unroll
+    for (int dd = 1; dd < 64; dd <<= 1) {
+      const uint32_t x0 = __shfl_up(i0, dd, 64), x1 = __shfl_up(i1, dd, 64);
+      if (lane >= (uint32_t)dd) {
+        i0 += x0;
+        i1 += x1;
+      }
+    }
+    const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)i0, 63);
+    wc[lane] = i0 - c0;
+    wc[lane + 64u] = t0 + i1 - c1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t i = lane; i < cnt; i += 64u) {
+      const uint32_t key = src[i];
+      dst[wc[(key >> shift) & mask] + rk[i]] = key;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t* const tmp = src;
+    src = dst;
+    dst = tmp;
+    low += bits;
+  }
+  for (uint32_t i = lane; i < cnt; i += 64u) {
+    const uint32_t key = src[i];
+    b.entries[off + i] = ea[key & 0xFFFu];
+    if (b.keys20) b.keys20[off + i] = key + (qmin << 12);
+  }
+}
+
 // Round 1 of a deferred frame with first chunks: every large tile's first chunk (largest tiles
 // first, from large_order[]) and then every small tile's sort, one workgroup per item, in one
-// launch: the small tiles fill the CUs the last large tiles leave idle.
+// launch: the small tiles fill the CUs the last large tiles leave idle.  A small tile of at most
+// kWaveSortMax entries is sorted by the workgroup's first wave alone (wave_sort_small).
 template <uint32_t kT>
-__global__ __launch_bounds__(kT) void k_first_chunks(Bufs b) {
+__global__ __launch_bounds__(kT, 8) void k_first_chunks(Bufs b) {
   __shared__ FirstLds<kT> S_;
   const uint32_t nl = min(b.stats->large_tiles, b.first_tiles);
   const uint32_t n = nl + b.stats->small_tiles;
@@ -632,7 +714,13 @@ __global__ __launch_bounds__(kT) void k_first_chunks(Bufs b) {
       first_chunk_tile<kT>(b, __builtin_amdgcn_readfirstlane(b.large_order[k]), S_);
     } else {
       const uint32_t m = __builtin_amdgcn_readfirstlane(b.small_list[k - nl]);
-      const uint32_t off = b.tile_offsets[m], cnt = b.tile_counts[m];
+      const uint32_t off = __builtin_amdgcn_readfirstlane(b.tile_offsets[m]);
+      const uint32_t cnt = __builtin_amdgcn_readfirstlane(b.tile_counts[m]);
+      if (cnt <= kWaveSortMax) {  // uniform
+        if (threadIdx.x < 64) wave_sort_small(b, off, cnt, S_.ka, S_.ea, S_.wc[0]);
+        __syncthreads();  // the shared arrays are reused by the next item
+        continue;
+      }
       sort_segment<kT, kFirstRow>(b, b.entries + off, b.qkey + off, cnt, off, S_.ka, S_.wc, S_.wsum, S_.red);
     }
   }
