@@ -42,18 +42,18 @@ SINGLE_KERNEL_STAGES_FIRST = {"project": "k_project", "emit": "k_emit", "bucket"
 # x1.0); see profiles/pmc_traffic_1M_sh3_1080p.md.  k_composite is the sum of its two launches
 # (k_composite and k_composite_resume).  Valid for the kernels as profiled (frames that select
 # first chunks, the default here).
-PMC_TRAFFIC_BYTES = {"k_project": 317.2e6, "k_emit": 30.6e6, "k_first_chunks": 365.0e6, "k_composite": 181.9e6}
+PMC_TRAFFIC_BYTES = {"k_project": 317.2e6, "k_emit": 30.6e6, "k_first_chunks": 898.4e6, "k_composite": 181.7e6}
 # What bounds the two heaviest kernels instead of HBM: their SQ counters on the same workload
 # (profiles/sq_counters_1M_sh3_1080p.md).  A wave64 VALU instruction occupies its SIMD for 4 cycles,
 # so 0.25 instructions per SIMD per cycle is the issue ceiling; the LDS peak is 128 B per clock per
 # CU over 256 CUs at 2.4 GHz.
-COMPOSITE_SQ = {"valu_busy": 0.95, "valu_instr_per_simd_cycle": 0.238, "valu_issue_ceiling": 0.25,
-                "valu_frac": round(0.238 / 0.25, 3), "occupancy": 0.52, "lds_GBps": 26109.0,
-                "lds_peak_GBps": 78643.0, "lds_frac": round(26109.0 / 78643.0, 3),
+COMPOSITE_SQ = {"valu_busy": 0.99, "valu_instr_per_simd_cycle": 0.245, "valu_issue_ceiling": 0.25,
+                "valu_frac": round(0.245 / 0.25, 3), "occupancy": 0.53, "lds_GBps": 26942.0,
+                "lds_peak_GBps": 78643.0, "lds_frac": round(26942.0 / 78643.0, 3),
                 "source": "rocprofv3 SQ counters, round 1 of k_composite (profiles/sq_counters_1M_sh3_1080p.md)"}
-FIRST_CHUNKS_SQ = {"valu_busy": 0.63, "valu_instr_per_simd_cycle": 0.157, "valu_issue_ceiling": 0.25,
-                   "valu_frac": round(0.157 / 0.25, 3), "occupancy": 0.44, "lds_GBps": 4938.0,
-                   "lds_peak_GBps": 78643.0, "lds_frac": round(4938.0 / 78643.0, 3), "waiting": 0.14,
+FIRST_CHUNKS_SQ = {"valu_busy": 0.69, "valu_instr_per_simd_cycle": 0.173, "valu_issue_ceiling": 0.25,
+                   "valu_frac": round(0.173 / 0.25, 3), "occupancy": 0.86, "lds_GBps": 5148.0,
+                   "lds_peak_GBps": 78643.0, "lds_frac": round(5148.0 / 78643.0, 3), "waiting": 0.20,
                    "source": "rocprofv3 SQ counters of k_first_chunks (profiles/sq_counters_1M_sh3_1080p.md)"}
 KERNEL_SQ = {"k_composite": COMPOSITE_SQ, "k_first_chunks": FIRST_CHUNKS_SQ}
 # launches per frame of the single-kernel stages (the deferred rounds)
